@@ -1,0 +1,163 @@
+/*
+ * pty_hip.h -- C ABI of the MI355X (gfx950) ptychography hot path.
+ *
+ * The reference (PtyLab 0.3.1) has no FFI: its hot path is the Python engine
+ * API Engines.ePIE|mPIE|multiPIE|e3PIE(reconstruction, experimentalData,
+ * params, monitor).reconstruct() (Engines/ePIE.py:20-51, mPIE.py:29-194,
+ * multiPIE.py:29-123, e3PIE.py:29-190) and the propagator pair
+ * Operators.object2detector/detector2object (Operators/Operators.py:561-584).
+ * The entry points below are what a binding of that API needs; each comment
+ * names the reference interface it replaces.  All pointers are device
+ * pointers unless stated; arrays are C-order and contiguous; complex numbers
+ * are interleaved float pairs (complex64).  Every function returns 0 on
+ * success or a negative PTY_E* code, with a message in pty_last_error().
+ * Nothing here synchronises the host unless stated.
+ */
+#ifndef PTY_HIP_H
+#define PTY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTY_ABI_VERSION 1
+
+/* error codes */
+#define PTY_OK 0
+#define PTY_EINVAL (-22)      /* bad argument / unsupported shape           */
+#define PTY_ENOMEM (-12)      /* device allocation failed                   */
+#define PTY_EDEVICE (-5)      /* HIP runtime error (launch, copy, ...)      */
+#define PTY_ETIMEDOUT (-110)  /* a cluster barrier exceeded its spin limit  */
+
+/* engines (Engines/ePIE.py, mPIE.py, multiPIE.py, e3PIE.py) */
+#define PTY_EPIE 0
+#define PTY_MPIE 1
+#define PTY_MULTIPIE 2
+#define PTY_E3PIE 3
+
+/* propagators (Operators.py:1082-1103 dispatch keys) */
+#define PTY_FRAUNHOFER 0          /* fft2c with shifts, fftshiftSwitch=False  */
+#define PTY_FRAUNHOFER_SHIFTED 1  /* fft2 ortho, fftshiftSwitch=True          */
+#define PTY_ASP 2                 /* band-limited angular spectrum            */
+
+/*
+ * One reconstruction.  Shapes follow the reference's 6D layout
+ * (Reconstruction.py:356-413):
+ *   object  (nlambda, nosm, 1, nslice, No, No)       complex64
+ *   probe   (nlambda, 1, npsm, nslice, Np, Np)       complex64
+ *   ptychogram (F, Np, Np) float32, natural detector order
+ *   positions (F, 2) int32 [row, col] of the patch's upper-left pixel
+ *             (Reconstruction.positions, Reconstruction.py:675-710)
+ *   order   (n_iter, F) int32: position index visited at step j of iteration i
+ *             (BaseEngine.setPositionOrder, BaseEngine.py:637-665)
+ *   momentum_mask (n_iter, F) uint8: 1 where np.random.rand(1) > 0.95 fired
+ *             after step j (mPIE.py:170-172); ignored by ePIE/e3PIE
+ *   energy  (F) float32 energyAtPos (ExperimentalData.py:248)
+ *   tf      (Np, Np) complex64 transfer function in FFT frequency order
+ *             (ifftshift applied): the ASP detector TF (Operators.py:861-893)
+ *             or the e3PIE slice TF (e3PIE.py:61-69); NULL for Fraunhofer ePIE
+ *   err_at_pos (n_iter, F) float64 and err_iter (n_iter) float64 outputs
+ *             (getErrorMetrics, BaseEngine.py:819-841)
+ * Buffers and momenta (mPIE/multiPIE only) have the object/probe shapes.
+ */
+typedef struct pty_instance {
+  int32_t engine;
+  int32_t propagator;
+  int32_t nlambda, nosm, npsm, nslice;
+  int32_t Np, No, F;
+  int32_t n_iter;
+  float betaObject, betaProbe;
+  float alphaObject, alphaProbe;
+  float momFriction;   /* mPIE frictionM / multiPIE stepM   */
+  float momFeedback;   /* mPIE feedbackM / multiPIE betaM   */
+  float probeWeight;   /* mPIE probe-update weight (1)      */
+  float sliceBeta;     /* e3PIE inner-slice probe step (0.9, e3PIE.py:140) */
+  void* object;
+  void* probe;
+  void* objectBuffer;
+  void* objectMomentum;
+  void* probeBuffer;
+  void* probeMomentum;
+  const float* ptychogram;
+  const int32_t* positions;
+  const int32_t* order;
+  const uint8_t* momentum_mask;
+  const float* energy;
+  const void* tf;
+  double* err_at_pos;
+  double* err_iter;
+  int32_t workgroups;  /* workgroups cooperating on this instance; 0 = auto */
+  int32_t reserved;
+} pty_instance;
+
+typedef struct pty_plan pty_plan;
+
+/* ABI version of the loaded library (compare with PTY_ABI_VERSION). */
+int pty_abi_version(void);
+
+/* Message of the last failing call on this host thread. */
+const char* pty_last_error(void);
+
+/*
+ * Build a plan for `ninst` independent reconstructions run by one persistent
+ * launch per iteration.  Replaces the set-up half of engine.reconstruct():
+ * BaseEngine._prepareReconstruction / _checkGPU (BaseEngine.py:96-111,
+ * 613-635) and the engines' initializeReconstructionParams.  Validates every
+ * shape, allocates the per-instance workspace, permutes each ptychogram and
+ * transfer function into the kernel's bit-reversed layout, uploads the
+ * descriptors.  Synchronises the device once (set-up only).
+ */
+int pty_plan_create(pty_plan** plan, const pty_instance* insts, int32_t ninst, void* hip_stream);
+
+/*
+ * Run iterations [it_begin, it_end) of every instance, stream-ordered, no
+ * host synchronisation, no allocation.  Replaces the body of
+ * engine.reconstruct(): the per-position loop plus getErrorMetrics
+ * (ePIE.py:69-112, mPIE.py:111-180, multiPIE.py:74-114, e3PIE.py:89-180).
+ */
+int pty_plan_run(pty_plan* plan, int32_t it_begin, int32_t it_end, void* hip_stream);
+
+/*
+ * Run positions [j_begin, j_end) of iteration `it` (every instance, clamped to
+ * its frame count): the granularity of ePIE.reconstruct_stepwise, which
+ * yields after every scan position (ePIE.py:53-112).  The iteration's error
+ * is reduced when its last position has run.
+ */
+int pty_plan_run_positions(pty_plan* plan, int32_t it, int32_t j_begin, int32_t j_end, void* hip_stream);
+
+/*
+ * Read the plan's device-side failure flag; synchronises the stream.
+ * Returns PTY_ETIMEDOUT if a cluster barrier gave up.
+ */
+int pty_plan_check(pty_plan* plan, void* hip_stream);
+
+/* Number of workgroups the plan launches per iteration (host query). */
+int pty_plan_workgroups(const pty_plan* plan, int32_t* total);
+
+/* Release a plan (device-synchronising). */
+int pty_plan_destroy(pty_plan* plan);
+
+/*
+ * In-place propagation of `nfields` contiguous (Np, Np) complex64 fields:
+ * Operators.object2detector (inverse=0) / detector2object (inverse=1) for
+ * the Fraunhofer and ASP propagators (Operators.py:27-74, 155-230).  `tf`
+ * is the ASP transfer function in FFT frequency order (NULL for Fraunhofer).
+ */
+int pty_propagate(void* fields, int32_t nfields, int32_t Np, int32_t propagator,
+                  int32_t inverse, const void* tf, void* hip_stream);
+
+/*
+ * Plain batched 2D DFT of `nfields` (Np, Np) complex64 fields, in place:
+ * sign=-1 forward, +1 inverse, unnormalised (numpy.fft.fft2 / ifft2 * N^2).
+ * The primitive behind utils.fft2c/ifft2c (utils.py:12-47).
+ */
+int pty_fft2(void* fields, int32_t nfields, int32_t Np, int32_t sign, void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTY_HIP_H */

@@ -1,0 +1,479 @@
+"""NumPy restatement of the reference's per-position reconstruction loop.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``): imported by ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg, never by
+the product package.
+
+Every function cites the reference file:line it restates (paths relative to
+the PtyLab 0.3.1 source tree).  The restatement keeps the reference's NumPy
+operations and dtype flow (NEP 50 promotion, complex64 state, float64 error
+sums, complex128 slice transfer function in e3PIE) so that it reproduces the
+reference's regression goldens to rtol 1e-5; ``precision="c128"`` promotes the
+whole state to complex128 for the "error against a double-precision run"
+figure.
+
+Parity pins (what checks each part; the tests live in ``tests/``):
+
+* ``fft2c``/``ifft2c``/Fraunhofer: the reference golden
+  ``propagator_fraunhofer.npz``, the unitary round trip, a brute-force DFT.
+* ``aspw_transfer``/ASP: golden ``propagator_asp.npz``; z=0 identity; the
+  closed-form band limit; |H|=1 inside the band.
+* ePIE/mPIE engines (single and mixed state): goldens ``epie_fraunhofer``,
+  ``epie_asp``, ``epie_mixed_state``, ``mpie_single``, ``mpie_mixed_state``.
+* e3PIE slice chain: golden ``e3pie_multislice``.
+* multiPIE: no golden exists; pinned by the identity multiPIE == mPIE when
+  nlambda == nosm == 1 (the two probe updates then sum over the same axes).
+* ``positionOrder='random'``: no golden; pinned by the RNG-consumption identity
+  with ``np.random.shuffle`` (tested) -- otherwise "parity unpinned".
+* ``orthogonalize_modes``: pinned by orthogonality, descending eigenvalues and
+  span preservation (the reference's own invariants,
+  tests/Engines/test_orthogonalization.py).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+__all__ = [
+    "fft2c", "ifft2c", "circ", "aspw_transfer", "asp_transfer_stack",
+    "propagate", "pixel_positions", "Geometry", "initial_guess",
+    "energy_at_pos", "position_order", "orthogonalize_modes", "OracleRun",
+    "nrmse", "GEMMA",
+]
+
+GEMMA = 1e-10  # zero-division guard of intensityProjection, BaseEngine.py:877
+
+
+# ---------------------------------------------------------------------------
+# FFT primitives -- utils/utils.py:12-47
+# ---------------------------------------------------------------------------
+def fft2c(field, shifted=False):
+    """Unitary centred 2D FFT over the last two axes (utils.py:12-28).
+
+    ``shifted=True`` is the reference's ``fftshiftSwitch`` (no shifts)."""
+    if shifted:
+        return np.fft.fft2(field, norm="ortho")
+    ax = (-2, -1)
+    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(field, axes=ax), norm="ortho"), axes=ax)
+
+
+def ifft2c(field, shifted=False):
+    """Unitary centred inverse 2D FFT (utils.py:31-47)."""
+    if shifted:
+        return np.fft.ifft2(field, norm="ortho")
+    ax = (-2, -1)
+    return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(field, axes=ax), norm="ortho"), axes=ax)
+
+
+def circ(x, y, D):
+    """Binary disc of diameter D on a meshgrid (utils.py:50-59)."""
+    return (x ** 2 + y ** 2) < (D / 2) ** 2
+
+
+# ---------------------------------------------------------------------------
+# Angular spectrum transfer function -- Operators.py:587-703, 861-893
+# ---------------------------------------------------------------------------
+def aspw_transfer(z, wavelength, N, L, bandlimit=True):
+    """Band-limited ASP transfer function, float64/complex128, centred
+    frequency order (Operators.py:631-684, complexexp :687-703)."""
+    z = float(z)
+    wavelength = float(wavelength)
+    L = float(L)
+    N = int(N)
+    a_z = abs(z)
+    k = 2 * np.pi / wavelength
+    f = np.arange(-N / 2, N / 2) / L
+    Fx, Fy = np.meshgrid(f, f)
+    f_max = L / (wavelength * np.sqrt(L ** 2 + 4 * a_z ** 2))
+    W = np.array(circ(Fx, Fy, 2 * f_max))
+    arg = 1 - (Fx * wavelength) ** 2 - (Fy * wavelength) ** 2
+    keep = arg > 0
+    if not bandlimit:
+        keep = 0 * keep + 1
+    arg = np.clip(arg, 0, np.inf)
+    phase = k * a_z * np.sqrt(arg)
+    H = np.array(keep * (np.cos(phase) + 1j * np.sin(phase)))
+    if z < 0:
+        H = H.conj()
+    return H * W
+
+
+def asp_transfer_stack(nosm, npsm, Np, z, wavelength, Lp, nlambda=1):
+    """The (nlambda, nosm, npsm, 1, Np, Np) complex64 stack the ASP
+    propagator multiplies with (Operators.py:861-893)."""
+    if nlambda > 1:
+        raise ValueError("For multi-wavelength, polychromeASP needs to be used instead of ASP")
+    one = aspw_transfer(z, wavelength, Np, Lp)
+    stack = np.empty((nlambda, nosm, npsm, 1, Np, Np), dtype=np.complex64)
+    stack[...] = one.astype(np.complex64)
+    return stack
+
+
+def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None):
+    """object2detector / detector2object for the in-scope propagators
+    (Operators.py:27-74 Fraunhofer, :155-230 ASP, dispatch :561-584, :1082-1103)."""
+    kind = kind.lower()
+    if kind == "fraunhofer":
+        return ifft2c(fields, shifted) if inverse else fft2c(fields, shifted)
+    if kind == "asp":
+        if shifted:
+            raise ValueError("ASP propagator only works with fftshiftswitch == False")
+        key = (geo.nosm, geo.npsm, geo.Np, geo.zo, geo.wavelength, geo.Lp, geo.nlambda)
+        tf = None if tf_cache is None else tf_cache.get(key)
+        if tf is None:
+            tf = asp_transfer_stack(geo.nosm, geo.npsm, geo.Np, geo.zo, geo.wavelength, geo.Lp, geo.nlambda)
+            tf = np.fft.ifftshift(tf, axes=(-2, -1))
+            if tf_cache is not None:
+                tf_cache[key] = tf
+        if inverse:
+            tf = tf.conj()
+        if fields.dtype == np.complex128:
+            tf = tf.astype(np.complex128)
+        return ifft2c(fft2c(fields, True) * tf, True)
+    raise NotImplementedError(f"propagator {kind!r} is outside the oracle's scope")
+
+
+# ---------------------------------------------------------------------------
+# Geometry -- Reconstruction.py:21-31, 145-204, 579-710
+# ---------------------------------------------------------------------------
+def pixel_positions(encoder, dxo, No, Np):
+    """round(encoder/dxo) + No//2 - Np//2 as int (Reconstruction.py:21-31)."""
+    positions = np.round(encoder / dxo)
+    positions = positions + No // 2 - Np // 2
+    return positions.astype(int)
+
+
+@dataclass
+class Geometry:
+    """Derived CPM geometry (Reconstruction.py:145-204, 579-710)."""
+    ptychogram: np.ndarray
+    encoder: np.ndarray
+    wavelength: float
+    dxd: float
+    zo: float
+    entrancePupilDiameter: Optional[float] = None
+    No: Optional[int] = None
+    nlambda: int = 1
+    nosm: int = 1
+    npsm: int = 1
+    nslice: int = 1
+
+    def __post_init__(self):
+        self.Nd = self.ptychogram.shape[-1]
+        self.Ld = self.Nd * self.dxd
+        self.dxp = self.wavelength * self.zo / self.Ld
+        self.Np = self.Nd
+        self.Lp = self.Np * self.dxp
+        self.dxo = self.dxp
+        if self.entrancePupilDiameter is None:
+            self.entrancePupilDiameter = self.Lp / 3
+        if self.No is None:
+            # Reconstruction.py:193-204
+            provisional = pixel_positions(self.encoder, self.dxo, self.Np * 2 ** 2, self.Np)
+            rng_px = np.max(provisional, axis=0) - np.min(provisional, axis=0)
+            rng_px = np.max(rng_px) + self.Np * 2
+            if rng_px % 2 == 1:
+                rng_px += 1
+            self.No = np.max([self.Np, rng_px])
+        self.positions = pixel_positions(self.encoder, self.dxo, self.No, self.Np)
+        xp = np.linspace(-self.Np / 2, self.Np / 2, int(self.Np)) * self.dxp
+        self.Xp, self.Yp = np.meshgrid(xp, xp)
+
+    @property
+    def shape_O(self):
+        return (self.nlambda, self.nosm, 1, self.nslice, int(self.No), int(self.No))
+
+    @property
+    def shape_P(self):
+        return (self.nlambda, 1, self.npsm, self.nslice, int(self.Np), int(self.Np))
+
+
+def initial_guess(shape, kind, geo):
+    """'ones' / 'circ' initial guesses with 1e-3 uniform noise drawn from the
+    global NumPy RNG (initializationFunctions.py:10-62; cast to complex64 as in
+    Reconstruction.py:356-413)."""
+    if kind == "ones":
+        out = np.ones(shape) + 0.001 * np.random.rand(*shape)
+    elif kind == "circ":
+        pupil = circ(geo.Xp, geo.Yp, geo.entrancePupilDiameter)
+        out = np.ones(shape, dtype=np.complex64) + 0.001 * np.random.rand(*shape)
+        out *= pupil
+    else:
+        raise NotImplementedError(kind)
+    return out.astype(np.complex64)
+
+
+def energy_at_pos(ptychogram):
+    """Per-frame energy (ExperimentalData.py:248)."""
+    return np.sum(abs(ptychogram), (-1, -2))
+
+
+def position_order(kind, n_frames, n_errors, positions=None):
+    """setPositionOrder (BaseEngine.py:637-665); consumes the global RNG exactly
+    as the reference (one ``np.random.shuffle`` per iteration once two errors
+    exist)."""
+    if kind == "sequential":
+        return np.arange(n_frames)
+    if kind == "random":
+        idx = np.arange(n_frames)
+        if n_errors >= 2:
+            np.random.shuffle(idx)
+        return idx
+    if kind == "NA":
+        rows = positions[:, 0] - np.mean(positions[:, 0])
+        cols = positions[:, 1] - np.mean(positions[:, 1])
+        return np.argsort(np.sqrt(rows ** 2 + cols ** 2))
+    raise ValueError("position order not properly set")
+
+
+def orthogonalize_modes(p):
+    """'snapShots' mode orthogonalisation (utils.py:130-202): Gram matrix,
+    host eigh, descending order, modes = V^T p."""
+    p2 = p.reshape(p.shape[0], p.shape[1] * p.shape[2])
+    G = np.dot(p2.conj(), np.transpose(p2))
+    w, V = np.linalg.eigh(G)
+    order = np.argsort(w)[::-1]
+    s = np.sqrt(np.clip(w[order].real, 0.0, None))
+    ev = s ** 2 / np.sum(s ** 2)
+    V = np.ascontiguousarray(V[:, order].T)
+    modes = np.dot(V, p2).reshape(p.shape)
+    return modes, ev, V
+
+
+def nrmse(rec, true):
+    """min over complex gamma of ||gamma*rec - true|| / ||true|| (project metric)."""
+    rec = np.asarray(rec, dtype=np.complex128).ravel()
+    true = np.asarray(true, dtype=np.complex128).ravel()
+    den = np.vdot(rec, rec)
+    gamma = np.vdot(rec, true) / den if abs(den) > 0 else 0.0
+    return float(np.linalg.norm(gamma * rec - true) / np.linalg.norm(true))
+
+
+# ---------------------------------------------------------------------------
+# Engines -- ePIE.py:53-160, mPIE.py:73-270, multiPIE.py:46-205, e3PIE.py:46-225,
+# BaseEngine.py:819-1012
+# ---------------------------------------------------------------------------
+@dataclass
+class OracleRun:
+    """State and settings of one reconstruction, mirroring the reference's
+    Reconstruction/Engine attributes that the hot path reads and writes."""
+    geo: Geometry
+    object: np.ndarray
+    probe: np.ndarray
+    engine: str = "ePIE"
+    propagator: str = "Fraunhofer"
+    fftshiftSwitch: bool = False
+    positionOrder: str = "random"
+    numIterations: int = 50
+    betaObject: float = 0.25
+    betaProbe: float = 0.25
+    alphaObject: float = 0.1
+    alphaProbe: float = 0.1
+    feedbackM: float = 0.3   # mPIE feedback / multiPIE betaM
+    frictionM: float = 0.7   # mPIE friction / multiPIE stepM
+    dz: float = 0.0
+    refrIndex: float = 1.0
+    orthogonalizationSwitch: bool = False
+    orthogonalizationFrequency: int = 10
+    precision: str = "c64"
+    error: List[float] = field(default_factory=list)
+
+    def __post_init__(self):
+        g = self.geo
+        c = np.complex128 if self.precision == "c128" else np.complex64
+        self.cdtype = c
+        self.object = np.asarray(self.object).astype(c)
+        self.probe = np.asarray(self.probe).astype(c)
+        self.ptychogram = g.ptychogram.astype(np.float64) if self.precision == "c128" else g.ptychogram
+        self.energy = energy_at_pos(self.ptychogram)
+        self.err_at_pos_sum = np.zeros(self.ptychogram.shape[0])
+        self.errorAtPos = None
+        self._tf = {}
+        self.schedule = []  # (order, momentum triggers) per iteration, for inspection
+        if self.engine in ("mPIE", "multiPIE"):
+            # mPIE.py:87-94 / multiPIE.py:61-66
+            self.objectMomentum = np.zeros_like(self.object)
+            self.probeMomentum = np.zeros_like(self.probe)
+        if self.engine == "e3PIE":
+            # e3PIE.py:61-69
+            H = aspw_transfer(self.dz, g.wavelength / self.refrIndex, g.Np, g.Lp)
+            self.H = np.fft.ifftshift(H)
+
+    # -- shared pieces ------------------------------------------------------
+    def _fwd(self, x):
+        return propagate(x, self.propagator, self.geo, False, self.fftshiftSwitch, self._tf)
+
+    def _bwd(self, x):
+        return propagate(x, self.propagator, self.geo, True, self.fftshiftSwitch, self._tf)
+
+    def _intensity_projection(self, k, esw):
+        """BaseEngine.intensityProjection 'standard' branch (:870-1012) with
+        getRMSD (:843-868)."""
+        ESW = self._fwd(esw)
+        Iest = np.sum(np.abs(ESW) ** 2, axis=(0, 1, 2))[-1]
+        Imeas = self.ptychogram[k]
+        det_err = abs(Imeas - Iest)
+        self.err_at_pos_sum[k] = np.sum(np.abs(det_err.astype(np.float64)))
+        frac = np.sqrt(Imeas / (Iest + GEMMA))
+        ESW = ESW * frac
+        return self._bwd(ESW)
+
+    def _error_metrics(self):
+        """getErrorMetrics (BaseEngine.py:819-841)."""
+        self.errorAtPos = self.err_at_pos_sum / (self.energy + 1e-20)
+        self.error.append(float(np.sum(self.errorAtPos)))
+
+    def _apply_constraints(self, loop):
+        """applyConstraints (BaseEngine.py:1440-1612): only orthogonalization
+        (:1614-1722) is in scope; every other switch is off by default."""
+        if self.orthogonalizationSwitch and loop % self.orthogonalizationFrequency == 0:
+            g = self.geo
+            if g.npsm > 1:
+                for il in range(g.nlambda):
+                    for isl in range(g.nslice):
+                        modes, ev, V = orthogonalize_modes(self.probe[il, 0, :, isl])
+                        self.probe[il, 0, :, isl] = modes
+                        if self.engine in ("mPIE", "multiPIE"):
+                            for buf in (self.probeBuffer, self.probeMomentum):
+                                p = buf[il, 0, :, isl].reshape(g.npsm, g.Np ** 2)
+                                buf[il, 0, :, isl] = (V @ p).reshape(g.npsm, g.Np, g.Np)
+            elif g.nosm > 1:
+                for il in range(g.nlambda):
+                    for isl in range(g.nslice):
+                        modes, ev, V = orthogonalize_modes(self.object[il, :, 0, isl])
+                        self.object[il, :, 0, isl] = modes
+                        if self.engine in ("mPIE", "multiPIE"):
+                            for buf in (self.objectBuffer, self.objectMomentum):
+                                p = buf[il, :, 0, isl].reshape(g.nosm, g.No ** 2)
+                                buf[il, :, 0, isl] = (V @ p).reshape(g.nosm, g.No, g.No)
+
+    # -- engines ----------------------------------------------------------------
+    def reconstruct(self, positions_limit=None):
+        """Run ``numIterations`` iterations; ``positions_limit`` truncates each
+        iteration's position loop (used only to time a bounded CPU sample)."""
+        g = self.geo
+        if self.engine in ("mPIE", "multiPIE"):
+            # mPIE.py:104-105
+            self.objectBuffer = self.object.copy()
+            self.probeBuffer = self.probe.copy()
+        F = self.ptychogram.shape[0]
+        for loop in range(self.numIterations):
+            order = position_order(self.positionOrder, F, len(self.error), g.positions)
+            trig = []
+            for j, k in enumerate(order):
+                if positions_limit is not None and j >= positions_limit:
+                    break
+                if self.engine == "e3PIE":
+                    self._e3pie_step(k)
+                else:
+                    self._pie_step(k)
+                if self.engine in ("mPIE", "multiPIE"):
+                    t = bool(np.random.rand(1) > 0.95)
+                    trig.append(t)
+                    if t:
+                        self._momentum()
+            self.schedule.append((np.array(order), np.array(trig, dtype=bool)))
+            self._error_metrics()
+            self._apply_constraints(loop)
+        return self
+
+    def _pie_step(self, k):
+        g = self.geo
+        Np = int(g.Np)
+        row, col = g.positions[k]
+        sy = slice(row, row + Np)
+        sx = slice(col, col + Np)
+        patch = self.object[..., sy, sx].copy()
+        esw = patch * self.probe
+        esw_upd = self._intensity_projection(k, esw)
+        DELTA = esw_upd - esw
+        P = self.probe
+        if self.engine == "ePIE":
+            # ePIE.py:128-160
+            fracO = P.conj() / np.max(np.sum(np.abs(P) ** 2, axis=(0, 1, 2, 3)))
+            self.object[..., sy, sx] = patch + self.betaObject * np.sum(fracO * DELTA, axis=(0, 2, 3), keepdims=True)
+            fracP = patch.conj() / np.max(np.sum(np.abs(patch) ** 2, axis=(0, 1, 2, 3)))
+            self.probe = P + self.betaProbe * np.sum(fracP * DELTA, axis=(0, 1, 3), keepdims=True)
+        else:
+            # mPIE.py:226-270 / multiPIE.py:153-205 (CPM branch)
+            absP2 = np.abs(P) ** 2
+            Pmax = np.max(np.sum(absP2, axis=(0, 1, 2, 3)), axis=(-1, -2))
+            fracO = P.conj() / (self.alphaObject * Pmax + (1 - self.alphaObject) * absP2)
+            self.object[..., sy, sx] = patch + self.betaObject * np.sum(fracO * DELTA, axis=2, keepdims=True)
+            absO2 = np.abs(patch) ** 2
+            Omax = np.max(np.sum(absO2, axis=(0, 1, 2, 3)), axis=(-1, -2))
+            fracP = patch.conj() / (self.alphaProbe * Omax + (1 - self.alphaProbe) * absO2)
+            if self.engine == "mPIE":
+                weight = 1
+                self.probe = P + weight * self.betaProbe * np.sum(fracP * DELTA, axis=1, keepdims=True)
+            else:
+                self.probe = P + self.betaProbe * np.sum(fracP * DELTA, axis=(0, 1), keepdims=True)
+
+    def _momentum(self):
+        """objectMomentumUpdate/probeMomentumUpdate (mPIE.py:196-224,
+        multiPIE.py:125-151)."""
+        grad = self.objectBuffer - self.object
+        self.objectMomentum = grad + self.frictionM * self.objectMomentum
+        self.object = self.object - self.feedbackM * self.objectMomentum
+        self.objectBuffer = self.object.copy()
+        grad = self.probeBuffer - self.probe
+        self.probeMomentum = grad + self.frictionM * self.probeMomentum
+        self.probe = self.probe - self.feedbackM * self.probeMomentum
+        self.probeBuffer = self.probe.copy()
+
+    def _e3pie_step(self, k):
+        """e3PIE.py:89-173 plus its update rules :192-225."""
+        g = self.geo
+        Np = int(g.Np)
+        nS = g.nslice
+        row, col = g.positions[k]
+        sy = slice(row, row + Np)
+        sx = slice(col, col + Np)
+        if not hasattr(self, "esw"):
+            self.esw = self.probe.copy()  # e3PIE.py:77
+        H = self.H if self.precision == "c64" else self.H.astype(np.complex128)
+        patch = self.object[..., sy, sx].copy()
+        self.esw[:, :, :, 0] = patch[:, :, :, 0] * self.probe[:, :, :, 0]
+        for s in range(1, nS):
+            self.probe[:, :, :, s] = np.fft.ifft2(np.fft.fft2(self.esw[:, :, :, s - 1]) * H)
+            self.esw[:, :, :, s] = self.probe[:, :, :, s] * patch[:, :, :, s]
+        esw_upd = self._intensity_projection(k, self.esw)
+        DELTA = (esw_upd - self.esw)[:, :, :, -1]
+
+        def obj_upd(op, d, lp):
+            frac = lp.conj() / np.max(np.sum(np.abs(lp) ** 2, axis=(0, 1, 2)))
+            return op + self.betaObject * np.sum(frac * d, axis=(0, 2), keepdims=True)
+
+        def probe_upd(op, d, lp, beth):
+            frac = op.conj() / np.max(np.sum(np.abs(op) ** 2, axis=(0, 1, 2)))
+            return lp + beth * np.sum(frac * d, axis=(0, 1), keepdims=True)
+
+        for t in range(nS - 1):
+            s = nS - 1 - t
+            self.object[..., s, sy, sx] = obj_upd(patch[:, :, :, s], DELTA, self.probe[:, :, :, s])
+            self.probe[:, :, :, s] = probe_upd(patch[:, :, :, s], DELTA, self.probe[:, :, :, s], 0.9)
+            DELTA = np.fft.ifft2(np.fft.fft2(self.probe[:, :, :, s]) * H.conj()) - self.esw[:, :, :, s - 1]
+        self.object[..., 0, sy, sx] = obj_upd(patch[:, :, :, 0], DELTA, self.probe[:, :, :, 0])
+        self.probe[:, :, :, 0] = probe_upd(patch[:, :, :, 0], DELTA, self.probe[:, :, :, 0], self.betaProbe)
+
+
+PINNED = {
+    "fft2c": "golden propagator_fraunhofer + unitary round trip + brute-force DFT",
+    "ifft2c": "golden propagator_fraunhofer + round trip",
+    "aspw_transfer": "golden propagator_asp + z=0 identity + closed-form band limit",
+    "propagate": "goldens propagator_fraunhofer / propagator_asp",
+    "pixel_positions": "engine goldens (positions drive the patch indexing)",
+    "initial_guess": "engine goldens (seeded init is their input)",
+    "energy_at_pos": "engine goldens (error vectors)",
+    "position_order": "sequential: goldens; random: np.random.shuffle identity test",
+    "OracleRun.ePIE": "goldens epie_fraunhofer, epie_asp, epie_mixed_state",
+    "OracleRun.mPIE": "goldens mpie_single, mpie_mixed_state",
+    "OracleRun.multiPIE": "identity with mPIE at nlambda=nosm=1 (no golden exists)",
+    "OracleRun.e3PIE": "golden e3pie_multislice",
+    "orthogonalize_modes": "orthogonality + descending eigenvalues + span invariants",
+    "nrmse": "closed form: invariant to a global complex scale; zero at gamma*true",
+}

@@ -1,0 +1,325 @@
+"""Shared engine machinery (Engines/BaseEngine.py), MI355X edition.
+
+An engine keeps references to the reconstruction, data, params and monitor
+(BaseEngine.py:78-94).  ``reconstruct()`` does, in order:
+
+1. the host-side preparation of ``_prepareReconstruction`` that the hot path
+   depends on (fftshift of the data, error containers), and a check that no
+   switch outside the GPU path is on;
+2. the random schedule: the position order of every iteration
+   (setPositionOrder, BaseEngine.py:637-665) and, for mPIE/multiPIE, the
+   momentum triggers ``np.random.rand(1) > 0.95`` after every position
+   (mPIE.py:170-172).  Neither depends on the data, so they are drawn up
+   front from NumPy's global RNG in exactly the reference's order and the RNG
+   state afterwards is the reference's;
+3. upload of the state to torch tensors on the GPU, one ``pty_plan`` for the
+   reconstruction, one persistent launch per iteration, the per-iteration
+   constraints between launches;
+4. download: object, probe, momenta and errors are NumPy again
+   (ePIE.py:123-126).
+"""
+
+import logging
+
+import numpy as np
+
+from .. import _native
+from ..Operators import asp_tf_fftorder, aspw_transfer_function
+
+# switches whose non-default value changes the per-position math or the
+# per-iteration constraints; the GPU engines implement the defaults (and
+# orthogonalization), as the reference raises NotImplementedError for the
+# switches it does not implement (BaseEngine.py:862, :1491)
+_UNSUPPORTED_TRUE = [
+    "FourierMaskSwitch", "CPSCswitch", "backgroundModeSwitch", "adaptiveDenoisingSwitch",
+    "positionCorrectionSwitch", "objectTVregSwitch", "OPRP", "saveMemory", "l2reg",
+    "modulusEnforcedProbeSwitch", "probePowerCorrectionSwitch", "probeSpectralPowerCorrectionSwitch",
+    "comStabilizationSwitch", "PSDestimationSwitch", "probeBoundary", "absorbingProbeBoundary",
+    "probeSmoothenessSwitch", "objectSmoothenessSwitch", "absObjectSwitch", "absProbeSwitch",
+    "objectContrastSwitch", "couplingSwitch", "binaryProbeSwitch", "TV_autofocus",
+    "map_position_to_z_change", "weigh_probe_updates_by_intensity",
+]
+
+
+class BaseEngine:
+    ENGINE = None
+
+    def __init__(self, reconstruction, experimentalData, params, monitor):
+        self.betaObject = 0.25
+        self.reconstruction = reconstruction
+        self.experimentalData = experimentalData
+        self.params = params
+        self.monitor = monitor
+        self.monitor.reconstruction = reconstruction
+        # workgroups cooperating on one reconstruction (0: chosen by the
+        # library from Np); an MI355X-specific knob with no reference twin
+        self.workgroups = 0
+        self.logger = logging.getLogger(self.ENGINE or "BaseEngine")
+
+    # -- preparation -------------------------------------------------------
+    def _check_supported(self):
+        p = self.params
+        for name in _UNSUPPORTED_TRUE:
+            if bool(getattr(p, name, False)):
+                raise NotImplementedError(f"params.{name} is not implemented by the MI355X engines")
+        if p.intensityConstraint != "standard":
+            if p.intensityConstraint in ("fluctuation", "exponential", "poission", "interferometric"):
+                raise NotImplementedError(f"intensityConstraint {p.intensityConstraint!r} is not implemented")
+            raise ValueError("intensity constraint not properly specified!")
+        if p.positionOrder not in ("sequential", "random", "NA"):
+            raise ValueError("position order not properly set")
+        kind = p.propagatorType.lower()
+        if kind not in ("fraunhofer", "asp"):
+            raise NotImplementedError(f"propagatorType {p.propagatorType!r} is not implemented by the MI355X engines")
+        if p.propagatorType == "ASP" and p.fftshiftSwitch:
+            raise ValueError("ASP propagatorType works only with fftshiftSwitch = False")
+        if self.experimentalData.operationMode != "CPM":
+            raise NotImplementedError("only CPM is implemented")
+
+    def _checkFFT(self):
+        """Shift the data once for fftshiftSwitch (BaseEngine.py:405-455)."""
+        d = self.experimentalData
+        if self.params.fftshiftSwitch:
+            if not self.params.fftshiftFlag:
+                d.ptychogram = np.fft.ifftshift(d.ptychogram, axes=(-1, -2))
+                self.params.fftshiftFlag = 1
+        elif self.params.fftshiftFlag:
+            d.ptychogram = np.fft.fftshift(d.ptychogram, axes=(-1, -2))
+            self.params.fftshiftFlag = 0
+
+    def _initializeErrors(self):
+        r = self.reconstruction
+        if not hasattr(r, "errorAtPos"):
+            r.errorAtPos = np.zeros((self.experimentalData.numFrames, 1), dtype=np.float32)
+        if not hasattr(r, "error"):
+            r.error = []
+
+    def _prepareReconstruction(self):
+        self._check_supported()
+        if not self.params.gpuSwitch:
+            self.logger.warning("params.gpuSwitch is False: the MI355X engines run on the GPU regardless")
+        self._checkFFT()
+        self._initializeErrors()
+        self.monitor.initializeMonitors()
+
+    # -- schedule -----------------------------------------------------------
+    def _position_order(self, n_errors):
+        F = self.experimentalData.numFrames
+        order = self.params.positionOrder
+        if order == "sequential":
+            return np.arange(F)
+        if order == "random":
+            idx = np.arange(F)
+            if n_errors >= 2:
+                np.random.shuffle(idx)
+            return idx
+        pos = self.reconstruction.positions
+        rows = pos[:, 0] - np.mean(pos[:, 0])
+        cols = pos[:, 1] - np.mean(pos[:, 1])
+        return np.argsort(np.sqrt(rows ** 2 + cols ** 2))
+
+    def _schedule(self, n_iter, momentum):
+        F = self.experimentalData.numFrames
+        n_err = len(self.reconstruction.error)
+        orders = np.empty((n_iter, F), dtype=np.int32)
+        masks = np.zeros((n_iter, F), dtype=np.uint8)
+        for it in range(n_iter):
+            orders[it] = self._position_order(n_err + it)
+            if momentum:
+                masks[it] = np.random.rand(F) > 0.95
+        return orders, masks
+
+    # -- engine hooks ---------------------------------------------------------
+    def _engine_coefficients(self):
+        return dict(betaObject=self.betaObject, betaProbe=self.betaProbe)
+
+    def _transfer_function(self):
+        r = self.reconstruction
+        if self.params.propagatorType.lower() == "asp":
+            if r.nlambda > 1:
+                raise ValueError("For multi-wavelength, set polychromeASP instead of ASP")
+            return asp_tf_fftorder(r.zo, r.wavelength, r.Np, r.Lp)
+        return None
+
+    def _propagator_id(self):
+        if self.params.propagatorType.lower() == "asp":
+            return _native.PTY_ASP
+        return _native.PTY_FRAUNHOFER_SHIFTED if self.params.fftshiftSwitch else _native.PTY_FRAUNHOFER
+
+    # -- device side ------------------------------------------------------------
+    def _upload(self, n_iter, momentum):
+        import torch
+
+        dev = torch.device("cuda")
+        r, d = self.reconstruction, self.experimentalData
+        T = {}
+
+        def c64(a):
+            return torch.as_tensor(np.ascontiguousarray(np.asarray(a), dtype=np.complex64), device=dev).contiguous()
+
+        T["object"] = c64(r.object)
+        T["probe"] = c64(r.probe)
+        if tuple(T["object"].shape) != tuple(r.shape_O):
+            raise ValueError(f"object shape {tuple(T['object'].shape)} != {r.shape_O}")
+        if tuple(T["probe"].shape) != tuple(r.shape_P):
+            raise ValueError(f"probe shape {tuple(T['probe'].shape)} != {r.shape_P}")
+        if momentum:
+            T["objectBuffer"] = T["object"].clone()
+            T["probeBuffer"] = T["probe"].clone()
+            T["objectMomentum"] = c64(getattr(r, "objectMomentum", np.zeros(r.shape_O, np.complex64)))
+            T["probeMomentum"] = c64(getattr(r, "probeMomentum", np.zeros(r.shape_P, np.complex64)))
+        T["ptychogram"] = torch.as_tensor(np.ascontiguousarray(d.ptychogram, dtype=np.float32), device=dev)
+        T["positions"] = torch.as_tensor(np.ascontiguousarray(r.positions, dtype=np.int32), device=dev)
+        T["energy"] = torch.as_tensor(np.ascontiguousarray(np.asarray(d.energyAtPos, dtype=np.float32)), device=dev)
+        orders, masks = self._schedule(n_iter, momentum)
+        self._orders, self._masks = orders, masks
+        T["order"] = torch.as_tensor(orders, device=dev)
+        T["mask"] = torch.as_tensor(masks, device=dev)
+        F = d.numFrames
+        T["err_at_pos"] = torch.zeros((n_iter, F), dtype=torch.float64, device=dev)
+        T["err_iter"] = torch.zeros((n_iter,), dtype=torch.float64, device=dev)
+        tf = self._transfer_function()
+        if tf is not None:
+            T["tf"] = torch.as_tensor(np.ascontiguousarray(tf, dtype=np.complex64), device=dev)
+        return T
+
+    def _instance(self, T, n_iter, momentum):
+        r = self.reconstruction
+        inst = _native.PtyInstance()
+        inst.engine = _native.ENGINE_IDS[self.ENGINE]
+        inst.propagator = self._propagator_id()
+        inst.nlambda, inst.nosm, inst.npsm, inst.nslice = int(r.nlambda), int(r.nosm), int(r.npsm), int(r.nslice)
+        inst.Np, inst.No, inst.F = int(r.Np), int(r.No), int(self.experimentalData.numFrames)
+        inst.n_iter = int(n_iter)
+        co = self._engine_coefficients()
+        inst.betaObject = float(co.get("betaObject", 0.25))
+        inst.betaProbe = float(co.get("betaProbe", 0.25))
+        inst.alphaObject = float(co.get("alphaObject", 0.1))
+        inst.alphaProbe = float(co.get("alphaProbe", 0.1))
+        inst.momFriction = float(co.get("momFriction", 0.7))
+        inst.momFeedback = float(co.get("momFeedback", 0.3))
+        inst.probeWeight = float(co.get("probeWeight", 1.0))
+        inst.sliceBeta = float(co.get("sliceBeta", 0.9))
+        ptr = lambda k: T[k].data_ptr() if k in T else None  # noqa: E731
+        inst.object, inst.probe = ptr("object"), ptr("probe")
+        inst.objectBuffer, inst.objectMomentum = ptr("objectBuffer"), ptr("objectMomentum")
+        inst.probeBuffer, inst.probeMomentum = ptr("probeBuffer"), ptr("probeMomentum")
+        inst.ptychogram, inst.positions = ptr("ptychogram"), ptr("positions")
+        inst.order = ptr("order")
+        inst.momentum_mask = ptr("mask") if momentum else None
+        inst.energy, inst.tf = ptr("energy"), ptr("tf")
+        inst.err_at_pos, inst.err_iter = ptr("err_at_pos"), ptr("err_iter")
+        inst.workgroups = int(self.workgroups)
+        return inst
+
+    def _momentum_enabled(self):
+        return False
+
+    def _begin(self):
+        """Upload, schedule, plan.  Returns (plan, tensors)."""
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("the MI355X engines need a HIP GPU (torch.cuda.is_available() is False)")
+        n_iter = int(self.numIterations)
+        momentum = self._momentum_enabled()
+        T = self._upload(n_iter, momentum)
+        inst = self._instance(T, n_iter, momentum)
+        stream = torch.cuda.current_stream()
+        plan = _native.Plan([inst], T, stream)
+        self._T, self._plan = T, plan
+        r = self.reconstruction
+        # expose the live device state like the reference's GPU path does
+        r.object, r.probe = T["object"], T["probe"]
+        if momentum:
+            r.objectBuffer, r.probeBuffer = T["objectBuffer"], T["probeBuffer"]
+            r.objectMomentum, r.probeMomentum = T["objectMomentum"], T["probeMomentum"]
+        return plan, T
+
+    def _end(self, n_done):
+        """Check the device flag, then errors and state back to NumPy."""
+        plan, T = self._plan, self._T
+        plan.check()
+        r = self.reconstruction
+        errs = T["err_iter"][:n_done].cpu().numpy()
+        r.error = np.append(np.asarray(r.error, dtype=np.float64), errs)
+        if n_done:
+            r.errorAtPos = T["err_at_pos"][n_done - 1].cpu().numpy()
+        r._move_data_to_cpu()
+        for k in ("object", "probe", "objectBuffer", "probeBuffer", "objectMomentum", "probeMomentum"):
+            if k in T:
+                setattr(r, k, T[k].cpu().numpy())
+        plan.close()
+        self._plan = None
+        self._T = None
+
+    # -- per-iteration constraints (BaseEngine.applyConstraints :1440-1612) -----
+    def applyConstraints(self, loop):
+        if self.params.orthogonalizationSwitch and loop % self.params.orthogonalizationFrequency == 0:
+            self.orthogonalization()
+
+    def orthogonalization(self):
+        """'snapShots' orthogonalisation of the probe (npsm > 1) or object
+        modes (BaseEngine.py:1614-1722, utils.py:130-202): the Gram matrix is a
+        complex64 GEMM on the GPU, its eigendecomposition runs on the host as
+        in the reference, the mixing is a GEMM on the GPU; buffer and momentum
+        follow the same mixing when momentum is on."""
+        import torch
+
+        r = self.reconstruction
+        T = self._T
+        if r.npsm > 1:
+            key, n, npx, sel = "probe", r.npsm, r.Np, lambda X, il, isl: X[il, 0, :, isl]
+            extra = ("probeBuffer", "probeMomentum")
+        elif r.nosm > 1:
+            key, n, npx, sel = "object", r.nosm, r.No, lambda X, il, isl: X[il, :, 0, isl]
+            extra = ("objectBuffer", "objectMomentum")
+        else:
+            return
+        X = T[key]
+        for il in range(r.nlambda):
+            for isl in range(r.nslice):
+                p2 = sel(X, il, isl).reshape(n, npx * npx)
+                G = torch.matmul(p2.conj(), p2.transpose(0, 1).contiguous())
+                w, V = np.linalg.eigh(G.cpu().numpy())
+                order = np.argsort(w)[::-1]
+                s = np.sqrt(np.clip(w[order].real, 0.0, None))
+                ev = s ** 2 / np.sum(s ** 2)
+                Vt = torch.as_tensor(np.ascontiguousarray(V[:, order].T), device=X.device)
+                sel(X, il, isl).copy_(torch.matmul(Vt, p2).reshape(n, npx, npx))
+                if key == "probe":
+                    r.purityProbe = float(np.sqrt(np.sum(ev ** 2)))
+                    r.purityProbeHist.append(r.purityProbe)
+                else:
+                    r.purityObject = float(np.sqrt(np.sum(ev ** 2)))
+                if self._momentum_enabled():
+                    for k in extra:
+                        q = sel(T[k], il, isl).reshape(n, npx * npx)
+                        sel(T[k], il, isl).copy_(torch.matmul(Vt, q).reshape(n, npx, npx))
+
+    # -- public API ---------------------------------------------------------------
+    def reconstruct(self, experimentalData=None, reconstruction=None):
+        if experimentalData is not None:
+            self.experimentalData = experimentalData
+            self.reconstruction.data = experimentalData
+        if reconstruction is not None:
+            self.reconstruction = reconstruction
+        self._prepareReconstruction()
+        plan, _ = self._begin()
+        n_iter = int(self.numIterations)
+        done = 0
+        try:
+            orth = self.params.orthogonalizationSwitch
+            if not orth:
+                # no host work between iterations: one launch per iteration,
+                # no synchronisation until the end
+                for loop in range(n_iter):
+                    plan.run(loop, loop + 1)
+                    done = loop + 1
+            else:
+                for loop in range(n_iter):
+                    plan.run(loop, loop + 1)
+                    done = loop + 1
+                    self.applyConstraints(loop)
+        finally:
+            self._end(done)

@@ -1,0 +1,203 @@
+"""ctypes binding of ``libptyhip.so`` (declarations in include/pty_hip.h).
+
+The library is built in-tree (``ptylab_amd/libptyhip.so``) by
+``ptylab_amd.build.build()``.  There is no fallback: if the library is
+missing or a GPU is absent every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libptyhip.so"
+
+PTY_ABI_VERSION = 1
+PTY_EPIE, PTY_MPIE, PTY_MULTIPIE, PTY_E3PIE = 0, 1, 2, 3
+PTY_FRAUNHOFER, PTY_FRAUNHOFER_SHIFTED, PTY_ASP = 0, 1, 2
+PTY_ETIMEDOUT = -110
+
+ENGINE_IDS = {"ePIE": PTY_EPIE, "mPIE": PTY_MPIE, "multiPIE": PTY_MULTIPIE, "e3PIE": PTY_E3PIE}
+
+EXPORTED = (
+    "pty_abi_version",
+    "pty_last_error",
+    "pty_plan_create",
+    "pty_plan_run",
+    "pty_plan_run_positions",
+    "pty_plan_check",
+    "pty_plan_workgroups",
+    "pty_plan_destroy",
+    "pty_propagate",
+    "pty_fft2",
+)
+
+
+class PtyInstance(ctypes.Structure):
+    """Mirror of ``pty_instance`` (include/pty_hip.h)."""
+
+    _fields_ = [
+        ("engine", ctypes.c_int32),
+        ("propagator", ctypes.c_int32),
+        ("nlambda", ctypes.c_int32),
+        ("nosm", ctypes.c_int32),
+        ("npsm", ctypes.c_int32),
+        ("nslice", ctypes.c_int32),
+        ("Np", ctypes.c_int32),
+        ("No", ctypes.c_int32),
+        ("F", ctypes.c_int32),
+        ("n_iter", ctypes.c_int32),
+        ("betaObject", ctypes.c_float),
+        ("betaProbe", ctypes.c_float),
+        ("alphaObject", ctypes.c_float),
+        ("alphaProbe", ctypes.c_float),
+        ("momFriction", ctypes.c_float),
+        ("momFeedback", ctypes.c_float),
+        ("probeWeight", ctypes.c_float),
+        ("sliceBeta", ctypes.c_float),
+        ("object", ctypes.c_void_p),
+        ("probe", ctypes.c_void_p),
+        ("objectBuffer", ctypes.c_void_p),
+        ("objectMomentum", ctypes.c_void_p),
+        ("probeBuffer", ctypes.c_void_p),
+        ("probeMomentum", ctypes.c_void_p),
+        ("ptychogram", ctypes.c_void_p),
+        ("positions", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("momentum_mask", ctypes.c_void_p),
+        ("energy", ctypes.c_void_p),
+        ("tf", ctypes.c_void_p),
+        ("err_at_pos", ctypes.c_void_p),
+        ("err_iter", ctypes.c_void_p),
+        ("workgroups", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call returned a nonzero code; the message is pty_last_error()."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: os.PathLike | None = None):
+    """Load and type the shared library (idempotent)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build it with `python -c 'import ptylab_amd.build as b; b.build()'`"
+        )
+    lib = ctypes.CDLL(str(p))
+    for name in EXPORTED:
+        getattr(lib, name)  # raises AttributeError if absent
+    vp, i32 = ctypes.c_void_p, ctypes.c_int32
+    lib.pty_abi_version.restype = ctypes.c_int
+    lib.pty_last_error.restype = ctypes.c_char_p
+    lib.pty_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(PtyInstance), i32, vp]
+    lib.pty_plan_run.argtypes = [vp, i32, i32, vp]
+    lib.pty_plan_run_positions.argtypes = [vp, i32, i32, i32, vp]
+    lib.pty_plan_check.argtypes = [vp, vp]
+    lib.pty_plan_workgroups.argtypes = [vp, ctypes.POINTER(i32)]
+    lib.pty_plan_destroy.argtypes = [vp]
+    lib.pty_propagate.argtypes = [vp, i32, i32, i32, i32, vp, vp]
+    lib.pty_fft2.argtypes = [vp, i32, i32, i32, vp]
+    for name in EXPORTED[2:]:
+        getattr(lib, name).restype = ctypes.c_int
+    if lib.pty_abi_version() != PTY_ABI_VERSION:
+        raise RuntimeError(f"libptyhip ABI {lib.pty_abi_version()} != {PTY_ABI_VERSION}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        msg = load().pty_last_error()
+        raise NativeError(code, msg.decode() if msg else "unknown error")
+
+
+def _stream_handle(stream):
+    return ctypes.c_void_p(stream.cuda_stream if stream is not None else 0)
+
+
+def _require_cuda(t):
+    import torch
+
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("expected a CUDA (HIP) torch tensor")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+
+
+def propagate(fields, propagator: int, inverse: bool, tf=None, stream=None):
+    """In-place pty_propagate on a (..., Np, Np) complex64 CUDA tensor."""
+    import torch
+
+    _require_cuda(fields)
+    if fields.dtype != torch.complex64:
+        raise TypeError("fields must be complex64")
+    Np = fields.shape[-1]
+    nf = fields.numel() // (Np * Np)
+    tfp = None
+    if tf is not None:
+        _require_cuda(tf)
+        tfp = tf.data_ptr()
+    check(load().pty_propagate(fields.data_ptr(), nf, Np, propagator, int(bool(inverse)), tfp, _stream_handle(stream)))
+    return fields
+
+
+def fft2(fields, sign: int, stream=None):
+    """In-place unnormalised batched 2D DFT (pty_fft2)."""
+    _require_cuda(fields)
+    Np = fields.shape[-1]
+    nf = fields.numel() // (Np * Np)
+    check(load().pty_fft2(fields.data_ptr(), nf, Np, int(sign), _stream_handle(stream)))
+    return fields
+
+
+class Plan:
+    """RAII wrapper of a pty_plan; keeps the tensors it points to alive."""
+
+    def __init__(self, instances, keepalive, stream=None):
+        self._lib = load()
+        self.stream = stream
+        arr = (PtyInstance * len(instances))(*instances)
+        self._instances = arr
+        self._keep = keepalive
+        h = ctypes.c_void_p()
+        check(self._lib.pty_plan_create(ctypes.byref(h), arr, len(instances), _stream_handle(stream)))
+        self._h = h
+
+    def run(self, it_begin, it_end):
+        check(self._lib.pty_plan_run(self._h, int(it_begin), int(it_end), _stream_handle(self.stream)))
+
+    def run_positions(self, it, j_begin, j_end):
+        check(self._lib.pty_plan_run_positions(self._h, int(it), int(j_begin), int(j_end), _stream_handle(self.stream)))
+
+    def check(self):
+        check(self._lib.pty_plan_check(self._h, _stream_handle(self.stream)))
+
+    def workgroups(self):
+        n = ctypes.c_int32()
+        check(self._lib.pty_plan_workgroups(self._h, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pty_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

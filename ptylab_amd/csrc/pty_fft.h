@@ -1,0 +1,108 @@
+// Wavefront-resident FFT building blocks for gfx950 (wave64).
+//
+// A length-N line (N = L * LPF, LPF = min(N, 64) lanes per line, L = N / LPF
+// complex values per lane) lives in registers: element n = l * LPF + t sits in
+// register l of lane t of its lane group.  Butterflies whose span is a
+// multiple of LPF are done inside a lane; smaller spans exchange values
+// between lanes with one ds_bpermute per float (__shfl_xor).  No LDS and no
+// workgroup barrier is needed inside a transform.
+//
+// fft_fwd: radix-2 decimation in frequency, natural order in, bit-reversed
+//          order out, kernel exp(-2 pi i nk / N), unnormalised.
+// fft_inv: radix-2 decimation in time, bit-reversed order in, natural order
+//          out, kernel exp(+2 pi i nk / N), unnormalised.
+// Running fft_inv(fft_fwd(x)) gives N * x, so a forward transform whose
+// output is consumed in bit-reversed order never needs a permutation.
+//
+// `tw` holds exp(-2 pi i j / N), j < N / 2, computed in double on the host
+// and rounded to float.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pty {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+
+__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
+  return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
+}
+
+template <int N>
+struct Line {
+  static constexpr int LPF = N >= 64 ? 64 : N;  // lanes per line
+  static constexpr int L = N / LPF;             // values per lane
+  static constexpr int SUB = 64 / LPF;          // lines per wave
+  static constexpr int LOG2 = N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6 : N == 128 ? 7 : N == 256 ? 8 : 9;
+};
+
+template <int N>
+__device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
+  constexpr int L = Line<N>::L;
+  constexpr int LPF = Line<N>::LPF;
+#pragma unroll
+  for (int sr = L / 2; sr >= 1; sr >>= 1) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      if ((l & sr) == 0) {
+        const int i = (l & (sr - 1)) * LPF + t;
+        const float2 w = tw[i * (L / (2 * sr))];
+        const float2 a = v[l], b = v[l + sr];
+        v[l] = cadd(a, b);
+        v[l + sr] = cmul(csub(a, b), w);
+      }
+    }
+  }
+#pragma unroll
+  for (int s = LPF / 2; s >= 1; s >>= 1) {
+    const bool up = (t & s) != 0;
+    const float2 w = tw[(t & (s - 1)) * (N / (2 * s))];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const float2 x = v[l];
+      const float2 p = shfl_xor2(x, s);
+      v[l] = up ? cmul(csub(p, x), w) : cadd(x, p);
+    }
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
+  constexpr int L = Line<N>::L;
+  constexpr int LPF = Line<N>::LPF;
+#pragma unroll
+  for (int s = 1; s < LPF; s <<= 1) {
+    const bool up = (t & s) != 0;
+    const float2 w = conjf2(tw[(t & (s - 1)) * (N / (2 * s))]);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const float2 x = up ? cmul(v[l], w) : v[l];
+      const float2 p = shfl_xor2(x, s);
+      v[l] = up ? csub(p, x) : cadd(x, p);
+    }
+  }
+#pragma unroll
+  for (int sr = 1; sr < L; sr <<= 1) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      if ((l & sr) == 0) {
+        const int i = (l & (sr - 1)) * LPF + t;
+        const float2 w = conjf2(tw[i * (L / (2 * sr))]);
+        const float2 a = v[l];
+        const float2 b = cmul(v[l + sr], w);
+        v[l] = cadd(a, b);
+        v[l + sr] = csub(a, b);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int brev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }
+
+}  // namespace pty

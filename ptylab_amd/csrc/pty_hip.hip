@@ -1,0 +1,1552 @@
+// MI355X (gfx950) kernels and C ABI for the per-position ptychography loop.
+//
+// One persistent launch runs whole iterations of one or many independent
+// reconstructions ("instances").  Instance i owns G_i workgroups (a cluster);
+// positions of one instance are strictly sequential (position k+1 gathers the
+// object that position k wrote, ePIE.py:84-105), so the parallelism is the
+// pixels x modes x slices of one position and the instances of a batch.
+//
+// Per position the cluster alternates row passes and column passes over the
+// (fields, Np, Np) exit-wave workspace W; a wave owns one line (a row or a
+// column) of every field at a time and keeps all fields of that line in
+// registers, so the mode sums of the intensity constraint and of the
+// object/probe updates are register-local.  Cluster-wide ordering between
+// passes is an agent-scope release/acquire barrier on a per-instance
+// counter; with G == 1 it is a plain workgroup barrier.
+//
+// Layout of W between passes: after a forward row transform the x axis is in
+// bit-reversed order, after a forward column transform the y axis too; the
+// inverse transforms consume bit-reversed input, so no permutation pass
+// exists anywhere.  The measured intensities and transfer functions are
+// permuted once at plan creation to match (transposed, bit-reversed) so the
+// column pass reads them coalesced.
+//
+// Numerics follow the reference's complex64 CPU path operation by operation
+// (BaseEngine.py:870-1012 projection, ePIE.py:128-160, mPIE.py:196-270,
+// multiPIE.py:125-205, e3PIE.py:89-225); the per-position error is summed in
+// FP64 and divided by the FP32 frame energy as getErrorMetrics does
+// (BaseEngine.py:819-841).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "pty_fft.h"
+#include "pty_hip.h"
+
+using namespace pty;
+
+#define NTHREADS 256
+#define NWAVE (NTHREADS / 64)
+#define MAX_SLICES 15
+#define SPIN_LIMIT (1u << 22)
+
+namespace {
+
+thread_local std::string g_err;
+int set_err(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int set_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err(PTY_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device-side instance descriptor
+// ---------------------------------------------------------------------------
+struct DevInst {
+  pty_instance a;
+  int G;             // workgroups of this instance
+  int nF;            // fields in W
+  int fs;            // slice stride of W fields (last-slice fields: f % fs == fs-1)
+  int nO, nPr;       // object / probe fields
+  float omAlphaO;    // (float)(1 - alphaObject), rounded as NumPy does
+  float omAlphaP;
+  float2* W;         // (nF, Np, Np)
+  float2* ESW;       // e3PIE exit waves (nslice, nF, Np, Np)
+  const float* Ik;   // ptychogram in kernel layout
+  const float2* HT;  // transfer function, transposed bit-reversed, / Np^2
+  unsigned* ctrl;    // [0] barrier counter, [1] failure flag, [16 + 32*slot + q] maxima
+  double* errpart;   // (2, F, G) per-position per-workgroup error partials
+};
+
+__device__ float2 g_twiddle[6][256];  // exp(-2 pi i j / N), j < N/2, N = 16..512
+
+__host__ __device__ static int log2_np(int np) {
+  switch (np) {
+    case 16: return 4;
+    case 32: return 5;
+    case 64: return 6;
+    case 128: return 7;
+    case 256: return 8;
+    case 512: return 9;
+  }
+  return -1;
+}
+
+template <int NP>
+__host__ __device__ constexpr int fcap() {
+  return NP <= 256 ? 8 : 4;
+}
+
+struct Run {
+  const DevInst* D;
+  int g, G, wave, t, sub, lane;
+  unsigned gen;
+  const float2* tw;
+  double* red;
+  float* redf;
+  int* fail;
+};
+
+// ---------------------------------------------------------------------------
+// reductions and the cluster barrier
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// Block-wide FP64 sum; result valid in thread 0.
+__device__ double block_sum(Run& R, double v) {
+  v = wave_sum(v);
+  __syncthreads();
+  if (R.lane == 0) R.red[R.wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < NWAVE; ++w) s += R.red[w];
+  }
+  return s;
+}
+
+__device__ __forceinline__ void atomic_fmax_slot(unsigned* p, float v) {
+  __hip_atomic_fetch_max(p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float load_slot(const unsigned* p) {
+  return __uint_as_float(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Every wave writes back its stores (agent-scope release fence: buffer_wbl2 +
+// s_waitcnt vmcnt(0)) before the workgroup barrier, so thread 0's counter
+// increment is ordered after all stores of the workgroup; after the counter
+// reaches the generation target every wave invalidates its caches
+// (agent-scope acquire fence) before reading data written by other
+// workgroups, which may sit on another XCD.  The spin is bounded: on time-out
+// the failure flag is raised and every workgroup of the instance leaves.
+__device__ bool cluster_sync(Run& R) {
+  if (R.G == 1) {
+    __syncthreads();
+    return true;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  R.gen += 1u;
+  if (threadIdx.x == 0) {
+    unsigned* ctr = R.D->ctrl;
+    const unsigned target = R.gen * (unsigned)R.G;
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    int fail = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      ++spins;
+      if ((spins & 255u) == 0u &&
+          (spins > SPIN_LIMIT || __hip_atomic_load(ctr + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *R.fail = fail;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *R.fail == 0;
+}
+
+// ---------------------------------------------------------------------------
+// line helpers
+// ---------------------------------------------------------------------------
+template <int NP>
+__device__ __forceinline__ void load_row(float2 (&v)[Line<NP>::L], const float2* __restrict__ row, int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = row[l * Line<NP>::LPF + t];
+}
+template <int NP>
+__device__ __forceinline__ void store_row(float2* __restrict__ row, const float2 (&v)[Line<NP>::L], int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) row[l * Line<NP>::LPF + t] = v[l];
+}
+template <int NP>
+__device__ __forceinline__ void load_col(float2 (&v)[Line<NP>::L], const float2* __restrict__ f, int p, int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = f[(size_t)(l * Line<NP>::LPF + t) * NP + p];
+}
+template <int NP>
+__device__ __forceinline__ void store_col(float2* __restrict__ f, const float2 (&v)[Line<NP>::L], int p, int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) f[(size_t)(l * Line<NP>::LPF + t) * NP + p] = v[l];
+}
+
+// Lines [line0, line0 + per) of this workgroup are visited SUB at a time by
+// each wave; `li` is the line handled by this lane group, `act` is false for
+// the padding groups of the last round (they compute on a valid line but
+// store nothing).  Usage: LINES_BEGIN(R, NP, r, act) ... LINES_END
+#define LINES_BEGIN(R, NP, li, act)                                          \
+  {                                                                          \
+    const int per_ = NP / (R).G;                                             \
+    const int line0_ = (R).g * per_;                                         \
+    for (int base_ = 0; base_ < per_; base_ += NWAVE * Line<NP>::SUB) {      \
+      const int rel_ = base_ + (R).wave * Line<NP>::SUB + (R).sub;           \
+      const bool act = rel_ < per_;                                          \
+      const int li = line0_ + (act ? rel_ : per_ - 1);
+#define LINES_END \
+  }               \
+  }
+
+// ---------------------------------------------------------------------------
+// ePIE / mPIE / multiPIE position passes
+// ---------------------------------------------------------------------------
+template <int NP>
+__device__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int nl = D.a.nlambda, no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No;
+  const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  const float2* __restrict__ O = (const float2*)D.a.object;
+  const float2* __restrict__ P = (const float2*)D.a.probe;
+  float2* __restrict__ W = D.W;
+  const bool cb = D.a.propagator == PTY_FRAUNHOFER;
+  if (R.g == 0 && threadIdx.x < 32) zslot[threadIdx.x] = 0u;
+  float pmax = 0.f, omax = 0.f;
+  LINES_BEGIN(R, NP, r, act)
+    float sp[L], so[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) sp[l] = so[l] = 0.f;
+    for (int fp = 0; fp < D.nPr; ++fp) {
+      float2 v[L];
+      load_row<NP>(v, P + fp * Np2 + (size_t)r * NP, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) sp[l] += cabs2(v[l]);
+    }
+    for (int fo = 0; fo < D.nO; ++fo) {
+      float2 v[L];
+      load_row<NP>(v, O + fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) so[l] += cabs2(v[l]);
+    }
+    if (act) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        pmax = fmaxf(pmax, sp[l]);
+        omax = fmaxf(omax, so[l]);
+      }
+    }
+    for (int il = 0; il < nl; ++il)
+      for (int io = 0; io < no; ++io)
+        for (int ip = 0; ip < npm; ++ip)
+          for (int is = 0; is < ns; ++is) {
+            const int f = ((il * no + io) * npm + ip) * ns + is;
+            const float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + (size_t)(row0 + r) * No + col0;
+            const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + (size_t)r * NP;
+            float2 v[L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              const int x = l * LPF + R.t;
+              float2 e = cmul(orow[x], prow[x]);
+              if (cb && ((r + x) & 1)) e = make_float2(-e.x, -e.y);
+              v[l] = e;
+            }
+            fft_fwd<NP>(v, R.tw, R.t);
+            if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+          }
+  LINES_END
+  pmax = wave_max(pmax);
+  omax = wave_max(omax);
+  if (R.lane == 0) {
+    atomic_fmax_slot(slot + 0, pmax);
+    atomic_fmax_slot(slot + 1, omax);
+  }
+}
+
+// Fraunhofer detector column pass: forward column transform, unitary scale,
+// intensity constraint with the measured frame, inverse column transform.
+// Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
+// BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
+template <int NP>
+__device__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int nF, int fs) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  const DevInst& D = *R.D;
+  const size_t Np2 = (size_t)NP * NP;
+  float2* __restrict__ W = D.W;
+  const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
+  const float invN = 1.0f / (float)NP;
+  double err = 0.0;
+  LINES_BEGIN(R, NP, p, act)
+    float2 v[FC][L];
+    float iest[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) iest[l] = 0.f;
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        load_col<NP>(v[f], W + f * Np2, p, R.t);
+        fft_fwd<NP>(v[f], R.tw, R.t);
+#pragma unroll
+        for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], invN);
+        if (f % fs == fs - 1) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) iest[l] += cabs2(v[f][l]);
+        }
+      }
+    }
+    float amp[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const float im = Ik[(size_t)p * NP + l * LPF + R.t];
+      if (act) err += (double)fabsf(im - iest[l]);
+      amp[l] = sqrtf(im / (iest[l] + 1e-10f));
+    }
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], amp[l]);
+        fft_inv<NP>(v[f], R.tw, R.t);
+        if (act) store_col<NP>(W + f * Np2, v[f], p, R.t);
+      }
+    }
+  LINES_END
+  const double s = block_sum(R, err);
+  if (threadIdx.x == 0) ep[(size_t)j * R.G + R.g] = s;
+}
+
+// Column pass of an angular-spectrum propagation: forward column transform,
+// multiply by the (transposed, bit-reversed, 1/Np^2-scaled) transfer function
+// or its conjugate, inverse column transform.
+template <int NP>
+__device__ void tf_cols(Run& R, int nF, bool conj) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const size_t Np2 = (size_t)NP * NP;
+  float2* __restrict__ W = D.W;
+  const float2* __restrict__ HT = D.HT;
+  LINES_BEGIN(R, NP, p, act)
+    float2 h[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      h[l] = HT[(size_t)p * NP + l * LPF + R.t];
+      if (conj) h[l] = conjf2(h[l]);
+    }
+    for (int f = 0; f < nF; ++f) {
+      float2 v[L];
+      load_col<NP>(v, W + f * Np2, p, R.t);
+      fft_fwd<NP>(v, R.tw, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) v[l] = cmul(v[l], h[l]);
+      fft_inv<NP>(v, R.tw, R.t);
+      if (act) store_col<NP>(W + f * Np2, v, p, R.t);
+    }
+  LINES_END
+}
+
+// ASP detector row pass: inverse row transform gives the detector field, the
+// intensity constraint is applied in natural order, then the forward row
+// transform starts the back-propagation.
+template <int NP>
+__device__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  const DevInst& D = *R.D;
+  const size_t Np2 = (size_t)NP * NP;
+  float2* __restrict__ W = D.W;
+  const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
+  const int nF = D.nF, fs = D.fs;
+  double err = 0.0;
+  LINES_BEGIN(R, NP, r, act)
+    float2 v[FC][L];
+    float iest[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) iest[l] = 0.f;
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        load_row<NP>(v[f], W + f * Np2 + (size_t)r * NP, R.t);
+        fft_inv<NP>(v[f], R.tw, R.t);
+        if (f % fs == fs - 1) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) iest[l] += cabs2(v[f][l]);
+        }
+      }
+    }
+    float amp[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const float im = Ik[(size_t)r * NP + l * LPF + R.t];
+      if (act) err += (double)fabsf(im - iest[l]);
+      amp[l] = sqrtf(im / (iest[l] + 1e-10f));
+    }
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], amp[l]);
+        fft_fwd<NP>(v[f], R.tw, R.t);
+        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v[f], R.t);
+      }
+    }
+  LINES_END
+  const double s = block_sum(R, err);
+  if (threadIdx.x == 0) ep[(size_t)j * R.G + R.g] = s;
+}
+
+// Last row pass: inverse row transform gives the updated exit waves, the
+// difference to the exit waves of the same position, and the engine's
+// object-patch and probe updates (both from the pre-update patch and probe),
+// written back in place.  Each object pixel of the patch has one owner.
+template <int NP>
+__device__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  const DevInst& D = *R.D;
+  const int nl = D.a.nlambda, no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No;
+  const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  float2* __restrict__ O = (float2*)D.a.object;
+  float2* __restrict__ P = (float2*)D.a.probe;
+  const float2* __restrict__ W = D.W;
+  const int prop = D.a.propagator;
+  const bool cb = prop == PTY_FRAUNHOFER;
+  const float invN = 1.0f / (float)NP;
+  const int engine = D.a.engine;
+  const float Pmax = load_slot(slot + 0), Omax = load_slot(slot + 1);
+  const float bO = D.a.betaObject, bP = D.a.betaProbe;
+  LINES_BEGIN(R, NP, r, act)
+    const size_t orow_off = (size_t)(row0 + r) * No + col0;
+    const size_t prow_off = (size_t)r * NP;
+    float2 d[FC][L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < D.nF) {
+        load_row<NP>(d[f], W + f * Np2 + prow_off, R.t);
+        fft_inv<NP>(d[f], R.tw, R.t);
+        const int is = f % ns;
+        const int ip = (f / ns) % npm;
+        const int io = (f / (ns * npm)) % no;
+        const int il = f / (ns * npm * no);
+        const float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
+        const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const int x = l * LPF + R.t;
+          float2 u = d[f][l];
+          if (prop != PTY_ASP) {
+            u = cscale(u, invN);
+            if (cb && ((r + x) & 1)) u = make_float2(-u.x, -u.y);
+          }
+          d[f][l] = csub(u, cmul(orow[x], prow[x]));
+        }
+      }
+    }
+    if (engine == PTY_EPIE) {
+      // ePIE.py:128-160: P/max(sum|P|^2) and O/max(sum|O|^2) weights; object
+      // increment summed over (lambda, psm, slice), probe over (lambda, osm, slice)
+      const float invP = 1.0f / Pmax, invO = 1.0f / Omax;
+      float2 accP[FC][L];
+#pragma unroll
+      for (int ip = 0; ip < FC; ++ip) {
+        if (ip < npm) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) accP[ip][l] = make_float2(0.f, 0.f);
+          for (int il = 0; il < nl; ++il)
+            for (int io = 0; io < no; ++io)
+              for (int is = 0; is < ns; ++is) {
+                const float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
+                const int f = ((il * no + io) * npm + ip) * ns + is;
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                  const float2 o = orow[l * LPF + R.t];
+                  const float2 fr = make_float2(o.x * invO, -o.y * invO);
+                  float2 dd = make_float2(0.f, 0.f);
+#pragma unroll
+                  for (int ff = 0; ff < FC; ++ff)
+                    if (ff == f) dd = d[ff][l];
+                  accP[ip][l] = cadd(accP[ip][l], cmul(fr, dd));
+                }
+              }
+        }
+      }
+      for (int io = 0; io < no; ++io) {
+        float2 accO[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) accO[l] = make_float2(0.f, 0.f);
+        for (int il = 0; il < nl; ++il)
+          for (int ip = 0; ip < npm; ++ip)
+            for (int is = 0; is < ns; ++is) {
+              const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
+              const int f = ((il * no + io) * npm + ip) * ns + is;
+#pragma unroll
+              for (int l = 0; l < L; ++l) {
+                const float2 pv = prow[l * LPF + R.t];
+                const float2 fr = make_float2(pv.x * invP, -pv.y * invP);
+                float2 dd = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int ff = 0; ff < FC; ++ff)
+                  if (ff == f) dd = d[ff][l];
+                accO[l] = cadd(accO[l], cmul(fr, dd));
+              }
+            }
+        for (int il = 0; il < nl; ++il)
+          for (int is = 0; is < ns; ++is) {
+            float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              const int x = l * LPF + R.t;
+              const float2 nv = cadd(orow[x], cscale(accO[l], bO));
+              if (act) orow[x] = nv;
+            }
+          }
+      }
+      for (int ip = 0; ip < npm; ++ip) {
+        float2 a[L];
+#pragma unroll
+        for (int ipp = 0; ipp < FC; ++ipp)
+          if (ipp == ip) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) a[l] = accP[ipp][l];
+          }
+        for (int il = 0; il < nl; ++il)
+          for (int is = 0; is < ns; ++is) {
+            float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              const int x = l * LPF + R.t;
+              const float2 nv = cadd(prow[x], cscale(a[l], bP));
+              if (act) prow[x] = nv;
+            }
+          }
+      }
+    } else {
+      // mPIE.py:226-270 / multiPIE.py:153-205 (CPM): regularised weights
+      // conj(P)/(aO*Pmax + (1-aO)|P|^2) and conj(O)/(aP*Omax + (1-aP)|O|^2)
+      const float aPm = D.a.alphaObject * Pmax;
+      const float aOm = D.a.alphaProbe * Omax;
+      const float omO = D.omAlphaO, omP = D.omAlphaP;
+      const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
+      const bool sumLambda = engine == PTY_MULTIPIE;
+      float2 newP[FC][L];
+#pragma unroll
+      for (int fp = 0; fp < FC; ++fp) {
+        if (fp < D.nPr) {
+          const int is = fp % ns, ip = (fp / ns) % npm, il = fp / (ns * npm);
+          float2 acc[L];
+#pragma unroll
+          for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+          const int l0 = sumLambda ? 0 : il, l1 = sumLambda ? nl : il + 1;
+          for (int jl = l0; jl < l1; ++jl)
+            for (int io = 0; io < no; ++io) {
+              const float2* orow = O + (size_t)((jl * no + io) * ns + is) * No2 + orow_off;
+              const int f = ((jl * no + io) * npm + ip) * ns + is;
+#pragma unroll
+              for (int l = 0; l < L; ++l) {
+                const float2 o = orow[l * LPF + R.t];
+                const float inv = 1.0f / (aOm + omP * cabs2(o));
+                const float2 fr = make_float2(o.x * inv, -o.y * inv);
+                float2 dd = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int ff = 0; ff < FC; ++ff)
+                  if (ff == f) dd = d[ff][l];
+                acc[l] = cadd(acc[l], cmul(fr, dd));
+              }
+            }
+          const float2* prow = P + (size_t)fp * Np2 + prow_off;
+#pragma unroll
+          for (int l = 0; l < L; ++l) newP[fp][l] = cadd(prow[l * LPF + R.t], cscale(acc[l], stepP));
+        }
+      }
+      for (int il = 0; il < nl; ++il)
+        for (int io = 0; io < no; ++io)
+          for (int is = 0; is < ns; ++is) {
+            float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
+            float2 acc[L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+            for (int ip = 0; ip < npm; ++ip) {
+              const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
+              const int f = ((il * no + io) * npm + ip) * ns + is;
+#pragma unroll
+              for (int l = 0; l < L; ++l) {
+                const float2 pv = prow[l * LPF + R.t];
+                const float inv = 1.0f / (aPm + omO * cabs2(pv));
+                const float2 fr = make_float2(pv.x * inv, -pv.y * inv);
+                float2 dd = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int ff = 0; ff < FC; ++ff)
+                  if (ff == f) dd = d[ff][l];
+                acc[l] = cadd(acc[l], cmul(fr, dd));
+              }
+            }
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              const int x = l * LPF + R.t;
+              const float2 nv = cadd(orow[x], cscale(acc[l], bO));
+              if (act) orow[x] = nv;
+            }
+          }
+#pragma unroll
+      for (int fp = 0; fp < FC; ++fp) {
+        if (fp < D.nPr && act) store_row<NP>(P + (size_t)fp * Np2 + prow_off, newP[fp], R.t);
+      }
+    }
+  LINES_END
+}
+
+// mPIE / multiPIE momentum step over the whole object and probe
+// (mPIE.py:196-224, multiPIE.py:125-151); each workgroup owns a contiguous
+// chunk, so the step is atomic-free.
+__device__ void momentum_pass(Run& R) {
+  const DevInst& D = *R.D;
+  const float eta = D.a.momFriction, nu = D.a.momFeedback;
+  for (int which = 0; which < 2; ++which) {
+    const size_t n = which == 0 ? (size_t)D.nO * D.a.No * D.a.No : (size_t)D.nPr * D.a.Np * D.a.Np;
+    float2* X = (float2*)(which == 0 ? D.a.object : D.a.probe);
+    float2* B = (float2*)(which == 0 ? D.a.objectBuffer : D.a.probeBuffer);
+    float2* M = (float2*)(which == 0 ? D.a.objectMomentum : D.a.probeMomentum);
+    const size_t chunk = (n + R.G - 1) / R.G;
+    const size_t b = chunk * R.g;
+    const size_t e = b + chunk < n ? b + chunk : n;
+    for (size_t i = b + threadIdx.x; i < e; i += NTHREADS) {
+      const float2 x = X[i], bb = B[i], m = M[i];
+      const float2 grad = csub(bb, x);
+      const float2 mn = cadd(grad, cscale(m, eta));
+      const float2 xn = csub(x, cscale(mn, nu));
+      M[i] = mn;
+      X[i] = xn;
+      B[i] = xn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// e3PIE slice chain (e3PIE.py:89-173, updates :192-225).  W holds the
+// (lambda, psm) fields of the slice being propagated; ESW keeps the exit
+// wave of every slice of the current position.
+// ---------------------------------------------------------------------------
+template <int NP>
+__device__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No;
+  const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  const float2* __restrict__ O = (const float2*)D.a.object;
+  const float2* __restrict__ P = (const float2*)D.a.probe;
+  float2* __restrict__ W = D.W;
+  float2* __restrict__ E = D.ESW;
+  const bool cb = D.a.propagator == PTY_FRAUNHOFER && ns == 1;
+  const int nF = D.nF;
+  if (R.g == 0 && threadIdx.x < 32) zslot[threadIdx.x] = 0u;
+  float omx[MAX_SLICES];
+#pragma unroll
+  for (int s = 0; s < MAX_SLICES; ++s) omx[s] = 0.f;
+  float pmax = 0.f;
+  LINES_BEGIN(R, NP, r, act)
+    const size_t orow_off = (size_t)(row0 + r) * No + col0;
+#pragma unroll
+    for (int s = 0; s < MAX_SLICES; ++s) {
+      if (s < ns) {
+        float so[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) so[l] = 0.f;
+        for (int il = 0; il < nl; ++il) {
+          const float2* orow = O + (size_t)(il * ns + s) * No2 + orow_off;
+#pragma unroll
+          for (int l = 0; l < L; ++l) so[l] += cabs2(orow[l * LPF + R.t]);
+        }
+        if (act) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) omx[s] = fmaxf(omx[s], so[l]);
+        }
+      }
+    }
+    float sp[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) sp[l] = 0.f;
+    for (int f = 0; f < nF; ++f) {  // f = il * npsm + ip
+      const float2* prow = P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP;
+#pragma unroll
+      for (int l = 0; l < L; ++l) sp[l] += cabs2(prow[l * LPF + R.t]);
+    }
+    if (act) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
+    }
+    for (int f = 0; f < nF; ++f) {
+      const int il = f / npm;
+      const float2* orow = O + (size_t)(il * ns + 0) * No2 + orow_off;
+      const float2* prow = P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP;
+      float2 v[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const int x = l * LPF + R.t;
+        v[l] = cmul(orow[x], prow[x]);
+      }
+      if (act) store_row<NP>(E + (size_t)(0 * nF + f) * Np2 + (size_t)r * NP, v, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const int x = l * LPF + R.t;
+        if (cb && ((r + x) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+      }
+      fft_fwd<NP>(v, R.tw, R.t);
+      if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    }
+  LINES_END
+  pmax = wave_max(pmax);
+  if (R.lane == 0) atomic_fmax_slot(slot + 0, pmax);
+#pragma unroll
+  for (int s = 0; s < MAX_SLICES; ++s) {
+    if (s < ns) {
+      const float m = wave_max(omx[s]);
+      if (R.lane == 0) atomic_fmax_slot(slot + 2 * s + 1, m);
+    }
+  }
+}
+
+// Forward chain, slice s >= 1: P_s = ifft2(fft2(esw_{s-1}) H) (the column
+// pass already applied H / Np^2), stored into probe slice s; esw_s = P_s O_s;
+// then the forward row transform of esw_s (with the detector checkerboard
+// when s is the last slice).
+template <int NP>
+__device__ void e3_rows_forward(Run& R, int k, int s, unsigned* slot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No;
+  const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  const float2* __restrict__ O = (const float2*)D.a.object;
+  float2* __restrict__ P = (float2*)D.a.probe;
+  float2* __restrict__ W = D.W;
+  float2* __restrict__ E = D.ESW;
+  const bool cb = D.a.propagator == PTY_FRAUNHOFER && s == ns - 1;
+  const int nF = D.nF;
+  float pmax = 0.f;
+  LINES_BEGIN(R, NP, r, act)
+    const size_t orow_off = (size_t)(row0 + r) * No + col0;
+    float sp[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) sp[l] = 0.f;
+    for (int f = 0; f < nF; ++f) {
+      const int il = f / npm;
+      float2 v[L];
+      load_row<NP>(v, W + f * Np2 + (size_t)r * NP, R.t);
+      fft_inv<NP>(v, R.tw, R.t);
+      if (act) store_row<NP>(P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v, R.t);
+      const float2* orow = O + (size_t)(il * ns + s) * No2 + orow_off;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const int x = l * LPF + R.t;
+        sp[l] += cabs2(v[l]);
+        v[l] = cmul(v[l], orow[x]);
+      }
+      if (act) store_row<NP>(E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const int x = l * LPF + R.t;
+        if (cb && ((r + x) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+      }
+      fft_fwd<NP>(v, R.tw, R.t);
+      if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    }
+    if (act) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
+    }
+  LINES_END
+  pmax = wave_max(pmax);
+  if (R.lane == 0) atomic_fmax_slot(slot + 2 * s, pmax);
+}
+
+// Update of slice su from the difference wave: `fromDetector` selects the
+// detector back-propagation (Fraunhofer scale and checkerboard, difference to
+// the last exit wave) or a slice back-propagation (difference to esw_su).
+// Object slice su and probe slice su are rewritten; for su >= 1 the updated
+// probe slice is forward-transformed into W for the next back-propagation.
+template <int NP>
+__device__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool fromDetector) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  const DevInst& D = *R.D;
+  const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No;
+  const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  float2* __restrict__ O = (float2*)D.a.object;
+  float2* __restrict__ P = (float2*)D.a.probe;
+  float2* __restrict__ W = D.W;
+  const float2* __restrict__ E = D.ESW;
+  const int nF = D.nF;
+  const bool cb = D.a.propagator == PTY_FRAUNHOFER;
+  const float invN = 1.0f / (float)NP;
+  const float invP = 1.0f / load_slot(slot + 2 * su);
+  const float invO = 1.0f / load_slot(slot + 2 * su + 1);
+  const float bO = D.a.betaObject;
+  const float bP = su >= 1 ? D.a.sliceBeta : D.a.betaProbe;
+  const int esl = fromDetector ? ns - 1 : su;
+  LINES_BEGIN(R, NP, r, act)
+    const size_t orow_off = (size_t)(row0 + r) * No + col0;
+    const size_t prow_off = (size_t)r * NP;
+    float2 d[FC][L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        load_row<NP>(d[f], W + f * Np2 + prow_off, R.t);
+        fft_inv<NP>(d[f], R.tw, R.t);
+        const float2* erow = E + (size_t)(esl * nF + f) * Np2 + prow_off;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const int x = l * LPF + R.t;
+          float2 u = d[f][l];
+          if (fromDetector) {
+            u = cscale(u, invN);
+            if (cb && ((r + x) & 1)) u = make_float2(-u.x, -u.y);
+          }
+          d[f][l] = csub(u, erow[x]);
+        }
+      }
+    }
+    // probe increments: sum over lambda of conj(O_su)/Omax * delta (axes (0,1))
+    float2 newP[FC][L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        const int ip = f % npm;
+        float2 acc[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+        for (int jl = 0; jl < nl; ++jl) {
+          const float2* orow = O + (size_t)(jl * ns + su) * No2 + orow_off;
+          const int g = jl * npm + ip;
+#pragma unroll
+          for (int l = 0; l < L; ++l) {
+            const float2 o = orow[l * LPF + R.t];
+            const float2 fr = make_float2(o.x * invO, -o.y * invO);
+            float2 dd = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int ff = 0; ff < FC; ++ff)
+              if (ff == g) dd = d[ff][l];
+            acc[l] = cadd(acc[l], cmul(fr, dd));
+          }
+        }
+        const float2* prow = P + (size_t)(f * ns + su) * Np2 + prow_off;
+#pragma unroll
+        for (int l = 0; l < L; ++l) newP[f][l] = cadd(prow[l * LPF + R.t], cscale(acc[l], bP));
+      }
+    }
+    // object increment: sum over (lambda, psm) of conj(P_su)/Pmax * delta (axes (0,2))
+    {
+      float2 acc[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int f = 0; f < FC; ++f) {
+        if (f < nF) {
+          const float2* prow = P + (size_t)(f * ns + su) * Np2 + prow_off;
+#pragma unroll
+          for (int l = 0; l < L; ++l) {
+            const float2 pv = prow[l * LPF + R.t];
+            const float2 fr = make_float2(pv.x * invP, -pv.y * invP);
+            acc[l] = cadd(acc[l], cmul(fr, d[f][l]));
+          }
+        }
+      }
+      for (int il = 0; il < nl; ++il) {
+        float2* orow = O + (size_t)(il * ns + su) * No2 + orow_off;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const int x = l * LPF + R.t;
+          const float2 nv = cadd(orow[x], cscale(acc[l], bO));
+          if (act) orow[x] = nv;
+        }
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        if (act) store_row<NP>(P + (size_t)(f * ns + su) * Np2 + prow_off, newP[f], R.t);
+        if (su >= 1) {
+          fft_fwd<NP>(newP[f], R.tw, R.t);
+          if (act) store_row<NP>(W + f * Np2 + prow_off, newP[f], R.t);
+        }
+      }
+    }
+  LINES_END
+}
+
+// ---------------------------------------------------------------------------
+// per-position drivers
+// ---------------------------------------------------------------------------
+template <int NP>
+__device__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
+  const DevInst& D = *R.D;
+  pie_rows_first<NP>(R, k, slot, zslot);
+  if (!cluster_sync(R)) return false;
+  if (D.a.propagator == PTY_ASP) {
+    tf_cols<NP>(R, D.nF, false);
+    if (!cluster_sync(R)) return false;
+    asp_rows_constraint<NP>(R, k, j, ep);
+    if (!cluster_sync(R)) return false;
+    tf_cols<NP>(R, D.nF, true);
+    if (!cluster_sync(R)) return false;
+  } else {
+    detector_cols_fraunhofer<NP>(R, k, j, ep, D.nF, D.fs);
+    if (!cluster_sync(R)) return false;
+  }
+  pie_rows_update<NP>(R, k, slot);
+  return cluster_sync(R);
+}
+
+template <int NP>
+__device__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
+  const DevInst& D = *R.D;
+  const int ns = D.a.nslice;
+  e3_rows_first<NP>(R, k, slot, zslot);
+  if (!cluster_sync(R)) return false;
+  for (int s = 1; s < ns; ++s) {
+    tf_cols<NP>(R, D.nF, false);
+    if (!cluster_sync(R)) return false;
+    e3_rows_forward<NP>(R, k, s, slot);
+    if (!cluster_sync(R)) return false;
+  }
+  detector_cols_fraunhofer<NP>(R, k, j, ep, D.nF, 1);
+  if (!cluster_sync(R)) return false;
+  e3_rows_update<NP>(R, k, ns - 1, slot, true);
+  if (!cluster_sync(R)) return false;
+  for (int s = ns - 1; s >= 1; --s) {
+    tf_cols<NP>(R, D.nF, true);
+    if (!cluster_sync(R)) return false;
+    e3_rows_update<NP>(R, k, s - 1, slot, false);
+    if (!cluster_sync(R)) return false;
+  }
+  return true;
+}
+
+// Iterations [it0, it1); the first iteration starts at position jb and the
+// last one stops before position je (full iterations: jb = 0, je >= F).  The
+// error of an iteration is reduced when its last position has run.
+template <int NP>
+__device__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
+  const DevInst& D = *R.D;
+  const int F = D.a.F;
+  const int engine = D.a.engine;
+  const bool momentum = (engine == PTY_MPIE || engine == PTY_MULTIPIE) && D.a.momentum_mask != nullptr;
+  unsigned q = 0;
+  for (int it = it0; it < it1; ++it) {
+    const int* order = D.a.order + (size_t)it * F;
+    double* ep = D.errpart + (size_t)(it & 1) * F * R.G;
+    const int j0 = it == it0 ? (jb < F ? jb : F) : 0;
+    const int j1 = it == it1 - 1 ? (je < F ? je : F) : F;
+    for (int j = j0; j < j1; ++j, ++q) {
+      const int k = order[j];
+      unsigned* slot = D.ctrl + 16 + (q & 3u) * 32;
+      unsigned* zslot = D.ctrl + 16 + ((q + 2u) & 3u) * 32;
+      const bool ok = engine == PTY_E3PIE ? e3pie_position<NP>(R, k, j, slot, zslot, ep)
+                                          : pie_position<NP>(R, k, j, slot, zslot, ep);
+      if (!ok) return;
+      if (momentum && D.a.momentum_mask[(size_t)it * F + j]) {
+        momentum_pass(R);
+        if (!cluster_sync(R)) return;
+      }
+    }
+    if (j1 == F && R.g == 0) {
+      // getErrorMetrics (BaseEngine.py:819-841): errorAtPos = sum|I - Iest| /
+      // (energyAtPos + 1e-20) in FP64, error = sum over positions
+      double acc = 0.0;
+      for (int jj = threadIdx.x; jj < F; jj += NTHREADS) {
+        const int kk = order[jj];
+        double s = 0.0;
+        for (int g = 0; g < R.G; ++g) s += ep[(size_t)jj * R.G + g];
+        const float en = D.a.energy[kk] + 1e-20f;
+        const double e = s / (double)en;
+        D.a.err_at_pos[(size_t)it * F + kk] = e;
+        acc += e;
+      }
+      const double tot = block_sum(R, acc);
+      if (threadIdx.x == 0) D.a.err_iter[it] = tot;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NTHREADS) void pty_iterate_kernel(const DevInst* __restrict__ insts,
+                                                               const int2* __restrict__ bmap, int it0, int it1,
+                                                               int jb, int je) {
+  __shared__ float2 s_tw[256];
+  __shared__ double s_red[NWAVE];
+  __shared__ float s_redf[2 * NWAVE];
+  __shared__ int s_fail;
+  const int2 bm = bmap[blockIdx.x];
+  const DevInst* D = insts + bm.x;
+  Run R;
+  R.D = D;
+  R.g = bm.y;
+  R.G = D->G;
+  R.wave = threadIdx.x >> 6;
+  R.lane = threadIdx.x & 63;
+  R.gen = 0u;
+  R.tw = s_tw;
+  R.red = s_red;
+  R.redf = s_redf;
+  R.fail = &s_fail;
+  const int np = D->a.Np;
+  const int lg = log2_np(np) - 4;
+  for (int i = threadIdx.x; i < np / 2; i += NTHREADS) s_tw[i] = g_twiddle[lg][i];
+  __syncthreads();
+  switch (np) {
+#define CASE(N)                              \
+  case N:                                    \
+    R.t = R.lane & (Line<N>::LPF - 1);       \
+    R.sub = R.lane / Line<N>::LPF;           \
+    run_iterations<N>(R, it0, it1, jb, je);          \
+    break;
+    CASE(16)
+    CASE(32)
+    CASE(64)
+    CASE(128)
+    CASE(256)
+    CASE(512)
+#undef CASE
+    default:
+      break;
+  }
+}
+
+__global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
+  unsigned* c = insts[blockIdx.x].ctrl;
+  for (int i = threadIdx.x; i < 16 + 4 * 32; i += blockDim.x) c[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// layout preparation kernels (plan creation only)
+// ---------------------------------------------------------------------------
+// dst[f][p][n] = src[f][brev(n)][brev(p)]  (measured frame for the column pass)
+__global__ void pty_permute_frames_kernel(const float* __restrict__ src, float* __restrict__ dst, int F, int np,
+                                          int lg) {
+  const size_t np2 = (size_t)np * np;
+  const size_t total = (size_t)F * np2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t f = i / np2;
+    const int rem = (int)(i % np2);
+    const int p = rem / np, n = rem % np;
+    dst[i] = src[f * np2 + (size_t)brev(n, lg) * np + brev(p, lg)];
+  }
+}
+
+// HT[p][n] = tf[brev(n)][brev(p)] / np^2 (np^2 is a power of two: exact)
+__global__ void pty_permute_tf_kernel(const float2* __restrict__ tf, float2* __restrict__ ht, int np, int lg) {
+  const int total = np * np;
+  const float sc = 1.0f / ((float)np * (float)np);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int p = i / np, n = i % np;
+    ht[i] = cscale(tf[brev(n, lg) * np + brev(p, lg)], sc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// standalone propagators and FFT (Operators.object2detector/detector2object)
+// ---------------------------------------------------------------------------
+// mode bits for pty_lines_kernel
+#define LN_COLS 1        // process columns instead of rows
+#define LN_INVERSE 2     // conjugate kernel
+#define LN_PRE_CB 4      // multiply by (-1)^(y+x) before the transform
+#define LN_POST_CB 8     // multiply by (-1)^(y+x) after the transform
+#define LN_SCALE 16      // multiply by 1/Np after the transform
+
+// Forward-structured (DIF) transform of every line of every field, written
+// back in natural order (bit-reversal applied on the store).
+template <int NP>
+__global__ __launch_bounds__(NTHREADS) void pty_lines_kernel(float2* __restrict__ data, int nfields, int mode) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB, LG = Line<NP>::LOG2;
+  __shared__ float2 s_tw[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane & (LPF - 1), sub = lane / LPF;
+  const bool inv = mode & LN_INVERSE;
+  for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) {
+    const float2 w = g_twiddle[LG - 4][i];
+    s_tw[i] = inv ? conjf2(w) : w;
+  }
+  __syncthreads();
+  const int line = (blockIdx.x * NWAVE + wave) * SUB + sub;
+  if (line >= nfields * NP) return;
+  const int f = line / NP, li = line % NP;
+  float2* base = data + (size_t)f * NP * NP;
+  const bool cols = mode & LN_COLS;
+  float2 v[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const int n = l * LPF + t;
+    const size_t off = cols ? (size_t)n * NP + li : (size_t)li * NP + n;
+    float2 x = base[off];
+    if ((mode & LN_PRE_CB) && ((n + li) & 1)) x = make_float2(-x.x, -x.y);
+    v[l] = x;
+  }
+  fft_fwd<NP>(v, s_tw, t);
+  const float sc = 1.0f / (float)NP;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const int n = brev(l * LPF + t, LG);
+    float2 x = v[l];
+    if (mode & LN_SCALE) x = cscale(x, sc);
+    if ((mode & LN_POST_CB) && ((n + li) & 1)) x = make_float2(-x.x, -x.y);
+    const size_t off = cols ? (size_t)n * NP + li : (size_t)li * NP + n;
+    base[off] = x;
+  }
+}
+
+// ASP column pass for the standalone propagator: natural-order rows were
+// forward transformed in place (bit-reversed x); transform the columns,
+// multiply by tf (natural FFT order, read through the bit reversal), inverse
+// column transform.
+template <int NP>
+__global__ __launch_bounds__(NTHREADS) void pty_asp_cols_kernel(float2* __restrict__ data, int nfields,
+                                                                const float2* __restrict__ tf, int conj) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB, LG = Line<NP>::LOG2;
+  __shared__ float2 s_tw[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane & (LPF - 1), sub = lane / LPF;
+  for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[LG - 4][i];
+  __syncthreads();
+  const int line = (blockIdx.x * NWAVE + wave) * SUB + sub;
+  if (line >= nfields * NP) return;
+  const int f = line / NP, p = line % NP;
+  float2* base = data + (size_t)f * NP * NP;
+  float2 v[L];
+  load_col<NP>(v, base, p, t);
+  fft_fwd<NP>(v, s_tw, t);
+  const float sc = 1.0f / ((float)NP * (float)NP);
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    float2 h = tf[(size_t)brev(l * LPF + t, LG) * NP + brev(p, LG)];
+    if (conj) h = conjf2(h);
+    v[l] = cmul(v[l], cscale(h, sc));
+  }
+  fft_inv<NP>(v, s_tw, t);
+  store_col<NP>(base, v, p, t);
+}
+
+template <int NP>
+__global__ __launch_bounds__(NTHREADS) void pty_rows_fwd_kernel(float2* __restrict__ data, int nfields) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
+  __shared__ float2 s_tw[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane & (LPF - 1), sub = lane / LPF;
+  for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
+  __syncthreads();
+  const int line = (blockIdx.x * NWAVE + wave) * SUB + sub;
+  if (line >= nfields * NP) return;
+  float2* row = data + (size_t)line * NP;
+  float2 v[L];
+  load_row<NP>(v, row, t);
+  fft_fwd<NP>(v, s_tw, t);
+  store_row<NP>(row, v, t);
+}
+
+template <int NP>
+__global__ __launch_bounds__(NTHREADS) void pty_rows_inv_kernel(float2* __restrict__ data, int nfields) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
+  __shared__ float2 s_tw[256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane & (LPF - 1), sub = lane / LPF;
+  for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
+  __syncthreads();
+  const int line = (blockIdx.x * NWAVE + wave) * SUB + sub;
+  if (line >= nfields * NP) return;
+  float2* row = data + (size_t)line * NP;
+  float2 v[L];
+  load_row<NP>(v, row, t);
+  fft_inv<NP>(v, s_tw, t);
+  store_row<NP>(row, v, t);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct pty_plan {
+  int device = 0;
+  int ninst = 0;
+  int nblocks = 0;
+  bool cooperative = false;
+  DevInst* d_insts = nullptr;
+  int2* d_bmap = nullptr;
+  std::vector<DevInst> h_insts;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+bool g_tw_ready[64];
+
+int ensure_twiddles() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(PTY_EINVAL, "device index %d out of range", dev);
+  if (g_tw_ready[dev]) return PTY_OK;
+  static float2 host[6][256];
+  for (int lg = 4; lg <= 9; ++lg) {
+    const int n = 1 << lg;
+    for (int j = 0; j < n / 2; ++j) {
+      const double a = -2.0 * 3.14159265358979323846 * (double)j / (double)n;
+      host[lg - 4][j] = make_float2((float)cos(a), (float)sin(a));
+    }
+  }
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_twiddle), host, sizeof host, 0, hipMemcpyHostToDevice));
+  g_tw_ready[dev] = true;
+  return PTY_OK;
+}
+
+int dalloc(pty_plan* p, void** out, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess) return set_err(PTY_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  p->allocs.push_back(*out);
+  return PTY_OK;
+}
+
+void free_plan(pty_plan* p) {
+  for (void* a : p->allocs) (void)hipFree(a);
+  p->allocs.clear();
+}
+
+int auto_workgroups(const pty_instance& a, int ninst) {
+  if (ninst > 1) return 1;
+  const int sub = a.Np >= 64 ? 1 : 64 / a.Np;
+  int g = a.Np / (NWAVE * sub);
+  if (g < 1) g = 1;
+  if (g > 64) g = 64;
+  return g;
+}
+
+int validate(const pty_instance& a, int idx) {
+  if (a.engine < PTY_EPIE || a.engine > PTY_E3PIE) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
+  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_ASP)
+    return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
+  if (log2_np(a.Np) < 0)
+    return set_err(PTY_EINVAL, "instance %d: Np=%d unsupported (powers of two 16..512)", idx, a.Np);
+  if (a.nlambda < 1 || a.nosm < 1 || a.npsm < 1 || a.nslice < 1)
+    return set_err(PTY_EINVAL, "instance %d: mode counts must be >= 1", idx);
+  if (a.No < a.Np) return set_err(PTY_EINVAL, "instance %d: No=%d < Np=%d", idx, a.No, a.Np);
+  if (a.F < 1 || a.n_iter < 1) return set_err(PTY_EINVAL, "instance %d: F and n_iter must be >= 1", idx);
+  if (!a.object || !a.probe || !a.ptychogram || !a.positions || !a.order || !a.energy || !a.err_at_pos || !a.err_iter)
+    return set_err(PTY_EINVAL, "instance %d: missing array pointer", idx);
+  if (a.propagator == PTY_ASP && a.nlambda > 1)
+    return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, set polychromeASP instead of ASP", idx);
+  if (a.propagator == PTY_ASP && !a.tf) return set_err(PTY_EINVAL, "instance %d: ASP needs a transfer function", idx);
+  const int fc = a.Np == 512 ? fcap<512>() : fcap<256>();
+  if (a.engine == PTY_E3PIE) {
+    if (a.nosm != 1) return set_err(PTY_EINVAL, "instance %d: e3PIE needs nosm == 1", idx);
+    if (a.nslice > MAX_SLICES) return set_err(PTY_EINVAL, "instance %d: nslice > %d", idx, MAX_SLICES);
+    if (a.propagator == PTY_ASP)
+      return set_err(PTY_EINVAL, "instance %d: e3PIE supports the Fraunhofer detector propagator only", idx);
+    if (!a.tf) return set_err(PTY_EINVAL, "instance %d: e3PIE needs the slice transfer function", idx);
+    if (a.nlambda * a.npsm > fc) return set_err(PTY_EINVAL, "instance %d: too many modes for Np=%d", idx, a.Np);
+  } else {
+    const int nF = a.nlambda * a.nosm * a.npsm * a.nslice;
+    if (nF > fc) return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed %d for Np=%d", idx, nF, fc, a.Np);
+    if (a.nlambda * a.npsm * a.nslice > fc) return set_err(PTY_EINVAL, "instance %d: too many probe fields", idx);
+    if ((a.engine == PTY_MPIE || a.engine == PTY_MULTIPIE) && a.momentum_mask &&
+        (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
+      return set_err(PTY_EINVAL, "instance %d: momentum needs buffer and momentum arrays", idx);
+  }
+  return PTY_OK;
+}
+
+}  // namespace
+
+template <int NP>
+static int launch_propagate(float2* data, int nf, int prop, int inverse, const float2* tf, hipStream_t st) {
+  const int lines = nf * NP;
+  const int per_block = NWAVE * Line<NP>::SUB;
+  const int blocks = (lines + per_block - 1) / per_block;
+  if (prop == PTY_ASP) {
+    pty_rows_fwd_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf);
+    pty_asp_cols_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf, tf, inverse);
+    pty_rows_inv_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf);
+  } else {
+    const int cbm = prop == PTY_FRAUNHOFER ? 1 : 0;
+    const int inv = inverse ? LN_INVERSE : 0;
+    pty_lines_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf, inv | (cbm ? LN_PRE_CB : 0));
+    pty_lines_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf, inv | LN_COLS | LN_SCALE | (cbm ? LN_POST_CB : 0));
+  }
+  HIP_TRY(hipGetLastError());
+  return PTY_OK;
+}
+
+template <int NP>
+static int launch_fft2(float2* data, int nf, int sign, hipStream_t st) {
+  const int lines = nf * NP;
+  const int per_block = NWAVE * Line<NP>::SUB;
+  const int blocks = (lines + per_block - 1) / per_block;
+  const int inv = sign > 0 ? LN_INVERSE : 0;
+  pty_lines_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf, inv);
+  pty_lines_kernel<NP><<<blocks, NTHREADS, 0, st>>>(data, nf, inv | LN_COLS);
+  HIP_TRY(hipGetLastError());
+  return PTY_OK;
+}
+
+extern "C" {
+
+int pty_abi_version(void) { return PTY_ABI_VERSION; }
+
+const char* pty_last_error(void) { return g_err.c_str(); }
+
+int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, void* hip_stream) {
+  if (!out || !insts || ninst < 1) return set_err(PTY_EINVAL, "pty_plan_create: bad arguments");
+  *out = nullptr;
+  int rc = ensure_twiddles();
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)hip_stream;
+  pty_plan* p = new pty_plan();
+  HIP_TRY(hipGetDevice(&p->device));
+  p->ninst = ninst;
+  p->h_insts.resize(ninst);
+  std::vector<int2> bmap;
+  int maxG = 1;
+  for (int i = 0; i < ninst; ++i) {
+    const pty_instance& a = insts[i];
+    rc = validate(a, i);
+    if (rc) {
+      free_plan(p);
+      delete p;
+      return rc;
+    }
+    DevInst& d = p->h_insts[i];
+    memset(&d, 0, sizeof d);
+    d.a = a;
+    int G = a.workgroups > 0 ? a.workgroups : auto_workgroups(a, ninst);
+    const int sub = a.Np >= 64 ? 1 : 64 / a.Np;
+    if (G > a.Np / sub) G = a.Np / sub;
+    if (G < 1) G = 1;
+    while (a.Np % G) --G;
+    d.G = G;
+    maxG = G > maxG ? G : maxG;
+    const int np = a.Np;
+    const size_t np2 = (size_t)np * np;
+    if (a.engine == PTY_E3PIE) {
+      d.nF = a.nlambda * a.npsm;
+      d.fs = 1;
+    } else {
+      d.nF = a.nlambda * a.nosm * a.npsm * a.nslice;
+      d.fs = a.nslice;
+    }
+    d.nO = a.nlambda * a.nosm * a.nslice;
+    d.nPr = a.nlambda * a.npsm * a.nslice;
+    d.omAlphaO = (float)(1.0 - (double)a.alphaObject);
+    d.omAlphaP = (float)(1.0 - (double)a.alphaProbe);
+    // positions must keep every patch inside the object
+    std::vector<int32_t> pos((size_t)a.F * 2);
+    HIP_TRY(hipMemcpy(pos.data(), a.positions, pos.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int f = 0; f < a.F; ++f) {
+      const int r = pos[2 * f], c = pos[2 * f + 1];
+      if (r < 0 || c < 0 || r + np > a.No || c + np > a.No) {
+        free_plan(p);
+        delete p;
+        return set_err(PTY_EINVAL, "instance %d: position %d (%d, %d) puts the %d-pixel patch outside the %d-pixel object",
+                       i, f, r, c, np, a.No);
+      }
+    }
+    std::vector<int32_t> ord((size_t)a.F * a.n_iter);
+    HIP_TRY(hipMemcpy(ord.data(), a.order, ord.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < ord.size(); ++q) {
+      if (ord[q] < 0 || ord[q] >= a.F) {
+        free_plan(p);
+        delete p;
+        return set_err(PTY_EINVAL, "instance %d: order entry %zu = %d out of range", i, q, ord[q]);
+      }
+    }
+    void* mem = nullptr;
+    if ((rc = dalloc(p, &mem, (size_t)d.nF * np2 * sizeof(float2)))) goto fail;
+    d.W = (float2*)mem;
+    if (a.engine == PTY_E3PIE) {
+      if ((rc = dalloc(p, &mem, (size_t)a.nslice * d.nF * np2 * sizeof(float2)))) goto fail;
+      d.ESW = (float2*)mem;
+    }
+    if ((rc = dalloc(p, &mem, (16 + 4 * 32) * sizeof(unsigned)))) goto fail;
+    d.ctrl = (unsigned*)mem;
+    if ((rc = dalloc(p, &mem, (size_t)2 * a.F * G * sizeof(double)))) goto fail;
+    d.errpart = (double*)mem;
+    if (a.propagator == PTY_ASP) {
+      d.Ik = a.ptychogram;  // natural order: the constraint runs in the row pass
+    } else {
+      if ((rc = dalloc(p, &mem, (size_t)a.F * np2 * sizeof(float)))) goto fail;
+      pty_permute_frames_kernel<<<1024, 256, 0, st>>>(a.ptychogram, (float*)mem, a.F, np, log2_np(np));
+      HIP_TRY(hipGetLastError());
+      d.Ik = (const float*)mem;
+    }
+    if (a.tf) {
+      if ((rc = dalloc(p, &mem, np2 * sizeof(float2)))) goto fail;
+      pty_permute_tf_kernel<<<256, 256, 0, st>>>((const float2*)a.tf, (float2*)mem, np, log2_np(np));
+      HIP_TRY(hipGetLastError());
+      d.HT = (const float2*)mem;
+    }
+    for (int g = 0; g < G; ++g) bmap.push_back(make_int2(i, g));
+  }
+  {
+    p->nblocks = (int)bmap.size();
+    p->cooperative = maxG > 1;
+    void* mem = nullptr;
+    if ((rc = dalloc(p, &mem, sizeof(DevInst) * ninst))) goto fail;
+    p->d_insts = (DevInst*)mem;
+    if ((rc = dalloc(p, &mem, sizeof(int2) * bmap.size()))) goto fail;
+    p->d_bmap = (int2*)mem;
+    HIP_TRY(hipMemcpyAsync(p->d_insts, p->h_insts.data(), sizeof(DevInst) * ninst, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p->d_bmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice, st));
+    if (p->cooperative) {
+      int per_cu = 0, cus = 0, coop = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)pty_iterate_kernel, NTHREADS, 0));
+      HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+      HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, p->device));
+      if (!coop) {
+        rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
+        goto fail;
+      }
+      if (p->nblocks > per_cu * cus) {
+        rc = set_err(PTY_EINVAL, "%d workgroups exceed the %d that can be co-resident", p->nblocks, per_cu * cus);
+        goto fail;
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  *out = p;
+  return PTY_OK;
+fail:
+  (void)hipStreamSynchronize(st);
+  free_plan(p);
+  delete p;
+  return rc;
+}
+
+static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb, int32_t je, void* hip_stream);
+
+int pty_plan_run(pty_plan* p, int32_t it_begin, int32_t it_end, void* hip_stream) {
+  return plan_launch(p, it_begin, it_end, 0, 0x7fffffff, hip_stream);
+}
+
+int pty_plan_run_positions(pty_plan* p, int32_t it, int32_t j_begin, int32_t j_end, void* hip_stream) {
+  if (j_begin < 0 || j_end < j_begin) return set_err(PTY_EINVAL, "pty_plan_run_positions: bad position range");
+  if (j_end == j_begin) return PTY_OK;
+  return plan_launch(p, it, it + 1, j_begin, j_end, hip_stream);
+}
+
+static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb, int32_t je, void* hip_stream) {
+  if (!p) return set_err(PTY_EINVAL, "pty_plan_run: null plan");
+  if (it_begin < 0 || it_end < it_begin) return set_err(PTY_EINVAL, "pty_plan_run: bad iteration range");
+  for (const DevInst& d : p->h_insts)
+    if (it_end > d.a.n_iter) return set_err(PTY_EINVAL, "pty_plan_run: it_end %d > n_iter %d", it_end, d.a.n_iter);
+  if (it_end == it_begin) return PTY_OK;
+  hipStream_t st = (hipStream_t)hip_stream;
+  pty_reset_ctrl_kernel<<<p->ninst, 64, 0, st>>>(p->d_insts);
+  HIP_TRY(hipGetLastError());
+  const DevInst* di = p->d_insts;
+  const int2* bm = p->d_bmap;
+  int a0 = it_begin, a1 = it_end, b0 = jb, b1 = je;
+  if (p->cooperative) {
+    void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1};
+    HIP_TRY(hipLaunchCooperativeKernel((const void*)pty_iterate_kernel, dim3(p->nblocks), dim3(NTHREADS), args, 0, st));
+  } else {
+    pty_iterate_kernel<<<p->nblocks, NTHREADS, 0, st>>>(di, bm, a0, a1, b0, b1);
+    HIP_TRY(hipGetLastError());
+  }
+  return PTY_OK;
+}
+
+int pty_plan_check(pty_plan* p, void* hip_stream) {
+  if (!p) return set_err(PTY_EINVAL, "pty_plan_check: null plan");
+  hipStream_t st = (hipStream_t)hip_stream;
+  HIP_TRY(hipStreamSynchronize(st));
+  for (size_t i = 0; i < p->h_insts.size(); ++i) {
+    unsigned flag = 0;
+    HIP_TRY(hipMemcpy(&flag, p->h_insts[i].ctrl + 1, sizeof flag, hipMemcpyDeviceToHost));
+    if (flag) return set_err(PTY_ETIMEDOUT, "instance %zu: a cluster barrier timed out", i);
+  }
+  return PTY_OK;
+}
+
+int pty_plan_workgroups(const pty_plan* p, int32_t* total) {
+  if (!p || !total) return set_err(PTY_EINVAL, "pty_plan_workgroups: bad arguments");
+  *total = p->nblocks;
+  return PTY_OK;
+}
+
+int pty_plan_destroy(pty_plan* p) {
+  if (!p) return PTY_OK;
+  (void)hipDeviceSynchronize();
+  free_plan(p);
+  delete p;
+  return PTY_OK;
+}
+
+int pty_propagate(void* fields, int32_t nfields, int32_t Np, int32_t propagator, int32_t inverse, const void* tf,
+                  void* hip_stream) {
+  if (!fields || nfields < 1) return set_err(PTY_EINVAL, "pty_propagate: bad arguments");
+  if (propagator < PTY_FRAUNHOFER || propagator > PTY_ASP) return set_err(PTY_EINVAL, "pty_propagate: bad propagator");
+  if (propagator == PTY_ASP && !tf) return set_err(PTY_EINVAL, "pty_propagate: ASP needs a transfer function");
+  int rc = ensure_twiddles();
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)hip_stream;
+  float2* d = (float2*)fields;
+  const float2* h = (const float2*)tf;
+  switch (Np) {
+    case 16: return launch_propagate<16>(d, nfields, propagator, inverse, h, st);
+    case 32: return launch_propagate<32>(d, nfields, propagator, inverse, h, st);
+    case 64: return launch_propagate<64>(d, nfields, propagator, inverse, h, st);
+    case 128: return launch_propagate<128>(d, nfields, propagator, inverse, h, st);
+    case 256: return launch_propagate<256>(d, nfields, propagator, inverse, h, st);
+    case 512: return launch_propagate<512>(d, nfields, propagator, inverse, h, st);
+  }
+  return set_err(PTY_EINVAL, "pty_propagate: Np=%d unsupported (powers of two 16..512)", Np);
+}
+
+int pty_fft2(void* fields, int32_t nfields, int32_t Np, int32_t sign, void* hip_stream) {
+  if (!fields || nfields < 1 || (sign != 1 && sign != -1)) return set_err(PTY_EINVAL, "pty_fft2: bad arguments");
+  int rc = ensure_twiddles();
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)hip_stream;
+  float2* d = (float2*)fields;
+  switch (Np) {
+    case 16: return launch_fft2<16>(d, nfields, sign, st);
+    case 32: return launch_fft2<32>(d, nfields, sign, st);
+    case 64: return launch_fft2<64>(d, nfields, sign, st);
+    case 128: return launch_fft2<128>(d, nfields, sign, st);
+    case 256: return launch_fft2<256>(d, nfields, sign, st);
+    case 512: return launch_fft2<512>(d, nfields, sign, st);
+  }
+  return set_err(PTY_EINVAL, "pty_fft2: Np=%d unsupported (powers of two 16..512)", Np);
+}
+
+}  // extern "C"
