@@ -1,0 +1,190 @@
+"""Benchmark: position updates/s of the mPIE reconstruction loop on one MI355X.
+
+Workload (configuration 1 of the project's benchmark set): mPIE with the
+band-limited angular-spectrum propagator, complex64, wavelength 632.8 nm,
+dxo 4 um, z 50 mm, object No=1024 (smoothed random amplitude/phase), probe
+Np=256 with npsm=4 orthogonalised Hermite-Gaussian x aperture modes, 1000
+jittered Fermat-spiral positions, 1e5 photons peak with Poisson noise,
+frictionM 0.7 / feedbackM 0.3.  Synthetic data (no network).
+
+A step is one iteration: all 1000 positions in the engine's order, one
+persistent launch.  W warm-up steps, then K timed steps between
+synchronisations; value = K * F / elapsed.  The inputs are resident in HBM
+before the timed region.
+
+Extra fields: ``roofline`` for the persistent kernel (algorithmic bytes per
+launch / launch time from HIP events on the launch stream, against 8 TB/s),
+``cpu_baseline`` (the NumPy oracle timed on one host core over one full
+iteration of the same workload).
+"""
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "position updates/s; time to object NRMSE 1e-2, learned-init vs plain mPIE"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "pty_iterate_kernel_mpie_asp_np256_traffic.txt"
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-positions", type=int, default=1000)
+    ap.add_argument("--workgroups", type=int, default=0)
+    ap.add_argument("--F", type=int, default=1000)
+    return ap.parse_args()
+
+
+def bytes_per_position(Np, No, nl, no, npm, ns, momentum_fraction):
+    """Algorithmic HBM bytes of one position update: object patch and probe
+    read and written once (complex64), the measured frame read once, plus the
+    momentum step's 6 full-array passes at the triggered fraction."""
+    core = 16 * Np * Np * (nl * no * ns + nl * npm * ns) + 4 * Np * Np
+    mom = 48 * (nl * no * ns * No * No + nl * npm * ns * Np * Np) * momentum_fraction
+    return core + mom
+
+
+def read_traffic():
+    if not TRAFFIC_FILE.exists():
+        return None
+    txt = TRAFFIC_FILE.read_text()
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(workload, n_positions):
+    import oracle as O
+
+    w = workload
+    geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No, npsm=4)
+    np.random.seed(1)
+    obj = O.initial_guess(geo.shape_O, "ones", geo)
+    pr = O.initial_guess(geo.shape_P, "circ", geo)
+    run = O.OracleRun(geo, obj, pr, engine="mPIE", propagator="ASP", numIterations=1)
+    t0 = time.perf_counter()
+    run.reconstruct(positions_limit=n_positions)
+    dt = time.perf_counter() - t0
+    n = min(n_positions, w.ptychogram.shape[0])
+    return {
+        "value": n / dt,
+        "unit": "position updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy oracle, mPIE ASP npsm=4 Np=256 No=1024: first {n} positions of iteration 0 ({dt:.1f} s)",
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent reconstructions "
+              "shard across GPUs without communication, which this benchmark does not measure", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import ptylab_amd as P
+    from ptylab_amd import synthetic as S
+
+    if not torch.cuda.is_available():
+        print("no HIP GPU available", file=sys.stderr)
+        sys.exit(1)
+    t_gen = time.perf_counter()
+    w = S.cpm_asp_modes(Np=256, No=1024, F=args.F, npsm=4, seed=10)
+    print(f"[bench] workload generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    data = P.ExperimentalData()
+    for k, v in w.as_dict().items():
+        setattr(data, k, v)
+    data._setData()
+    params = P.Params()
+    params.propagatorType = "ASP"
+    params.positionOrder = "random"
+    rec = P.Reconstruction(data, params)
+    rec.No = w.No
+    rec.npsm = 4
+    np.random.seed(1)
+    rec.initializeObjectProbe()
+    eng = P.Engines.mPIE(rec, data, params, P.DummyMonitor())
+    eng.frictionM, eng.feedbackM = 0.7, 0.3
+    eng.workgroups = args.workgroups
+    K, W = args.steps, args.warmup
+    eng.numIterations = W + K
+    np.random.seed(1)
+    eng._prepareReconstruction()
+    plan, T = eng._begin()
+    stream = torch.cuda.current_stream()
+    F = data.numFrames
+    for it in range(W):
+        plan.run(it, it + 1)
+    plan.check()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    t0 = time.perf_counter()
+    for i in range(K):
+        ev[i][0].record(stream)
+        plan.run(W + i, W + i + 1)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    plan.check()
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    masks = eng._masks[W:W + K]
+    n_wg = plan.workgroups()
+    eng._end(W + K)
+    triggers = float(masks.sum()) / (K * F)
+    bpos = bytes_per_position(256, w.No, 1, 1, 4, 1, triggers)
+    mean_launch_s = float(np.mean(launch_ms)) / 1e3
+    achieved = bpos * F / mean_launch_s / 1e9
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC,
+        "value": K * F / elapsed,
+        "unit": "position updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "mPIE, ASP, npsm=4 modes, Np=256, No=1024, F=%d positions per step (configuration 1)" % F,
+            "positions_per_step": F,
+            "engine_workgroups": n_wg,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "pty_iterate_kernel",
+            "bytes_per_position": bpos,
+            "mean_launch_ms": mean_launch_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_positions)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -283,6 +283,9 @@ class OracleRun:
     error: List[float] = field(default_factory=list)
 
     def __post_init__(self):
+        """precision: "c64" (the reference's CPU dtype flow), "c64pure"
+        (complex64 everywhere, including e3PIE's slice transfer function, as
+        the reference's GPU path casts it), "c128" (double precision)."""
         g = self.geo
         c = np.complex128 if self.precision == "c128" else np.complex64
         self.cdtype = c
@@ -435,7 +438,12 @@ class OracleRun:
         sx = slice(col, col + Np)
         if not hasattr(self, "esw"):
             self.esw = self.probe.copy()  # e3PIE.py:77
-        H = self.H if self.precision == "c64" else self.H.astype(np.complex128)
+        if self.precision == "c64":
+            H = self.H  # complex128, as the reference's CPU path (e3PIE.py:61-69)
+        elif self.precision == "c64pure":
+            H = self.H.astype(np.complex64)  # the reference's GPU dtype policy (gpuUtils.py:62-70)
+        else:
+            H = self.H.astype(np.complex128)
         patch = self.object[..., sy, sx].copy()
         self.esw[:, :, :, 0] = patch[:, :, :, 0] * self.probe[:, :, :, 0]
         for s in range(1, nS):
