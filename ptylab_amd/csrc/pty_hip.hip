@@ -103,10 +103,14 @@ __host__ __device__ static int log2_np(int np) {
   return -1;
 }
 
+// Register capacity for fields of one line: a wave holds FC lines of L
+// complex values per field kind.
 template <int NP>
 __host__ __device__ constexpr int fcap() {
-  return NP <= 256 ? 8 : 4;
+  return NP <= 128 ? 8 : (NP == 256 ? 4 : 2);
 }
+
+static int fcap_rt(int np) { return np <= 128 ? 8 : (np == 256 ? 4 : 2); }
 
 struct Run {
   const DevInst* D;
@@ -133,7 +137,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block-wide FP64 sum; result valid in thread 0.
-__device__ double block_sum(Run& R, double v) {
+__device__ __forceinline__ double block_sum(Run& R, double v) {
   v = wave_sum(v);
   __syncthreads();
   if (R.lane == 0) R.red[R.wave] = v;
@@ -152,20 +156,24 @@ __device__ __forceinline__ float load_slot(const unsigned* p) {
   return __uint_as_float(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// Every wave writes back its stores (agent-scope release fence: buffer_wbl2 +
-// s_waitcnt vmcnt(0)) before the workgroup barrier, so thread 0's counter
-// increment is ordered after all stores of the workgroup; after the counter
-// reaches the generation target every wave invalidates its caches
-// (agent-scope acquire fence) before reading data written by other
-// workgroups, which may sit on another XCD.  The spin is bounded: on time-out
-// the failure flag is raised and every workgroup of the instance leaves.
-__device__ bool cluster_sync(Run& R) {
-  if (R.G == 1) {
-    __syncthreads();
-    return true;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+// Cross-workgroup data of the persistent kernel (exit waves, object, probe,
+// momenta, maxima, error partials) is only ever read and written with
+// agent-scope coherent accesses (relaxed atomics: global_load/store ... sc1),
+// so no cache holds a stale copy of it.  A barrier then needs no cache
+// maintenance: every wave drains its outstanding stores (s_waitcnt
+// vmcnt(0): the stores have been performed at the coherence point), the
+// workgroup barrier orders that before thread 0's counter increment, and
+// once the counter reaches the generation target the workgroup proceeds.
+// This avoids the L2 write-back/invalidate of agent-scope fences, whose cost
+// grows with the number of workgroups per XCD.  The spin is bounded: on
+// time-out the failure flag is raised and every workgroup of the instance
+// leaves.
+__device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
+
+__device__ __forceinline__ bool cluster_sync(Run& R) {
+  drain_stores();
   __syncthreads();
+  if (R.G == 1) return true;
   R.gen += 1u;
   if (threadIdx.x == 0) {
     unsigned* ctr = R.D->ctrl;
@@ -186,32 +194,59 @@ __device__ bool cluster_sync(Run& R) {
     *R.fail = fail;
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return *R.fail == 0;
+}
+
+// coherent (agent-scope) accessors for cross-workgroup data
+__device__ __forceinline__ float2 cload(const float2* p) {
+  const uint64_t u = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_bit_cast(float2, u);
+}
+__device__ __forceinline__ void cstore(float2* p, float2 v) {
+  __hip_atomic_store((uint64_t*)p, __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double cloadd(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void cstored(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void cstoreu(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
 // line helpers
 // ---------------------------------------------------------------------------
+// Global-address-space views: pointers read from the descriptor are generic
+// pointers to the compiler; casting them back lets it emit global (not flat)
+// memory instructions.
+#define GAS __attribute__((address_space(1)))
+
 template <int NP>
-__device__ __forceinline__ void load_row(float2 (&v)[Line<NP>::L], const float2* __restrict__ row, int t) {
+__device__ __forceinline__ void load_row(float2 (&v)[Line<NP>::L], const float2* row, int t) {
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) v[l] = row[l * Line<NP>::LPF + t];
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = cload(row + l * Line<NP>::LPF + t);
 }
 template <int NP>
-__device__ __forceinline__ void store_row(float2* __restrict__ row, const float2 (&v)[Line<NP>::L], int t) {
+__device__ __forceinline__ void store_row(float2* row, const float2 (&v)[Line<NP>::L], int t) {
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) row[l * Line<NP>::LPF + t] = v[l];
+  for (int l = 0; l < Line<NP>::L; ++l) cstore(row + l * Line<NP>::LPF + t, v[l]);
 }
 template <int NP>
-__device__ __forceinline__ void load_col(float2 (&v)[Line<NP>::L], const float2* __restrict__ f, int p, int t) {
+__device__ __forceinline__ void load_col(float2 (&v)[Line<NP>::L], const float2* f, int p, int t) {
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) v[l] = f[(size_t)(l * Line<NP>::LPF + t) * NP + p];
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = cload(f + (size_t)(l * Line<NP>::LPF + t) * NP + p);
 }
 template <int NP>
-__device__ __forceinline__ void store_col(float2* __restrict__ f, const float2 (&v)[Line<NP>::L], int p, int t) {
+__device__ __forceinline__ void store_col(float2* f, const float2 (&v)[Line<NP>::L], int p, int t) {
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) f[(size_t)(l * Line<NP>::LPF + t) * NP + p] = v[l];
+  for (int l = 0; l < Line<NP>::L; ++l) cstore(f + (size_t)(l * Line<NP>::LPF + t) * NP + p, v[l]);
+}
+// read-only inputs (measured frames, transfer function): ordinary cached loads
+__device__ __forceinline__ float ldg(const float* p) { return *(const GAS float*)p; }
+__device__ __forceinline__ float2 ldg2(const float2* p) {
+  return __builtin_bit_cast(float2, *(const GAS unsigned long long*)p);
 }
 
 // Lines [line0, line0 + per) of this workgroup are visited SUB at a time by
@@ -231,37 +266,70 @@ __device__ __forceinline__ void store_col(float2* __restrict__ f, const float2 (
   }
 
 // ---------------------------------------------------------------------------
-// ePIE / mPIE / multiPIE position passes
+// field bookkeeping
+// ---------------------------------------------------------------------------
+// Exit-wave field f of the PIE engines enumerates (lambda, osm, psm, slice) in
+// C order; its object field is (lambda, osm, slice) and its probe field
+// (lambda, psm, slice), matching the 6D layouts.
+struct FieldIdx {
+  int il, io, ip, is, fo, fp;
+};
+__device__ __forceinline__ FieldIdx field_idx(int f, int no, int npm, int ns) {
+  FieldIdx x;
+  x.is = f % ns;
+  x.ip = (f / ns) % npm;
+  x.io = (f / (ns * npm)) % no;
+  x.il = f / (ns * npm * no);
+  x.fo = (x.il * no + x.io) * ns + x.is;
+  x.fp = (x.il * npm + x.ip) * ns + x.is;
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// ePIE / mPIE / multiPIE position passes.  Every pass first issues the loads
+// of all fields of its line, then computes, so a pass costs one memory
+// latency per line instead of one per field.
 // ---------------------------------------------------------------------------
 template <int NP>
-__device__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+__device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
-  const int nl = D.a.nlambda, no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
-  const int No = D.a.No;
+  const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No, nF = D.nF;
   const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
   const float2* __restrict__ O = (const float2*)D.a.object;
   const float2* __restrict__ P = (const float2*)D.a.probe;
   float2* __restrict__ W = D.W;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER;
-  if (R.g == 0 && threadIdx.x < 32) zslot[threadIdx.x] = 0u;
+  if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
   float pmax = 0.f, omax = 0.f;
   LINES_BEGIN(R, NP, r, act)
+    // per exit-wave field: its object row and its probe row
+    float2 of[FC][L], pf[FC][L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        const FieldIdx x = field_idx(f, no, npm, ns);
+        load_row<NP>(of[f], O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
+        load_row<NP>(pf[f], P + x.fp * Np2 + (size_t)r * NP, R.t);
+      }
+    }
+    // sum over every probe field (the fields with osm index 0 enumerate them
+    // once, in increasing order) and over every object field (psm index 0)
     float sp[L], so[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) sp[l] = so[l] = 0.f;
-    for (int fp = 0; fp < D.nPr; ++fp) {
-      float2 v[L];
-      load_row<NP>(v, P + fp * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
-      for (int l = 0; l < L; ++l) sp[l] += cabs2(v[l]);
-    }
-    for (int fo = 0; fo < D.nO; ++fo) {
-      float2 v[L];
-      load_row<NP>(v, O + fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        const FieldIdx x = field_idx(f, no, npm, ns);
 #pragma unroll
-      for (int l = 0; l < L; ++l) so[l] += cabs2(v[l]);
+        for (int l = 0; l < L; ++l) {
+          if (x.io == 0) sp[l] += cabs2(pf[f][l]);
+          if (x.ip == 0) so[l] += cabs2(of[f][l]);
+        }
+      }
     }
     if (act) {
 #pragma unroll
@@ -270,24 +338,20 @@ __device__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
         omax = fmaxf(omax, so[l]);
       }
     }
-    for (int il = 0; il < nl; ++il)
-      for (int io = 0; io < no; ++io)
-        for (int ip = 0; ip < npm; ++ip)
-          for (int is = 0; is < ns; ++is) {
-            const int f = ((il * no + io) * npm + ip) * ns + is;
-            const float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + (size_t)(row0 + r) * No + col0;
-            const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + (size_t)r * NP;
-            float2 v[L];
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-              const int x = l * LPF + R.t;
-              float2 e = cmul(orow[x], prow[x]);
-              if (cb && ((r + x) & 1)) e = make_float2(-e.x, -e.y);
-              v[l] = e;
-            }
-            fft_fwd<NP>(v, R.tw, R.t);
-            if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
-          }
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        float2 v[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          float2 e = cmul(of[f][l], pf[f][l]);
+          if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
+          v[l] = e;
+        }
+        fft_fwd<NP>(v, R.tw, R.t);
+        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+      }
+    }
   LINES_END
   pmax = wave_max(pmax);
   omax = wave_max(omax);
@@ -302,7 +366,7 @@ __device__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
 // Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
 // BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
 template <int NP>
-__device__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int nF, int fs) {
+__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int nF, int fs) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -312,13 +376,18 @@ __device__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int n
   double err = 0.0;
   LINES_BEGIN(R, NP, p, act)
     float2 v[FC][L];
+    float im[L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f)
+      if (f < nF) load_col<NP>(v[f], W + f * Np2, p, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)p * NP + l * LPF + R.t);
     float iest[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) iest[l] = 0.f;
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
       if (f < nF) {
-        load_col<NP>(v[f], W + f * Np2, p, R.t);
         fft_fwd<NP>(v[f], R.tw, R.t);
 #pragma unroll
         for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], invN);
@@ -331,9 +400,8 @@ __device__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int n
     float amp[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      const float im = Ik[(size_t)p * NP + l * LPF + R.t];
-      if (act) err += (double)fabsf(im - iest[l]);
-      amp[l] = sqrtf(im / (iest[l] + 1e-10f));
+      if (act) err += (double)fabsf(im[l] - iest[l]);
+      amp[l] = sqrtf(im[l] / (iest[l] + 1e-10f));
     }
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
@@ -346,34 +414,39 @@ __device__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int n
     }
   LINES_END
   const double s = block_sum(R, err);
-  if (threadIdx.x == 0) ep[(size_t)j * R.G + R.g] = s;
+  if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
 }
 
 // Column pass of an angular-spectrum propagation: forward column transform,
 // multiply by the (transposed, bit-reversed, 1/Np^2-scaled) transfer function
 // or its conjugate, inverse column transform.
 template <int NP>
-__device__ void tf_cols(Run& R, int nF, bool conj) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+__device__ __forceinline__ void tf_cols(Run& R, int nF, bool conj) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float2* __restrict__ HT = D.HT;
   LINES_BEGIN(R, NP, p, act)
     float2 h[L];
+    float2 v[FC][L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f)
+      if (f < nF) load_col<NP>(v[f], W + f * Np2, p, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      h[l] = HT[(size_t)p * NP + l * LPF + R.t];
+      h[l] = ldg2(HT + (size_t)p * NP + l * LPF + R.t);
       if (conj) h[l] = conjf2(h[l]);
     }
-    for (int f = 0; f < nF; ++f) {
-      float2 v[L];
-      load_col<NP>(v, W + f * Np2, p, R.t);
-      fft_fwd<NP>(v, R.tw, R.t);
 #pragma unroll
-      for (int l = 0; l < L; ++l) v[l] = cmul(v[l], h[l]);
-      fft_inv<NP>(v, R.tw, R.t);
-      if (act) store_col<NP>(W + f * Np2, v, p, R.t);
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        fft_fwd<NP>(v[f], R.tw, R.t);
+#pragma unroll
+        for (int l = 0; l < L; ++l) v[f][l] = cmul(v[f][l], h[l]);
+        fft_inv<NP>(v[f], R.tw, R.t);
+        if (act) store_col<NP>(W + f * Np2, v[f], p, R.t);
+      }
     }
   LINES_END
 }
@@ -382,7 +455,7 @@ __device__ void tf_cols(Run& R, int nF, bool conj) {
 // intensity constraint is applied in natural order, then the forward row
 // transform starts the back-propagation.
 template <int NP>
-__device__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
+__device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -392,13 +465,18 @@ __device__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
   double err = 0.0;
   LINES_BEGIN(R, NP, r, act)
     float2 v[FC][L];
+    float im[L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f)
+      if (f < nF) load_row<NP>(v[f], W + f * Np2 + (size_t)r * NP, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
     float iest[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) iest[l] = 0.f;
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
       if (f < nF) {
-        load_row<NP>(v[f], W + f * Np2 + (size_t)r * NP, R.t);
         fft_inv<NP>(v[f], R.tw, R.t);
         if (f % fs == fs - 1) {
 #pragma unroll
@@ -409,9 +487,8 @@ __device__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
     float amp[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      const float im = Ik[(size_t)r * NP + l * LPF + R.t];
-      if (act) err += (double)fabsf(im - iest[l]);
-      amp[l] = sqrtf(im / (iest[l] + 1e-10f));
+      if (act) err += (double)fabsf(im[l] - iest[l]);
+      amp[l] = sqrtf(im[l] / (iest[l] + 1e-10f));
     }
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
@@ -424,19 +501,23 @@ __device__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
     }
   LINES_END
   const double s = block_sum(R, err);
-  if (threadIdx.x == 0) ep[(size_t)j * R.G + R.g] = s;
+  if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
 }
 
 // Last row pass: inverse row transform gives the updated exit waves, the
 // difference to the exit waves of the same position, and the engine's
-// object-patch and probe updates (both from the pre-update patch and probe),
-// written back in place.  Each object pixel of the patch has one owner.
+// object-patch and probe updates (both from the pre-update patch and probe,
+// held in registers), written back in place.  Each object pixel of the patch
+// has one owner.  Sums run over the fields in increasing order, which is the
+// C order NumPy's axis sums use.  Every object field is written by the
+// exit-wave field that carries it with psm index 0, every probe field by the
+// one with osm index 0.
 template <int NP>
-__device__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
+__device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
-  const int nl = D.a.nlambda, no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
-  const int No = D.a.No;
+  const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No, nF = D.nF;
   const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
   float2* __restrict__ O = (float2*)D.a.object;
@@ -451,178 +532,139 @@ __device__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
   LINES_BEGIN(R, NP, r, act)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
-    float2 d[FC][L];
+    float2 d[FC][L], of[FC][L], pf[FC][L];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
-      if (f < D.nF) {
+      if (f < nF) {
+        const FieldIdx x = field_idx(f, no, npm, ns);
         load_row<NP>(d[f], W + f * Np2 + prow_off, R.t);
+        load_row<NP>(of[f], O + x.fo * No2 + orow_off, R.t);
+        load_row<NP>(pf[f], P + x.fp * Np2 + prow_off, R.t);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
         fft_inv<NP>(d[f], R.tw, R.t);
-        const int is = f % ns;
-        const int ip = (f / ns) % npm;
-        const int io = (f / (ns * npm)) % no;
-        const int il = f / (ns * npm * no);
-        const float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
-        const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-          const int x = l * LPF + R.t;
           float2 u = d[f][l];
           if (prop != PTY_ASP) {
             u = cscale(u, invN);
-            if (cb && ((r + x) & 1)) u = make_float2(-u.x, -u.y);
+            if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
           }
-          d[f][l] = csub(u, cmul(orow[x], prow[x]));
+          d[f][l] = csub(u, cmul(of[f][l], pf[f][l]));
         }
       }
     }
     if (engine == PTY_EPIE) {
-      // ePIE.py:128-160: P/max(sum|P|^2) and O/max(sum|O|^2) weights; object
-      // increment summed over (lambda, psm, slice), probe over (lambda, osm, slice)
+      // ePIE.py:128-160: weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
+      // object increment summed over (lambda, psm, slice) -> one per osm,
+      // probe increment over (lambda, osm, slice) -> one per psm
       const float invP = 1.0f / Pmax, invO = 1.0f / Omax;
-      float2 accP[FC][L];
 #pragma unroll
-      for (int ip = 0; ip < FC; ++ip) {
-        if (ip < npm) {
+      for (int f0 = 0; f0 < FC; ++f0) {
+        if (f0 < nF) {
+          const FieldIdx x0 = field_idx(f0, no, npm, ns);
+          if (x0.ip == 0) {
+            float2 acc[L];
 #pragma unroll
-          for (int l = 0; l < L; ++l) accP[ip][l] = make_float2(0.f, 0.f);
-          for (int il = 0; il < nl; ++il)
-            for (int io = 0; io < no; ++io)
-              for (int is = 0; is < ns; ++is) {
-                const float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
-                const int f = ((il * no + io) * npm + ip) * ns + is;
+            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int f = 0; f < FC; ++f) {
+              if (f < nF && field_idx(f, no, npm, ns).io == x0.io) {
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
-                  const float2 o = orow[l * LPF + R.t];
-                  const float2 fr = make_float2(o.x * invO, -o.y * invO);
-                  float2 dd = make_float2(0.f, 0.f);
-#pragma unroll
-                  for (int ff = 0; ff < FC; ++ff)
-                    if (ff == f) dd = d[ff][l];
-                  accP[ip][l] = cadd(accP[ip][l], cmul(fr, dd));
+                  const float2 pq = pf[f][l];
+                  acc[l] = cadd(acc[l], cmul(make_float2(pq.x * invP, -pq.y * invP), d[f][l]));
                 }
               }
-        }
-      }
-      for (int io = 0; io < no; ++io) {
-        float2 accO[L];
+            }
+            float2 nv[L];
 #pragma unroll
-        for (int l = 0; l < L; ++l) accO[l] = make_float2(0.f, 0.f);
-        for (int il = 0; il < nl; ++il)
-          for (int ip = 0; ip < npm; ++ip)
-            for (int is = 0; is < ns; ++is) {
-              const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
-              const int f = ((il * no + io) * npm + ip) * ns + is;
+            for (int l = 0; l < L; ++l) nv[l] = cadd(of[f0][l], cscale(acc[l], bO));
+            if (act) store_row<NP>(O + x0.fo * No2 + orow_off, nv, R.t);
+          }
+          if (x0.io == 0) {
+            float2 acc[L];
 #pragma unroll
-              for (int l = 0; l < L; ++l) {
-                const float2 pv = prow[l * LPF + R.t];
-                const float2 fr = make_float2(pv.x * invP, -pv.y * invP);
-                float2 dd = make_float2(0.f, 0.f);
+            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
 #pragma unroll
-                for (int ff = 0; ff < FC; ++ff)
-                  if (ff == f) dd = d[ff][l];
-                accO[l] = cadd(accO[l], cmul(fr, dd));
+            for (int f = 0; f < FC; ++f) {
+              if (f < nF && field_idx(f, no, npm, ns).ip == x0.ip) {
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                  const float2 oq = of[f][l];
+                  acc[l] = cadd(acc[l], cmul(make_float2(oq.x * invO, -oq.y * invO), d[f][l]));
+                }
               }
             }
-        for (int il = 0; il < nl; ++il)
-          for (int is = 0; is < ns; ++is) {
-            float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
+            float2 nv[L];
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-              const int x = l * LPF + R.t;
-              const float2 nv = cadd(orow[x], cscale(accO[l], bO));
-              if (act) orow[x] = nv;
-            }
+            for (int l = 0; l < L; ++l) nv[l] = cadd(pf[f0][l], cscale(acc[l], bP));
+            if (act) store_row<NP>(P + x0.fp * Np2 + prow_off, nv, R.t);
           }
-      }
-      for (int ip = 0; ip < npm; ++ip) {
-        float2 a[L];
-#pragma unroll
-        for (int ipp = 0; ipp < FC; ++ipp)
-          if (ipp == ip) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) a[l] = accP[ipp][l];
-          }
-        for (int il = 0; il < nl; ++il)
-          for (int is = 0; is < ns; ++is) {
-            float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-              const int x = l * LPF + R.t;
-              const float2 nv = cadd(prow[x], cscale(a[l], bP));
-              if (act) prow[x] = nv;
-            }
-          }
+        }
       }
     } else {
       // mPIE.py:226-270 / multiPIE.py:153-205 (CPM): regularised weights
-      // conj(P)/(aO*Pmax + (1-aO)|P|^2) and conj(O)/(aP*Omax + (1-aP)|O|^2)
+      // conj(P)/(aO Pmax + (1-aO)|P|^2) and conj(O)/(aP Omax + (1-aP)|O|^2)
       const float aPm = D.a.alphaObject * Pmax;
       const float aOm = D.a.alphaProbe * Omax;
       const float omO = D.omAlphaO, omP = D.omAlphaP;
       const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
       const bool sumLambda = engine == PTY_MULTIPIE;
-      float2 newP[FC][L];
 #pragma unroll
-      for (int fp = 0; fp < FC; ++fp) {
-        if (fp < D.nPr) {
-          const int is = fp % ns, ip = (fp / ns) % npm, il = fp / (ns * npm);
-          float2 acc[L];
-#pragma unroll
-          for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-          const int l0 = sumLambda ? 0 : il, l1 = sumLambda ? nl : il + 1;
-          for (int jl = l0; jl < l1; ++jl)
-            for (int io = 0; io < no; ++io) {
-              const float2* orow = O + (size_t)((jl * no + io) * ns + is) * No2 + orow_off;
-              const int f = ((jl * no + io) * npm + ip) * ns + is;
-#pragma unroll
-              for (int l = 0; l < L; ++l) {
-                const float2 o = orow[l * LPF + R.t];
-                const float inv = 1.0f / (aOm + omP * cabs2(o));
-                const float2 fr = make_float2(o.x * inv, -o.y * inv);
-                float2 dd = make_float2(0.f, 0.f);
-#pragma unroll
-                for (int ff = 0; ff < FC; ++ff)
-                  if (ff == f) dd = d[ff][l];
-                acc[l] = cadd(acc[l], cmul(fr, dd));
-              }
-            }
-          const float2* prow = P + (size_t)fp * Np2 + prow_off;
-#pragma unroll
-          for (int l = 0; l < L; ++l) newP[fp][l] = cadd(prow[l * LPF + R.t], cscale(acc[l], stepP));
-        }
-      }
-      for (int il = 0; il < nl; ++il)
-        for (int io = 0; io < no; ++io)
-          for (int is = 0; is < ns; ++is) {
-            float2* orow = O + (size_t)((il * no + io) * ns + is) * No2 + orow_off;
+      for (int f0 = 0; f0 < FC; ++f0) {
+        if (f0 < nF) {
+          const FieldIdx x0 = field_idx(f0, no, npm, ns);
+          if (x0.ip == 0) {
+            // object field x0.fo: sum over psm (axis 2)
             float2 acc[L];
 #pragma unroll
             for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-            for (int ip = 0; ip < npm; ++ip) {
-              const float2* prow = P + (size_t)((il * npm + ip) * ns + is) * Np2 + prow_off;
-              const int f = ((il * no + io) * npm + ip) * ns + is;
 #pragma unroll
-              for (int l = 0; l < L; ++l) {
-                const float2 pv = prow[l * LPF + R.t];
-                const float inv = 1.0f / (aPm + omO * cabs2(pv));
-                const float2 fr = make_float2(pv.x * inv, -pv.y * inv);
-                float2 dd = make_float2(0.f, 0.f);
+            for (int f = 0; f < FC; ++f) {
+              if (f < nF && field_idx(f, no, npm, ns).fo == x0.fo) {
 #pragma unroll
-                for (int ff = 0; ff < FC; ++ff)
-                  if (ff == f) dd = d[ff][l];
-                acc[l] = cadd(acc[l], cmul(fr, dd));
+                for (int l = 0; l < L; ++l) {
+                  const float2 pq = pf[f][l];
+                  const float inv = 1.0f / (aPm + omO * cabs2(pq));
+                  acc[l] = cadd(acc[l], cmul(make_float2(pq.x * inv, -pq.y * inv), d[f][l]));
+                }
               }
             }
+            float2 nv[L];
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-              const int x = l * LPF + R.t;
-              const float2 nv = cadd(orow[x], cscale(acc[l], bO));
-              if (act) orow[x] = nv;
-            }
+            for (int l = 0; l < L; ++l) nv[l] = cadd(of[f0][l], cscale(acc[l], bO));
+            if (act) store_row<NP>(O + x0.fo * No2 + orow_off, nv, R.t);
           }
+          if (x0.io == 0) {
+            // probe field x0.fp: mPIE sums over osm (axis 1), multiPIE over
+            // (lambda, osm) (axes (0, 1))
+            float2 acc[L];
 #pragma unroll
-      for (int fp = 0; fp < FC; ++fp) {
-        if (fp < D.nPr && act) store_row<NP>(P + (size_t)fp * Np2 + prow_off, newP[fp], R.t);
+            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int f = 0; f < FC; ++f) {
+              if (f < nF) {
+                const FieldIdx x = field_idx(f, no, npm, ns);
+                if (x.ip == x0.ip && x.is == x0.is && (sumLambda || x.il == x0.il)) {
+#pragma unroll
+                  for (int l = 0; l < L; ++l) {
+                    const float2 oq = of[f][l];
+                    const float inv = 1.0f / (aOm + omP * cabs2(oq));
+                    acc[l] = cadd(acc[l], cmul(make_float2(oq.x * inv, -oq.y * inv), d[f][l]));
+                  }
+                }
+              }
+            }
+            float2 nv[L];
+#pragma unroll
+            for (int l = 0; l < L; ++l) nv[l] = cadd(pf[f0][l], cscale(acc[l], stepP));
+            if (act) store_row<NP>(P + x0.fp * Np2 + prow_off, nv, R.t);
+          }
+        }
       }
     }
   LINES_END
@@ -630,8 +672,8 @@ __device__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
 
 // mPIE / multiPIE momentum step over the whole object and probe
 // (mPIE.py:196-224, multiPIE.py:125-151); each workgroup owns a contiguous
-// chunk, so the step is atomic-free.
-__device__ void momentum_pass(Run& R) {
+// chunk, so the step needs no read-modify-write atomics.
+__device__ __forceinline__ void momentum_pass(Run& R) {
   const DevInst& D = *R.D;
   const float eta = D.a.momFriction, nu = D.a.momFeedback;
   for (int which = 0; which < 2; ++which) {
@@ -643,25 +685,25 @@ __device__ void momentum_pass(Run& R) {
     const size_t b = chunk * R.g;
     const size_t e = b + chunk < n ? b + chunk : n;
     for (size_t i = b + threadIdx.x; i < e; i += NTHREADS) {
-      const float2 x = X[i], bb = B[i], m = M[i];
-      const float2 grad = csub(bb, x);
-      const float2 mn = cadd(grad, cscale(m, eta));
-      const float2 xn = csub(x, cscale(mn, nu));
-      M[i] = mn;
-      X[i] = xn;
-      B[i] = xn;
+      const float2 x = cload(X + i), bb = cload(B + i), m = cload(M + i);
+      const float2 mn = make_float2((bb.x - x.x) + m.x * eta, (bb.y - x.y) + m.y * eta);
+      const float2 xn = make_float2(x.x - mn.x * nu, x.y - mn.y * nu);
+      cstore(M + i, mn);
+      cstore(X + i, xn);
+      cstore(B + i, xn);
     }
   }
 }
 
+
 // ---------------------------------------------------------------------------
 // e3PIE slice chain (e3PIE.py:89-173, updates :192-225).  W holds the
 // (lambda, psm) fields of the slice being propagated; ESW keeps the exit
-// wave of every slice of the current position.
+// wave of every slice of the current position.  Field f = il * npsm + ip.
 // ---------------------------------------------------------------------------
 template <int NP>
-__device__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+__device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
   const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
   const int No = D.a.No;
@@ -673,13 +715,21 @@ __device__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
   float2* __restrict__ E = D.ESW;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER && ns == 1;
   const int nF = D.nF;
-  if (R.g == 0 && threadIdx.x < 32) zslot[threadIdx.x] = 0u;
+  if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
   float omx[MAX_SLICES];
 #pragma unroll
   for (int s = 0; s < MAX_SLICES; ++s) omx[s] = 0.f;
   float pmax = 0.f;
   LINES_BEGIN(R, NP, r, act)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
+    float2 of[FC][L], pv[FC][L];
+#pragma unroll
+    for (int q = 0; q < FC; ++q) {
+      if (q < nF) {
+        load_row<NP>(of[q], O + (size_t)((q / npm) * ns + 0) * No2 + orow_off, R.t);
+        load_row<NP>(pv[q], P + (size_t)(q * ns + 0) * Np2 + (size_t)r * NP, R.t);
+      }
+    }
 #pragma unroll
     for (int s = 0; s < MAX_SLICES; ++s) {
       if (s < ns) {
@@ -689,7 +739,7 @@ __device__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
         for (int il = 0; il < nl; ++il) {
           const float2* orow = O + (size_t)(il * ns + s) * No2 + orow_off;
 #pragma unroll
-          for (int l = 0; l < L; ++l) so[l] += cabs2(orow[l * LPF + R.t]);
+          for (int l = 0; l < L; ++l) so[l] += cabs2(cload(orow + l * LPF + R.t));
         }
         if (act) {
 #pragma unroll
@@ -700,33 +750,29 @@ __device__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
     float sp[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) sp[l] = 0.f;
-    for (int f = 0; f < nF; ++f) {  // f = il * npsm + ip
-      const float2* prow = P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP;
 #pragma unroll
-      for (int l = 0; l < L; ++l) sp[l] += cabs2(prow[l * LPF + R.t]);
-    }
+    for (int q = 0; q < FC; ++q)
+      if (q < nF) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) sp[l] += cabs2(pv[q][l]);
+      }
     if (act) {
 #pragma unroll
       for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
     }
-    for (int f = 0; f < nF; ++f) {
-      const int il = f / npm;
-      const float2* orow = O + (size_t)(il * ns + 0) * No2 + orow_off;
-      const float2* prow = P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP;
-      float2 v[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const int x = l * LPF + R.t;
-        v[l] = cmul(orow[x], prow[x]);
-      }
-      if (act) store_row<NP>(E + (size_t)(0 * nF + f) * Np2 + (size_t)r * NP, v, R.t);
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        float2 v[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const int x = l * LPF + R.t;
-        if (cb && ((r + x) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+        for (int l = 0; l < L; ++l) v[l] = cmul(of[f][l], pv[f][l]);
+        if (act) store_row<NP>(E + (size_t)f * Np2 + (size_t)r * NP, v, R.t);
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+        fft_fwd<NP>(v, R.tw, R.t);
+        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
       }
-      fft_fwd<NP>(v, R.tw, R.t);
-      if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
     }
   LINES_END
   pmax = wave_max(pmax);
@@ -745,10 +791,10 @@ __device__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
 // then the forward row transform of esw_s (with the detector checkerboard
 // when s is the last slice).
 template <int NP>
-__device__ void e3_rows_forward(Run& R, int k, int s, unsigned* slot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+__device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* slot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
-  const int npm = D.a.npsm, ns = D.a.nslice;
+  const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
   const int No = D.a.No;
   const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
@@ -761,30 +807,32 @@ __device__ void e3_rows_forward(Run& R, int k, int s, unsigned* slot) {
   float pmax = 0.f;
   LINES_BEGIN(R, NP, r, act)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
+    float2 v[FC][L], os[FC][L];
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) load_row<NP>(v[f], W + f * Np2 + (size_t)r * NP, R.t);
+      if (f < nF) load_row<NP>(os[f], O + (size_t)((f / npm) * ns + s) * No2 + orow_off, R.t);
+    }
     float sp[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) sp[l] = 0.f;
-    for (int f = 0; f < nF; ++f) {
-      const int il = f / npm;
-      float2 v[L];
-      load_row<NP>(v, W + f * Np2 + (size_t)r * NP, R.t);
-      fft_inv<NP>(v, R.tw, R.t);
-      if (act) store_row<NP>(P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v, R.t);
-      const float2* orow = O + (size_t)(il * ns + s) * No2 + orow_off;
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const int x = l * LPF + R.t;
-        sp[l] += cabs2(v[l]);
-        v[l] = cmul(v[l], orow[x]);
-      }
-      if (act) store_row<NP>(E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v, R.t);
+    for (int f = 0; f < FC; ++f) {
+      if (f < nF) {
+        fft_inv<NP>(v[f], R.tw, R.t);
+        if (act) store_row<NP>(P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v[f], R.t);
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const int x = l * LPF + R.t;
-        if (cb && ((r + x) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+        for (int l = 0; l < L; ++l) {
+          sp[l] += cabs2(v[f][l]);
+          v[f][l] = cmul(v[f][l], os[f][l]);
+        }
+        if (act) store_row<NP>(E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v[f], R.t);
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          if (cb && ((r + l * LPF + R.t) & 1)) v[f][l] = make_float2(-v[f][l].x, -v[f][l].y);
+        fft_fwd<NP>(v[f], R.tw, R.t);
+        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v[f], R.t);
       }
-      fft_fwd<NP>(v, R.tw, R.t);
-      if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
     }
     if (act) {
 #pragma unroll
@@ -801,7 +849,7 @@ __device__ void e3_rows_forward(Run& R, int k, int s, unsigned* slot) {
 // Object slice su and probe slice su are rewritten; for su >= 1 the updated
 // probe slice is forward-transformed into W for the next back-propagation.
 template <int NP>
-__device__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool fromDetector) {
+__device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool fromDetector) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
   const DevInst& D = *R.D;
   const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
@@ -823,54 +871,33 @@ __device__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool
   LINES_BEGIN(R, NP, r, act)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
-    float2 d[FC][L];
+    float2 d[FC][L], pv[FC][L], ov[FC][L], ev[FC][L];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
       if (f < nF) {
         load_row<NP>(d[f], W + f * Np2 + prow_off, R.t);
-        fft_inv<NP>(d[f], R.tw, R.t);
-        const float2* erow = E + (size_t)(esl * nF + f) * Np2 + prow_off;
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          const int x = l * LPF + R.t;
-          float2 u = d[f][l];
-          if (fromDetector) {
-            u = cscale(u, invN);
-            if (cb && ((r + x) & 1)) u = make_float2(-u.x, -u.y);
-          }
-          d[f][l] = csub(u, erow[x]);
-        }
+        load_row<NP>(pv[f], P + (size_t)(f * ns + su) * Np2 + prow_off, R.t);
+        load_row<NP>(ev[f], E + (size_t)(esl * nF + f) * Np2 + prow_off, R.t);
       }
+      if (f < nF) load_row<NP>(ov[f], O + (size_t)((f / npm) * ns + su) * No2 + orow_off, R.t);
     }
-    // probe increments: sum over lambda of conj(O_su)/Omax * delta (axes (0,1))
-    float2 newP[FC][L];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
       if (f < nF) {
-        const int ip = f % npm;
-        float2 acc[L];
+        fft_inv<NP>(d[f], R.tw, R.t);
 #pragma unroll
-        for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-        for (int jl = 0; jl < nl; ++jl) {
-          const float2* orow = O + (size_t)(jl * ns + su) * No2 + orow_off;
-          const int g = jl * npm + ip;
-#pragma unroll
-          for (int l = 0; l < L; ++l) {
-            const float2 o = orow[l * LPF + R.t];
-            const float2 fr = make_float2(o.x * invO, -o.y * invO);
-            float2 dd = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int ff = 0; ff < FC; ++ff)
-              if (ff == g) dd = d[ff][l];
-            acc[l] = cadd(acc[l], cmul(fr, dd));
+        for (int l = 0; l < L; ++l) {
+          float2 u = d[f][l];
+          if (fromDetector) {
+            u = cscale(u, invN);
+            if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
           }
+          d[f][l] = csub(u, ev[f][l]);
         }
-        const float2* prow = P + (size_t)(f * ns + su) * Np2 + prow_off;
-#pragma unroll
-        for (int l = 0; l < L; ++l) newP[f][l] = cadd(prow[l * LPF + R.t], cscale(acc[l], bP));
       }
     }
-    // object increment: sum over (lambda, psm) of conj(P_su)/Pmax * delta (axes (0,2))
+    // object increment: sum over (lambda, psm) of conj(P_su)/Pmax * delta
+    // (axes (0, 2)), the same for every lambda
     {
       float2 acc[L];
 #pragma unroll
@@ -878,32 +905,48 @@ __device__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool
 #pragma unroll
       for (int f = 0; f < FC; ++f) {
         if (f < nF) {
-          const float2* prow = P + (size_t)(f * ns + su) * Np2 + prow_off;
 #pragma unroll
           for (int l = 0; l < L; ++l) {
-            const float2 pv = prow[l * LPF + R.t];
-            const float2 fr = make_float2(pv.x * invP, -pv.y * invP);
-            acc[l] = cadd(acc[l], cmul(fr, d[f][l]));
+            const float2 pq = pv[f][l];
+            acc[l] = cadd(acc[l], cmul(make_float2(pq.x * invP, -pq.y * invP), d[f][l]));
           }
         }
       }
-      for (int il = 0; il < nl; ++il) {
-        float2* orow = O + (size_t)(il * ns + su) * No2 + orow_off;
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-          const int x = l * LPF + R.t;
-          const float2 nv = cadd(orow[x], cscale(acc[l], bO));
-          if (act) orow[x] = nv;
+      for (int q = 0; q < FC; ++q) {
+        if (q < nF && q % npm == 0) {  // one writer per lambda
+          float2 nv[L];
+#pragma unroll
+          for (int l = 0; l < L; ++l) nv[l] = cadd(ov[q][l], cscale(acc[l], bO));
+          if (act) store_row<NP>(O + (size_t)((q / npm) * ns + su) * No2 + orow_off, nv, R.t);
         }
       }
     }
+    // probe increments: sum over lambda of conj(O_su)/Omax * delta (axes (0, 1))
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
       if (f < nF) {
-        if (act) store_row<NP>(P + (size_t)(f * ns + su) * Np2 + prow_off, newP[f], R.t);
+        const int ip = f % npm;
+        float2 acc[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int g = 0; g < FC; ++g) {
+          if (g < nF && g % npm == ip) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              const float2 oq = ov[g][l];
+              acc[l] = cadd(acc[l], cmul(make_float2(oq.x * invO, -oq.y * invO), d[g][l]));
+            }
+          }
+        }
+        float2 nv[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) nv[l] = cadd(pv[f][l], cscale(acc[l], bP));
+        if (act) store_row<NP>(P + (size_t)(f * ns + su) * Np2 + prow_off, nv, R.t);
         if (su >= 1) {
-          fft_fwd<NP>(newP[f], R.tw, R.t);
-          if (act) store_row<NP>(W + f * Np2 + prow_off, newP[f], R.t);
+          fft_fwd<NP>(nv, R.tw, R.t);
+          if (act) store_row<NP>(W + f * Np2 + prow_off, nv, R.t);
         }
       }
     }
@@ -914,7 +957,7 @@ __device__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool
 // per-position drivers
 // ---------------------------------------------------------------------------
 template <int NP>
-__device__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
+__device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
   const DevInst& D = *R.D;
   pie_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
@@ -934,7 +977,7 @@ __device__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zsl
 }
 
 template <int NP>
-__device__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
+__device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
   const DevInst& D = *R.D;
   const int ns = D.a.nslice;
   e3_rows_first<NP>(R, k, slot, zslot);
@@ -962,7 +1005,7 @@ __device__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* z
 // last one stops before position je (full iterations: jb = 0, je >= F).  The
 // error of an iteration is reduced when its last position has run.
 template <int NP>
-__device__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
+__device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
   const DevInst& D = *R.D;
   const int F = D.a.F;
   const int engine = D.a.engine;
@@ -992,7 +1035,7 @@ __device__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
       for (int jj = threadIdx.x; jj < F; jj += NTHREADS) {
         const int kk = order[jj];
         double s = 0.0;
-        for (int g = 0; g < R.G; ++g) s += ep[(size_t)jj * R.G + g];
+        for (int g = 0; g < R.G; ++g) s += cloadd(ep + (size_t)jj * R.G + g);
         const float en = D.a.energy[kk] + 1e-20f;
         const double e = s / (double)en;
         D.a.err_at_pos[(size_t)it * F + kk] = e;
@@ -1004,6 +1047,10 @@ __device__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
   }
 }
 
+// One kernel per probe size: each instantiation gets its own register
+// allocation.  A plan with several probe sizes launches one kernel per size,
+// concurrently on its own streams.
+template <int NP>
 __global__ __launch_bounds__(NTHREADS) void pty_iterate_kernel(const DevInst* __restrict__ insts,
                                                                const int2* __restrict__ bmap, int it0, int it1,
                                                                int jb, int je) {
@@ -1019,32 +1066,16 @@ __global__ __launch_bounds__(NTHREADS) void pty_iterate_kernel(const DevInst* __
   R.G = D->G;
   R.wave = threadIdx.x >> 6;
   R.lane = threadIdx.x & 63;
+  R.t = R.lane & (Line<NP>::LPF - 1);
+  R.sub = R.lane / Line<NP>::LPF;
   R.gen = 0u;
   R.tw = s_tw;
   R.red = s_red;
   R.redf = s_redf;
   R.fail = &s_fail;
-  const int np = D->a.Np;
-  const int lg = log2_np(np) - 4;
-  for (int i = threadIdx.x; i < np / 2; i += NTHREADS) s_tw[i] = g_twiddle[lg][i];
+  for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
-  switch (np) {
-#define CASE(N)                              \
-  case N:                                    \
-    R.t = R.lane & (Line<N>::LPF - 1);       \
-    R.sub = R.lane / Line<N>::LPF;           \
-    run_iterations<N>(R, it0, it1, jb, je);          \
-    break;
-    CASE(16)
-    CASE(32)
-    CASE(64)
-    CASE(128)
-    CASE(256)
-    CASE(512)
-#undef CASE
-    default:
-      break;
-  }
+  run_iterations<NP>(R, it0, it1, jb, je);
 }
 
 __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
@@ -1197,16 +1228,39 @@ __global__ __launch_bounds__(NTHREADS) void pty_rows_inv_kernel(float2* __restri
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// Instances with the same probe size form a launch class: one kernel
+// instantiation, one block map, one launch per run.
+struct LaunchClass {
+  int np = 0;
+  int nblocks = 0;
+  bool cooperative = false;
+  int2* d_bmap = nullptr;
+  hipStream_t stream = nullptr;  // own stream when the plan has several classes
+  hipEvent_t done = nullptr;
+};
+
 struct pty_plan {
   int device = 0;
   int ninst = 0;
   int nblocks = 0;
-  bool cooperative = false;
   DevInst* d_insts = nullptr;
-  int2* d_bmap = nullptr;
   std::vector<DevInst> h_insts;
+  std::vector<LaunchClass> classes;
+  hipEvent_t fork = nullptr;
   std::vector<void*> allocs;
 };
+
+static const void* iterate_kernel_ptr(int np) {
+  switch (np) {
+    case 16: return (const void*)pty_iterate_kernel<16>;
+    case 32: return (const void*)pty_iterate_kernel<32>;
+    case 64: return (const void*)pty_iterate_kernel<64>;
+    case 128: return (const void*)pty_iterate_kernel<128>;
+    case 256: return (const void*)pty_iterate_kernel<256>;
+    case 512: return (const void*)pty_iterate_kernel<512>;
+  }
+  return nullptr;
+}
 
 namespace {
 
@@ -1239,6 +1293,13 @@ int dalloc(pty_plan* p, void** out, size_t bytes) {
 }
 
 void free_plan(pty_plan* p) {
+  for (LaunchClass& c : p->classes) {
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.done) (void)hipEventDestroy(c.done);
+  }
+  p->classes.clear();
+  if (p->fork) (void)hipEventDestroy(p->fork);
+  p->fork = nullptr;
   for (void* a : p->allocs) (void)hipFree(a);
   p->allocs.clear();
 }
@@ -1267,18 +1328,18 @@ int validate(const pty_instance& a, int idx) {
   if (a.propagator == PTY_ASP && a.nlambda > 1)
     return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, set polychromeASP instead of ASP", idx);
   if (a.propagator == PTY_ASP && !a.tf) return set_err(PTY_EINVAL, "instance %d: ASP needs a transfer function", idx);
-  const int fc = a.Np == 512 ? fcap<512>() : fcap<256>();
+  const int fc = fcap_rt(a.Np);
+  if (a.No % 2 != 0) return set_err(PTY_EINVAL, "instance %d: No=%d must be even", idx, a.No);
   if (a.engine == PTY_E3PIE) {
     if (a.nosm != 1) return set_err(PTY_EINVAL, "instance %d: e3PIE needs nosm == 1", idx);
     if (a.nslice > MAX_SLICES) return set_err(PTY_EINVAL, "instance %d: nslice > %d", idx, MAX_SLICES);
     if (a.propagator == PTY_ASP)
       return set_err(PTY_EINVAL, "instance %d: e3PIE supports the Fraunhofer detector propagator only", idx);
     if (!a.tf) return set_err(PTY_EINVAL, "instance %d: e3PIE needs the slice transfer function", idx);
-    if (a.nlambda * a.npsm > fc) return set_err(PTY_EINVAL, "instance %d: too many modes for Np=%d", idx, a.Np);
+    if (a.nlambda * a.npsm > fc) return set_err(PTY_EINVAL, "instance %d: more than %d modes for Np=%d", idx, fc, a.Np);
   } else {
     const int nF = a.nlambda * a.nosm * a.npsm * a.nslice;
     if (nF > fc) return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed %d for Np=%d", idx, nF, fc, a.Np);
-    if (a.nlambda * a.npsm * a.nslice > fc) return set_err(PTY_EINVAL, "instance %d: too many probe fields", idx);
     if ((a.engine == PTY_MPIE || a.engine == PTY_MULTIPIE) && a.momentum_mask &&
         (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
       return set_err(PTY_EINVAL, "instance %d: momentum needs buffer and momentum arrays", idx);
@@ -1335,8 +1396,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
   HIP_TRY(hipGetDevice(&p->device));
   p->ninst = ninst;
   p->h_insts.resize(ninst);
-  std::vector<int2> bmap;
-  int maxG = 1;
+  std::vector<std::vector<int2>> cmaps;  // per launch class
   for (int i = 0; i < ninst; ++i) {
     const pty_instance& a = insts[i];
     rc = validate(a, i);
@@ -1354,7 +1414,6 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     if (G < 1) G = 1;
     while (a.Np % G) --G;
     d.G = G;
-    maxG = G > maxG ? G : maxG;
     const int np = a.Np;
     const size_t np2 = (size_t)np * np;
     if (a.engine == PTY_E3PIE) {
@@ -1414,32 +1473,53 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       HIP_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
     }
-    for (int g = 0; g < G; ++g) bmap.push_back(make_int2(i, g));
+    int ci = -1;
+    for (size_t c = 0; c < p->classes.size(); ++c)
+      if (p->classes[c].np == a.Np) ci = (int)c;
+    if (ci < 0) {
+      p->classes.emplace_back();
+      p->classes.back().np = a.Np;
+      cmaps.emplace_back();
+      ci = (int)p->classes.size() - 1;
+    }
+    if (G > 1) p->classes[ci].cooperative = true;
+    for (int g = 0; g < G; ++g) cmaps[ci].push_back(make_int2(i, g));
   }
   {
-    p->nblocks = (int)bmap.size();
-    p->cooperative = maxG > 1;
     void* mem = nullptr;
     if ((rc = dalloc(p, &mem, sizeof(DevInst) * ninst))) goto fail;
     p->d_insts = (DevInst*)mem;
-    if ((rc = dalloc(p, &mem, sizeof(int2) * bmap.size()))) goto fail;
-    p->d_bmap = (int2*)mem;
     HIP_TRY(hipMemcpyAsync(p->d_insts, p->h_insts.data(), sizeof(DevInst) * ninst, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(p->d_bmap, bmap.data(), sizeof(int2) * bmap.size(), hipMemcpyHostToDevice, st));
-    if (p->cooperative) {
-      int per_cu = 0, cus = 0, coop = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)pty_iterate_kernel, NTHREADS, 0));
-      HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-      HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, p->device));
-      if (!coop) {
-        rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
-        goto fail;
+    int cus = 0, coop = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, p->device));
+    p->nblocks = 0;
+    for (size_t c = 0; c < p->classes.size(); ++c) {
+      LaunchClass& lc = p->classes[c];
+      lc.nblocks = (int)cmaps[c].size();
+      p->nblocks += lc.nblocks;
+      if ((rc = dalloc(p, &mem, sizeof(int2) * cmaps[c].size()))) goto fail;
+      lc.d_bmap = (int2*)mem;
+      HIP_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
+      if (lc.cooperative) {
+        int per_cu = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np), NTHREADS, 0));
+        if (!coop) {
+          rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
+          goto fail;
+        }
+        if (lc.nblocks > per_cu * cus) {
+          rc = set_err(PTY_EINVAL, "%d workgroups for Np=%d exceed the %d that can be co-resident", lc.nblocks, lc.np,
+                       per_cu * cus);
+          goto fail;
+        }
       }
-      if (p->nblocks > per_cu * cus) {
-        rc = set_err(PTY_EINVAL, "%d workgroups exceed the %d that can be co-resident", p->nblocks, per_cu * cus);
-        goto fail;
+      if (p->classes.size() > 1) {
+        HIP_TRY(hipStreamCreateWithFlags(&lc.stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&lc.done, hipEventDisableTiming));
       }
     }
+    if (p->classes.size() > 1) HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
     HIP_TRY(hipStreamSynchronize(st));
   }
   *out = p;
@@ -1473,14 +1553,22 @@ static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb
   pty_reset_ctrl_kernel<<<p->ninst, 64, 0, st>>>(p->d_insts);
   HIP_TRY(hipGetLastError());
   const DevInst* di = p->d_insts;
-  const int2* bm = p->d_bmap;
   int a0 = it_begin, a1 = it_end, b0 = jb, b1 = je;
-  if (p->cooperative) {
+  const bool fork = p->classes.size() > 1;
+  if (fork) HIP_TRY(hipEventRecord(p->fork, st));
+  for (LaunchClass& lc : p->classes) {
+    hipStream_t ls = fork ? lc.stream : st;
+    if (fork) HIP_TRY(hipStreamWaitEvent(ls, p->fork, 0));
+    const int2* bm = lc.d_bmap;
     void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1};
-    HIP_TRY(hipLaunchCooperativeKernel((const void*)pty_iterate_kernel, dim3(p->nblocks), dim3(NTHREADS), args, 0, st));
-  } else {
-    pty_iterate_kernel<<<p->nblocks, NTHREADS, 0, st>>>(di, bm, a0, a1, b0, b1);
-    HIP_TRY(hipGetLastError());
+    if (lc.cooperative)
+      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(NTHREADS), args, 0, ls));
+    else
+      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(NTHREADS), args, 0, ls));
+    if (fork) {
+      HIP_TRY(hipEventRecord(lc.done, ls));
+      HIP_TRY(hipStreamWaitEvent(st, lc.done, 0));
+    }
   }
   return PTY_OK;
 }
