@@ -47,6 +47,14 @@ using namespace pty;
 #define MAX_SLICES 15
 #define SPIN_LIMIT (1u << 22)
 
+// control block of an instance (32-bit words; each counter on its own 128-byte line)
+#define CTRL_TOP 0
+#define CTRL_FAIL 1
+#define CTRL_SLOTS 32      // 4 ring slots x 32 words of per-position maxima
+#define CTRL_GROUP 160     // 8 XCD-group arrival counters, 32 words apart
+#define CTRL_RELEASE 416   // published barrier generation
+#define CTRL_WORDS 448
+
 namespace {
 
 thread_local std::string g_err;
@@ -77,6 +85,7 @@ struct DevInst {
   pty_instance a;
   int G;             // workgroups of this instance
   int nF;            // fields in W
+  int nFp;           // nF rounded up to a power of two (item slots per line)
   int fs;            // slice stride of W fields (last-slice fields: f % fs == fs-1)
   int nO, nPr;       // object / probe fields
   float omAlphaO;    // (float)(1 - alphaObject), rounded as NumPy does
@@ -85,7 +94,9 @@ struct DevInst {
   float2* ESW;       // e3PIE exit waves (nslice, nF, Np, Np)
   const float* Ik;   // ptychogram in kernel layout
   const float2* HT;  // transfer function, transposed bit-reversed, / Np^2
-  unsigned* ctrl;    // [0] barrier counter, [1] failure flag, [16 + 32*slot + q] maxima
+  unsigned* ctrl;    // control block, CTRL_* layout
+  int gsize[8];      // workgroups of this instance in each XCD group
+  int ngroups;       // non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
 };
 
@@ -103,22 +114,20 @@ __host__ __device__ static int log2_np(int np) {
   return -1;
 }
 
-// Register capacity for fields of one line: a wave holds FC lines of L
-// complex values per field kind.
-template <int NP>
-__host__ __device__ constexpr int fcap() {
-  return NP <= 128 ? 8 : (NP == 256 ? 4 : 2);
-}
-
-static int fcap_rt(int np) { return np <= 128 ? 8 : (np == 256 ? 4 : 2); }
+// item slots per round of the persistent kernel for probe size np
+// (Cfg<NP>::IPR): waves per workgroup x lines per wave
+static int slots_rt(int np) { return (np >= 128 ? 8 : 16) * (np >= 64 ? 1 : 64 / np); }
+static int nthreads_rt(int np) { return np >= 128 ? 512 : 1024; }
 
 struct Run {
   const DevInst* D;
   int g, G, wave, t, sub, lane;
+  int nF, nFp, nwave;
   unsigned gen;
   const float2* tw;
   double* red;
-  float* redf;
+  float2* xa;  // LDS exchange buffers, one line-field slot per item
+  float2* xb;
   int* fail;
 };
 
@@ -144,7 +153,7 @@ __device__ __forceinline__ double block_sum(Run& R, double v) {
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0) {
-    for (int w = 0; w < NWAVE; ++w) s += R.red[w];
+    for (int w = 0; w < R.nwave; ++w) s += R.red[w];
   }
   return s;
 }
@@ -170,22 +179,38 @@ __device__ __forceinline__ float load_slot(const unsigned* p) {
 // leaves.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
 
+// Arrival is two-level: a workgroup increments the counter of its XCD group
+// (blocks of one launch are dispatched round-robin, block b on XCD b % 8;
+// the grouping only spreads the atomics over separate cache lines, nothing
+// depends on the placement), the last arriver of a group increments the top
+// counter, and the last arriver overall publishes the generation on a
+// release line that the waiting workgroups poll.  Arrivals and polls thus
+// never contend for one address.
 __device__ __forceinline__ bool cluster_sync(Run& R) {
   drain_stores();
   __syncthreads();
   if (R.G == 1) return true;
   R.gen += 1u;
   if (threadIdx.x == 0) {
-    unsigned* ctr = R.D->ctrl;
-    const unsigned target = R.gen * (unsigned)R.G;
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const DevInst& D = *R.D;
+    unsigned* ctrl = D.ctrl;
+    const int grp = blockIdx.x & 7;
+    unsigned* gctr = ctrl + CTRL_GROUP + 32 * grp;
+    unsigned* top = ctrl + CTRL_TOP;
+    unsigned* rel = ctrl + CTRL_RELEASE;
+    const unsigned gen = R.gen;
+    const unsigned old = __hip_atomic_fetch_add(gctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1u == gen * (unsigned)D.gsize[grp]) {
+      const unsigned t = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (t + 1u == gen * (unsigned)D.ngroups) __hip_atomic_store(rel, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     unsigned spins = 0;
     int fail = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(rel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
       ++spins;
       if ((spins & 255u) == 0u &&
-          (spins > SPIN_LIMIT || __hip_atomic_load(ctr + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-        __hip_atomic_store(ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          (spins > SPIN_LIMIT || __hip_atomic_load(ctrl + CTRL_FAIL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(ctrl + CTRL_FAIL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         fail = 1;
         break;
       }
@@ -249,21 +274,44 @@ __device__ __forceinline__ float2 ldg2(const float2* p) {
   return __builtin_bit_cast(float2, *(const GAS unsigned long long*)p);
 }
 
-// Lines [line0, line0 + per) of this workgroup are visited SUB at a time by
-// each wave; `li` is the line handled by this lane group, `act` is false for
-// the padding groups of the last round (they compute on a valid line but
-// store nothing).  Usage: LINES_BEGIN(R, NP, r, act) ... LINES_END
-#define LINES_BEGIN(R, NP, li, act)                                          \
-  {                                                                          \
-    const int per_ = NP / (R).G;                                             \
-    const int line0_ = (R).g * per_;                                         \
-    for (int base_ = 0; base_ < per_; base_ += NWAVE * Line<NP>::SUB) {      \
-      const int rel_ = base_ + (R).wave * Line<NP>::SUB + (R).sub;           \
-      const bool act = rel_ < per_;                                          \
-      const int li = line0_ + (act ? rel_ : per_ - 1);
-#define LINES_END \
+// ---------------------------------------------------------------------------
+// work decomposition
+// ---------------------------------------------------------------------------
+// A workgroup owns lines [g * Np/G, (g+1) * Np/G) of every pass.  Its waves
+// take (line, field) items: item slot s of a round is line s / nFp, field
+// s % nFp (nFp = nF rounded up to a power of two; padding slots compute on a
+// valid field and store nothing).  One wave transforms one field of one line
+// per round, so the four probe modes of a position run on four waves at once.
+// Sums over fields (the modelled intensity, the object and probe increments,
+// the maxima) go through LDS and are added in increasing field order, the C
+// order of NumPy's axis sums, so results are deterministic.
+template <int NP>
+struct Cfg {
+  static constexpr int NT = NP >= 128 ? 512 : 1024;  // threads per workgroup
+  static constexpr int NW = NT / 64;
+  static constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
+  static constexpr int IPR = NW * SUB;  // item slots per round
+};
+
+#define ITEMS_BEGIN(R, NP, line, f, act, islot, lslot, frw)                     \
+  {                                                                            \
+    const int per_ = NP / (R).G;                                               \
+    const int line0_ = (R).g * per_;                                           \
+    const int lpr_ = Cfg<NP>::IPR / (R).nFp;                                   \
+    const int islot = (R).wave * Line<NP>::SUB + (R).sub;                      \
+    const int lil_ = islot / (R).nFp;                                          \
+    const int frw = islot - lil_ * (R).nFp;                                    \
+    const int lslot = lil_ * (R).nFp;                                          \
+    const int f = frw < (R).nF ? frw : (R).nF - 1;                             \
+    for (int base_ = 0; base_ < per_; base_ += lpr_) {                         \
+      const int li_ = base_ + lil_;                                            \
+      const bool lok_ = li_ < per_;                                            \
+      const bool act = lok_ && frw < (R).nF;                                   \
+      const int line = line0_ + (lok_ ? li_ : per_ - 1);
+#define ITEMS_END \
   }               \
   }
+#define ITEM_LEAD(act, frw) ((act) && (frw) == 0)  // the slot that speaks for its line
 
 // ---------------------------------------------------------------------------
 // field bookkeeping
@@ -285,14 +333,22 @@ __device__ __forceinline__ FieldIdx field_idx(int f, int no, int npm, int ns) {
   return x;
 }
 
+template <int NP>
+__device__ __forceinline__ void lds_put(float2* buf, int islot, const float2 (&v)[Line<NP>::L], int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) buf[islot * NP + l * Line<NP>::LPF + t] = v[l];
+}
+
 // ---------------------------------------------------------------------------
-// ePIE / mPIE / multiPIE position passes.  Every pass first issues the loads
-// of all fields of its line, then computes, so a pass costs one memory
-// latency per line instead of one per field.
+// ePIE / mPIE / multiPIE position passes
 // ---------------------------------------------------------------------------
+// First row pass: exit wave psi_f = O_f P_f of the gathered object row and the
+// probe row, (checkerboard for the centred Fraunhofer transform), forward row
+// transform into W; per pixel sums of |P|^2 over the probe fields and of |O|^2
+// over the object fields give the maxima the updates divide by.
 template <int NP>
 __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
   const int No = D.a.No, nF = D.nF;
@@ -304,55 +360,40 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   const bool cb = D.a.propagator == PTY_FRAUNHOFER;
   if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
   float pmax = 0.f, omax = 0.f;
-  LINES_BEGIN(R, NP, r, act)
-    // per exit-wave field: its object row and its probe row
-    float2 of[FC][L], pf[FC][L];
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
+    const FieldIdx x = field_idx(f, no, npm, ns);
+    float2 of[L], pf[L], q[L];
+    load_row<NP>(of, O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
+    load_row<NP>(pf, P + x.fp * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        const FieldIdx x = field_idx(f, no, npm, ns);
-        load_row<NP>(of[f], O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
-        load_row<NP>(pf[f], P + x.fp * Np2 + (size_t)r * NP, R.t);
-      }
+    for (int l = 0; l < L; ++l)
+      q[l] = make_float2(act && x.io == 0 ? cabs2(pf[l]) : 0.f, act && x.ip == 0 ? cabs2(of[l]) : 0.f);
+    lds_put<NP>(R.xa, islot, q, R.t);
+    float2 v[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      float2 e = cmul(of[l], pf[l]);
+      if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
+      v[l] = e;
     }
-    // sum over every probe field (the fields with osm index 0 enumerate them
-    // once, in increasing order) and over every object field (psm index 0)
-    float sp[L], so[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) sp[l] = so[l] = 0.f;
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        const FieldIdx x = field_idx(f, no, npm, ns);
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          if (x.io == 0) sp[l] += cabs2(pf[f][l]);
-          if (x.ip == 0) so[l] += cabs2(of[f][l]);
-        }
-      }
-    }
-    if (act) {
+    fft_fwd<NP>(v, R.tw, R.t);
+    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    __syncthreads();
+    if (ITEM_LEAD(act, frw)) {
 #pragma unroll
       for (int l = 0; l < L; ++l) {
-        pmax = fmaxf(pmax, sp[l]);
-        omax = fmaxf(omax, so[l]);
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        float2 v[L];
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          float2 e = cmul(of[f][l], pf[f][l]);
-          if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
-          v[l] = e;
+        float sp = 0.f, so = 0.f;
+        for (int g = 0; g < nF; ++g) {
+          const float2 c = R.xa[(lslot + g) * NP + l * LPF + R.t];
+          sp += c.x;
+          so += c.y;
         }
-        fft_fwd<NP>(v, R.tw, R.t);
-        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+        pmax = fmaxf(pmax, sp);
+        omax = fmaxf(omax, so);
       }
     }
-  LINES_END
+    __syncthreads();
+  ITEMS_END
   pmax = wave_max(pmax);
   omax = wave_max(omax);
   if (R.lane == 0) {
@@ -366,89 +407,72 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
 // Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
 // BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
 template <int NP>
-__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int nF, int fs) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
   const float invN = 1.0f / (float)NP;
+  const int nF = D.nF;
   double err = 0.0;
-  LINES_BEGIN(R, NP, p, act)
-    float2 v[FC][L];
+  ITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
+    float2 v[L];
     float im[L];
-#pragma unroll
-    for (int f = 0; f < FC; ++f)
-      if (f < nF) load_col<NP>(v[f], W + f * Np2, p, R.t);
+    load_col<NP>(v, W + f * Np2, p, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)p * NP + l * LPF + R.t);
-    float iest[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) iest[l] = 0.f;
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        fft_fwd<NP>(v[f], R.tw, R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], invN);
-        if (f % fs == fs - 1) {
-#pragma unroll
-          for (int l = 0; l < L; ++l) iest[l] += cabs2(v[f][l]);
-        }
-      }
-    }
-    float amp[L];
+    fft_fwd<NP>(v, R.tw, R.t);
+    float2 q[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      if (act) err += (double)fabsf(im[l] - iest[l]);
-      amp[l] = sqrtf(im[l] / (iest[l] + 1e-10f));
+      v[l] = cscale(v[l], invN);
+      q[l] = make_float2(act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f, 0.f);
     }
+    lds_put<NP>(R.xa, islot, q, R.t);
+    __syncthreads();
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-#pragma unroll
-        for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], amp[l]);
-        fft_inv<NP>(v[f], R.tw, R.t);
-        if (act) store_col<NP>(W + f * Np2, v[f], p, R.t);
-      }
+    for (int l = 0; l < L; ++l) {
+      float iest = 0.f;
+      for (int g = 0; g < nF; ++g) iest += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
+      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest);
+      v[l] = cscale(v[l], sqrtf(im[l] / (iest + 1e-10f)));
     }
-  LINES_END
+    fft_inv<NP>(v, R.tw, R.t);
+    if (act) store_col<NP>(W + f * Np2, v, p, R.t);
+    __syncthreads();
+  ITEMS_END
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
 }
 
 // Column pass of an angular-spectrum propagation: forward column transform,
 // multiply by the (transposed, bit-reversed, 1/Np^2-scaled) transfer function
-// or its conjugate, inverse column transform.
+// or its conjugate, inverse column transform.  Fields are independent.
 template <int NP>
-__device__ __forceinline__ void tf_cols(Run& R, int nF, bool conj) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+__device__ __forceinline__ void tf_cols(Run& R, bool conj) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float2* __restrict__ HT = D.HT;
-  LINES_BEGIN(R, NP, p, act)
-    float2 h[L];
-    float2 v[FC][L];
-#pragma unroll
-    for (int f = 0; f < FC; ++f)
-      if (f < nF) load_col<NP>(v[f], W + f * Np2, p, R.t);
+  ITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
+    (void)islot;
+    (void)lslot;
+    (void)frw;
+    float2 v[L], h[L];
+    load_col<NP>(v, W + f * Np2, p, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       h[l] = ldg2(HT + (size_t)p * NP + l * LPF + R.t);
       if (conj) h[l] = conjf2(h[l]);
     }
+    fft_fwd<NP>(v, R.tw, R.t);
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        fft_fwd<NP>(v[f], R.tw, R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l) v[f][l] = cmul(v[f][l], h[l]);
-        fft_inv<NP>(v[f], R.tw, R.t);
-        if (act) store_col<NP>(W + f * Np2, v[f], p, R.t);
-      }
-    }
-  LINES_END
+    for (int l = 0; l < L; ++l) v[l] = cmul(v[l], h[l]);
+    fft_inv<NP>(v, R.tw, R.t);
+    if (act) store_col<NP>(W + f * Np2, v, p, R.t);
+  ITEMS_END
 }
 
 // ASP detector row pass: inverse row transform gives the detector field, the
@@ -456,65 +480,50 @@ __device__ __forceinline__ void tf_cols(Run& R, int nF, bool conj) {
 // transform starts the back-propagation.
 template <int NP>
 __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
   const int nF = D.nF, fs = D.fs;
   double err = 0.0;
-  LINES_BEGIN(R, NP, r, act)
-    float2 v[FC][L];
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
+    float2 v[L];
     float im[L];
-#pragma unroll
-    for (int f = 0; f < FC; ++f)
-      if (f < nF) load_row<NP>(v[f], W + f * Np2 + (size_t)r * NP, R.t);
+    load_row<NP>(v, W + f * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
-    float iest[L];
+    fft_inv<NP>(v, R.tw, R.t);
+    float2 q[L];
 #pragma unroll
-    for (int l = 0; l < L; ++l) iest[l] = 0.f;
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        fft_inv<NP>(v[f], R.tw, R.t);
-        if (f % fs == fs - 1) {
-#pragma unroll
-          for (int l = 0; l < L; ++l) iest[l] += cabs2(v[f][l]);
-        }
-      }
-    }
-    float amp[L];
+    for (int l = 0; l < L; ++l) q[l] = make_float2(act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f, 0.f);
+    lds_put<NP>(R.xa, islot, q, R.t);
+    __syncthreads();
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      if (act) err += (double)fabsf(im[l] - iest[l]);
-      amp[l] = sqrtf(im[l] / (iest[l] + 1e-10f));
+      float iest = 0.f;
+      for (int g = 0; g < nF; ++g) iest += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
+      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest);
+      v[l] = cscale(v[l], sqrtf(im[l] / (iest + 1e-10f)));
     }
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-#pragma unroll
-        for (int l = 0; l < L; ++l) v[f][l] = cscale(v[f][l], amp[l]);
-        fft_fwd<NP>(v[f], R.tw, R.t);
-        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v[f], R.t);
-      }
-    }
-  LINES_END
+    fft_fwd<NP>(v, R.tw, R.t);
+    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    __syncthreads();
+  ITEMS_END
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
 }
 
-// Last row pass: inverse row transform gives the updated exit waves, the
-// difference to the exit waves of the same position, and the engine's
-// object-patch and probe updates (both from the pre-update patch and probe,
-// held in registers), written back in place.  Each object pixel of the patch
-// has one owner.  Sums run over the fields in increasing order, which is the
-// C order NumPy's axis sums use.  Every object field is written by the
-// exit-wave field that carries it with psm index 0, every probe field by the
-// one with osm index 0.
+// Last row pass: inverse row transform gives the updated exit wave of the
+// field, the difference to the exit wave of the same position, and the
+// field's terms of the engine's object and probe increments (both from the
+// pre-update patch and probe); the terms are summed over fields through LDS
+// and each object / probe field is written by one owner slot (the field with
+// psm index 0, resp. osm index 0).  Each object pixel of the patch has one
+// owner.
 template <int NP>
 __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
   const int No = D.a.No, nF = D.nF;
@@ -528,151 +537,88 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
   const float Pmax = load_slot(slot + 0), Omax = load_slot(slot + 1);
-  const float bO = D.a.betaObject, bP = D.a.betaProbe;
-  LINES_BEGIN(R, NP, r, act)
+  const float bO = D.a.betaObject;
+  const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
+  // ePIE.py:128-160 weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
+  // mPIE.py:226-270 / multiPIE.py:153-205 (CPM) weights
+  // conj(P)/(aO Pmax + (1-aO)|P|^2), conj(O)/(aP Omax + (1-aP)|O|^2)
+  const float invP = 1.0f / Pmax, invO = 1.0f / Omax;
+  const float aPm = D.a.alphaObject * Pmax, aOm = D.a.alphaProbe * Omax;
+  const float omO = D.omAlphaO, omP = D.omAlphaP;
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
+    (void)frw;
+    const FieldIdx x = field_idx(f, no, npm, ns);
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
-    float2 d[FC][L], of[FC][L], pf[FC][L];
+    float2 d[L], of[L], pf[L];
+    load_row<NP>(d, W + f * Np2 + prow_off, R.t);
+    load_row<NP>(of, O + x.fo * No2 + orow_off, R.t);
+    load_row<NP>(pf, P + x.fp * Np2 + prow_off, R.t);
+    fft_inv<NP>(d, R.tw, R.t);
+    float2 tO[L], tP[L];
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        const FieldIdx x = field_idx(f, no, npm, ns);
-        load_row<NP>(d[f], W + f * Np2 + prow_off, R.t);
-        load_row<NP>(of[f], O + x.fo * No2 + orow_off, R.t);
-        load_row<NP>(pf[f], P + x.fp * Np2 + prow_off, R.t);
+    for (int l = 0; l < L; ++l) {
+      float2 u = d[l];
+      if (prop != PTY_ASP) {
+        u = cscale(u, invN);
+        if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
       }
+      const float2 dl = csub(u, cmul(of[l], pf[l]));
+      float wo, wp;
+      if (engine == PTY_EPIE) {
+        wo = invP;
+        wp = invO;
+      } else {
+        wo = 1.0f / (aPm + omO * cabs2(pf[l]));
+        wp = 1.0f / (aOm + omP * cabs2(of[l]));
+      }
+      tO[l] = cmul(make_float2(pf[l].x * wo, -pf[l].y * wo), dl);
+      tP[l] = cmul(make_float2(of[l].x * wp, -of[l].y * wp), dl);
     }
+    lds_put<NP>(R.xa, islot, tO, R.t);
+    lds_put<NP>(R.xb, islot, tP, R.t);
+    __syncthreads();
+    if (act && x.ip == 0) {
+      // object field x.fo: ePIE sums over (lambda, psm, slice), the m/multi
+      // engines over psm
+      float2 nv[L];
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        fft_inv<NP>(d[f], R.tw, R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          float2 u = d[f][l];
-          if (prop != PTY_ASP) {
-            u = cscale(u, invN);
-            if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
-          }
-          d[f][l] = csub(u, cmul(of[f][l], pf[f][l]));
+      for (int l = 0; l < L; ++l) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int g = 0; g < nF; ++g) {
+          const FieldIdx y = field_idx(g, no, npm, ns);
+          if (engine == PTY_EPIE ? y.io == x.io : y.fo == x.fo)
+            acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
         }
+        nv[l] = cadd(of[l], cscale(acc, bO));
       }
+      store_row<NP>(O + x.fo * No2 + orow_off, nv, R.t);
     }
-    if (engine == PTY_EPIE) {
-      // ePIE.py:128-160: weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
-      // object increment summed over (lambda, psm, slice) -> one per osm,
-      // probe increment over (lambda, osm, slice) -> one per psm
-      const float invP = 1.0f / Pmax, invO = 1.0f / Omax;
+    if (act && x.io == 0) {
+      // probe field x.fp: ePIE sums over (lambda, osm, slice), mPIE over osm,
+      // multiPIE over (lambda, osm)
+      float2 nv[L];
 #pragma unroll
-      for (int f0 = 0; f0 < FC; ++f0) {
-        if (f0 < nF) {
-          const FieldIdx x0 = field_idx(f0, no, npm, ns);
-          if (x0.ip == 0) {
-            float2 acc[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int f = 0; f < FC; ++f) {
-              if (f < nF && field_idx(f, no, npm, ns).io == x0.io) {
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                  const float2 pq = pf[f][l];
-                  acc[l] = cadd(acc[l], cmul(make_float2(pq.x * invP, -pq.y * invP), d[f][l]));
-                }
-              }
-            }
-            float2 nv[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) nv[l] = cadd(of[f0][l], cscale(acc[l], bO));
-            if (act) store_row<NP>(O + x0.fo * No2 + orow_off, nv, R.t);
-          }
-          if (x0.io == 0) {
-            float2 acc[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int f = 0; f < FC; ++f) {
-              if (f < nF && field_idx(f, no, npm, ns).ip == x0.ip) {
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                  const float2 oq = of[f][l];
-                  acc[l] = cadd(acc[l], cmul(make_float2(oq.x * invO, -oq.y * invO), d[f][l]));
-                }
-              }
-            }
-            float2 nv[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) nv[l] = cadd(pf[f0][l], cscale(acc[l], bP));
-            if (act) store_row<NP>(P + x0.fp * Np2 + prow_off, nv, R.t);
-          }
+      for (int l = 0; l < L; ++l) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int g = 0; g < nF; ++g) {
+          const FieldIdx y = field_idx(g, no, npm, ns);
+          const bool sel = engine == PTY_EPIE ? y.ip == x.ip
+                                              : (y.ip == x.ip && y.is == x.is && (engine == PTY_MULTIPIE || y.il == x.il));
+          if (sel) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
         }
+        nv[l] = cadd(pf[l], cscale(acc, engine == PTY_EPIE ? D.a.betaProbe : stepP));
       }
-    } else {
-      // mPIE.py:226-270 / multiPIE.py:153-205 (CPM): regularised weights
-      // conj(P)/(aO Pmax + (1-aO)|P|^2) and conj(O)/(aP Omax + (1-aP)|O|^2)
-      const float aPm = D.a.alphaObject * Pmax;
-      const float aOm = D.a.alphaProbe * Omax;
-      const float omO = D.omAlphaO, omP = D.omAlphaP;
-      const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
-      const bool sumLambda = engine == PTY_MULTIPIE;
-#pragma unroll
-      for (int f0 = 0; f0 < FC; ++f0) {
-        if (f0 < nF) {
-          const FieldIdx x0 = field_idx(f0, no, npm, ns);
-          if (x0.ip == 0) {
-            // object field x0.fo: sum over psm (axis 2)
-            float2 acc[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int f = 0; f < FC; ++f) {
-              if (f < nF && field_idx(f, no, npm, ns).fo == x0.fo) {
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                  const float2 pq = pf[f][l];
-                  const float inv = 1.0f / (aPm + omO * cabs2(pq));
-                  acc[l] = cadd(acc[l], cmul(make_float2(pq.x * inv, -pq.y * inv), d[f][l]));
-                }
-              }
-            }
-            float2 nv[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) nv[l] = cadd(of[f0][l], cscale(acc[l], bO));
-            if (act) store_row<NP>(O + x0.fo * No2 + orow_off, nv, R.t);
-          }
-          if (x0.io == 0) {
-            // probe field x0.fp: mPIE sums over osm (axis 1), multiPIE over
-            // (lambda, osm) (axes (0, 1))
-            float2 acc[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int f = 0; f < FC; ++f) {
-              if (f < nF) {
-                const FieldIdx x = field_idx(f, no, npm, ns);
-                if (x.ip == x0.ip && x.is == x0.is && (sumLambda || x.il == x0.il)) {
-#pragma unroll
-                  for (int l = 0; l < L; ++l) {
-                    const float2 oq = of[f][l];
-                    const float inv = 1.0f / (aOm + omP * cabs2(oq));
-                    acc[l] = cadd(acc[l], cmul(make_float2(oq.x * inv, -oq.y * inv), d[f][l]));
-                  }
-                }
-              }
-            }
-            float2 nv[L];
-#pragma unroll
-            for (int l = 0; l < L; ++l) nv[l] = cadd(pf[f0][l], cscale(acc[l], stepP));
-            if (act) store_row<NP>(P + x0.fp * Np2 + prow_off, nv, R.t);
-          }
-        }
-      }
+      store_row<NP>(P + x.fp * Np2 + prow_off, nv, R.t);
     }
-  LINES_END
+    __syncthreads();
+  ITEMS_END
 }
 
 // mPIE / multiPIE momentum step over the whole object and probe
 // (mPIE.py:196-224, multiPIE.py:125-151); each workgroup owns a contiguous
 // chunk, so the step needs no read-modify-write atomics.
+template <int NP>
 __device__ __forceinline__ void momentum_pass(Run& R) {
   const DevInst& D = *R.D;
   const float eta = D.a.momFriction, nu = D.a.momFeedback;
@@ -684,7 +630,7 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
     const size_t chunk = (n + R.G - 1) / R.G;
     const size_t b = chunk * R.g;
     const size_t e = b + chunk < n ? b + chunk : n;
-    for (size_t i = b + threadIdx.x; i < e; i += NTHREADS) {
+    for (size_t i = b + threadIdx.x; i < e; i += Cfg<NP>::NT) {
       const float2 x = cload(X + i), bb = cload(B + i), m = cload(M + i);
       const float2 mn = make_float2((bb.x - x.x) + m.x * eta, (bb.y - x.y) + m.y * eta);
       const float2 xn = make_float2(x.x - mn.x * nu, x.y - mn.y * nu);
@@ -695,7 +641,6 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
   }
 }
 
-
 // ---------------------------------------------------------------------------
 // e3PIE slice chain (e3PIE.py:89-173, updates :192-225).  W holds the
 // (lambda, psm) fields of the slice being propagated; ESW keeps the exit
@@ -703,10 +648,10 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
 // ---------------------------------------------------------------------------
 template <int NP>
 __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
-  const int No = D.a.No;
+  const int No = D.a.No, nF = D.nF;
   const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
   const float2* __restrict__ O = (const float2*)D.a.object;
@@ -714,67 +659,51 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
   float2* __restrict__ W = D.W;
   float2* __restrict__ E = D.ESW;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER && ns == 1;
-  const int nF = D.nF;
   if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
   float omx[MAX_SLICES];
 #pragma unroll
   for (int s = 0; s < MAX_SLICES; ++s) omx[s] = 0.f;
   float pmax = 0.f;
-  LINES_BEGIN(R, NP, r, act)
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
-    float2 of[FC][L], pv[FC][L];
+    float2 of[L], pv[L], q[L];
+    load_row<NP>(of, O + (size_t)((f / npm) * ns + 0) * No2 + orow_off, R.t);
+    load_row<NP>(pv, P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
-    for (int q = 0; q < FC; ++q) {
-      if (q < nF) {
-        load_row<NP>(of[q], O + (size_t)((q / npm) * ns + 0) * No2 + orow_off, R.t);
-        load_row<NP>(pv[q], P + (size_t)(q * ns + 0) * Np2 + (size_t)r * NP, R.t);
+    for (int l = 0; l < L; ++l) q[l] = make_float2(act ? cabs2(pv[l]) : 0.f, 0.f);
+    lds_put<NP>(R.xa, islot, q, R.t);
+    float2 v[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) v[l] = cmul(of[l], pv[l]);
+    if (act) store_row<NP>(E + (size_t)f * Np2 + (size_t)r * NP, v, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+    fft_fwd<NP>(v, R.tw, R.t);
+    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    __syncthreads();
+    if (ITEM_LEAD(act, frw)) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        float sp = 0.f;
+        for (int g = 0; g < nF; ++g) sp += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
+        pmax = fmaxf(pmax, sp);
       }
-    }
 #pragma unroll
-    for (int s = 0; s < MAX_SLICES; ++s) {
-      if (s < ns) {
-        float so[L];
+      for (int s = 0; s < MAX_SLICES; ++s) {
+        if (s < ns) {
 #pragma unroll
-        for (int l = 0; l < L; ++l) so[l] = 0.f;
-        for (int il = 0; il < nl; ++il) {
-          const float2* orow = O + (size_t)(il * ns + s) * No2 + orow_off;
-#pragma unroll
-          for (int l = 0; l < L; ++l) so[l] += cabs2(cload(orow + l * LPF + R.t));
+          for (int l = 0; l < L; ++l) {
+            float so = 0.f;
+            for (int il = 0; il < nl; ++il)
+              so += cabs2(cload(O + (size_t)(il * ns + s) * No2 + orow_off + l * LPF + R.t));
+            omx[s] = fmaxf(omx[s], so);
+          }
         }
-        if (act) {
-#pragma unroll
-          for (int l = 0; l < L; ++l) omx[s] = fmaxf(omx[s], so[l]);
-        }
       }
     }
-    float sp[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) sp[l] = 0.f;
-#pragma unroll
-    for (int q = 0; q < FC; ++q)
-      if (q < nF) {
-#pragma unroll
-        for (int l = 0; l < L; ++l) sp[l] += cabs2(pv[q][l]);
-      }
-    if (act) {
-#pragma unroll
-      for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
-    }
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        float2 v[L];
-#pragma unroll
-        for (int l = 0; l < L; ++l) v[l] = cmul(of[f][l], pv[f][l]);
-        if (act) store_row<NP>(E + (size_t)f * Np2 + (size_t)r * NP, v, R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l)
-          if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
-        fft_fwd<NP>(v, R.tw, R.t);
-        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
-      }
-    }
-  LINES_END
+    __syncthreads();
+  ITEMS_END
   pmax = wave_max(pmax);
   if (R.lane == 0) atomic_fmax_slot(slot + 0, pmax);
 #pragma unroll
@@ -792,10 +721,10 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
 // when s is the last slice).
 template <int NP>
 __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* slot) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
-  const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
-  const int No = D.a.No;
+  const int npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No, nF = D.nF;
   const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
   const float2* __restrict__ O = (const float2*)D.a.object;
@@ -803,42 +732,37 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
   float2* __restrict__ W = D.W;
   float2* __restrict__ E = D.ESW;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER && s == ns - 1;
-  const int nF = D.nF;
   float pmax = 0.f;
-  LINES_BEGIN(R, NP, r, act)
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
-    float2 v[FC][L], os[FC][L];
+    float2 v[L], os[L], q[L];
+    load_row<NP>(v, W + f * Np2 + (size_t)r * NP, R.t);
+    load_row<NP>(os, O + (size_t)((f / npm) * ns + s) * No2 + orow_off, R.t);
+    fft_inv<NP>(v, R.tw, R.t);
+    if (act) store_row<NP>(P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v, R.t);
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) load_row<NP>(v[f], W + f * Np2 + (size_t)r * NP, R.t);
-      if (f < nF) load_row<NP>(os[f], O + (size_t)((f / npm) * ns + s) * No2 + orow_off, R.t);
+    for (int l = 0; l < L; ++l) {
+      q[l] = make_float2(act ? cabs2(v[l]) : 0.f, 0.f);
+      v[l] = cmul(v[l], os[l]);
     }
-    float sp[L];
+    lds_put<NP>(R.xa, islot, q, R.t);
+    if (act) store_row<NP>(E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v, R.t);
 #pragma unroll
-    for (int l = 0; l < L; ++l) sp[l] = 0.f;
+    for (int l = 0; l < L; ++l)
+      if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
+    fft_fwd<NP>(v, R.tw, R.t);
+    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    __syncthreads();
+    if (ITEM_LEAD(act, frw)) {
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        fft_inv<NP>(v[f], R.tw, R.t);
-        if (act) store_row<NP>(P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v[f], R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          sp[l] += cabs2(v[f][l]);
-          v[f][l] = cmul(v[f][l], os[f][l]);
-        }
-        if (act) store_row<NP>(E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v[f], R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l)
-          if (cb && ((r + l * LPF + R.t) & 1)) v[f][l] = make_float2(-v[f][l].x, -v[f][l].y);
-        fft_fwd<NP>(v[f], R.tw, R.t);
-        if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v[f], R.t);
+      for (int l = 0; l < L; ++l) {
+        float sp = 0.f;
+        for (int g = 0; g < nF; ++g) sp += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
+        pmax = fmaxf(pmax, sp);
       }
     }
-    if (act) {
-#pragma unroll
-      for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
-    }
-  LINES_END
+    __syncthreads();
+  ITEMS_END
   pmax = wave_max(pmax);
   if (R.lane == 0) atomic_fmax_slot(slot + 2 * s, pmax);
 }
@@ -850,17 +774,16 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
 // probe slice is forward-transformed into W for the next back-propagation.
 template <int NP>
 __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool fromDetector) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, FC = fcap<NP>();
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
-  const int nl = D.a.nlambda, npm = D.a.npsm, ns = D.a.nslice;
-  const int No = D.a.No;
+  const int npm = D.a.npsm, ns = D.a.nslice;
+  const int No = D.a.No, nF = D.nF;
   const size_t No2 = (size_t)No * No, Np2 = (size_t)NP * NP;
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
   float2* __restrict__ O = (float2*)D.a.object;
   float2* __restrict__ P = (float2*)D.a.probe;
   float2* __restrict__ W = D.W;
   const float2* __restrict__ E = D.ESW;
-  const int nF = D.nF;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER;
   const float invN = 1.0f / (float)NP;
   const float invP = 1.0f / load_slot(slot + 2 * su);
@@ -868,89 +791,57 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
   const float bO = D.a.betaObject;
   const float bP = su >= 1 ? D.a.sliceBeta : D.a.betaProbe;
   const int esl = fromDetector ? ns - 1 : su;
-  LINES_BEGIN(R, NP, r, act)
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
+    (void)frw;
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
-    float2 d[FC][L], pv[FC][L], ov[FC][L], ev[FC][L];
+    float2 d[L], pv[L], ov[L], ev[L];
+    load_row<NP>(d, W + f * Np2 + prow_off, R.t);
+    load_row<NP>(pv, P + (size_t)(f * ns + su) * Np2 + prow_off, R.t);
+    load_row<NP>(ev, E + (size_t)(esl * nF + f) * Np2 + prow_off, R.t);
+    load_row<NP>(ov, O + (size_t)((f / npm) * ns + su) * No2 + orow_off, R.t);
+    fft_inv<NP>(d, R.tw, R.t);
+    float2 tO[L], tP[L];
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        load_row<NP>(d[f], W + f * Np2 + prow_off, R.t);
-        load_row<NP>(pv[f], P + (size_t)(f * ns + su) * Np2 + prow_off, R.t);
-        load_row<NP>(ev[f], E + (size_t)(esl * nF + f) * Np2 + prow_off, R.t);
+    for (int l = 0; l < L; ++l) {
+      float2 u = d[l];
+      if (fromDetector) {
+        u = cscale(u, invN);
+        if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
       }
-      if (f < nF) load_row<NP>(ov[f], O + (size_t)((f / npm) * ns + su) * No2 + orow_off, R.t);
+      const float2 dl = csub(u, ev[l]);
+      tO[l] = cmul(make_float2(pv[l].x * invP, -pv[l].y * invP), dl);
+      tP[l] = cmul(make_float2(ov[l].x * invO, -ov[l].y * invO), dl);
     }
+    lds_put<NP>(R.xa, islot, tO, R.t);
+    lds_put<NP>(R.xb, islot, tP, R.t);
+    __syncthreads();
+    if (act && f % npm == 0) {
+      // object slice su of this lambda: sum over (lambda, psm) (axes (0, 2))
+      float2 nv[L];
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        fft_inv<NP>(d[f], R.tw, R.t);
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          float2 u = d[f][l];
-          if (fromDetector) {
-            u = cscale(u, invN);
-            if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
-          }
-          d[f][l] = csub(u, ev[f][l]);
-        }
+      for (int l = 0; l < L; ++l) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int g = 0; g < nF; ++g) acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
+        nv[l] = cadd(ov[l], cscale(acc, bO));
       }
+      store_row<NP>(O + (size_t)((f / npm) * ns + su) * No2 + orow_off, nv, R.t);
     }
-    // object increment: sum over (lambda, psm) of conj(P_su)/Pmax * delta
-    // (axes (0, 2)), the same for every lambda
-    {
-      float2 acc[L];
+    // probe slice su of this field: sum over lambda (axes (0, 1))
+    float2 nv[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int f = 0; f < FC; ++f) {
-        if (f < nF) {
-#pragma unroll
-          for (int l = 0; l < L; ++l) {
-            const float2 pq = pv[f][l];
-            acc[l] = cadd(acc[l], cmul(make_float2(pq.x * invP, -pq.y * invP), d[f][l]));
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < FC; ++q) {
-        if (q < nF && q % npm == 0) {  // one writer per lambda
-          float2 nv[L];
-#pragma unroll
-          for (int l = 0; l < L; ++l) nv[l] = cadd(ov[q][l], cscale(acc[l], bO));
-          if (act) store_row<NP>(O + (size_t)((q / npm) * ns + su) * No2 + orow_off, nv, R.t);
-        }
-      }
+    for (int l = 0; l < L; ++l) {
+      float2 acc = make_float2(0.f, 0.f);
+      for (int g = f % npm; g < nF; g += npm) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
+      nv[l] = cadd(pv[l], cscale(acc, bP));
     }
-    // probe increments: sum over lambda of conj(O_su)/Omax * delta (axes (0, 1))
-#pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      if (f < nF) {
-        const int ip = f % npm;
-        float2 acc[L];
-#pragma unroll
-        for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int g = 0; g < FC; ++g) {
-          if (g < nF && g % npm == ip) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-              const float2 oq = ov[g][l];
-              acc[l] = cadd(acc[l], cmul(make_float2(oq.x * invO, -oq.y * invO), d[g][l]));
-            }
-          }
-        }
-        float2 nv[L];
-#pragma unroll
-        for (int l = 0; l < L; ++l) nv[l] = cadd(pv[f][l], cscale(acc[l], bP));
-        if (act) store_row<NP>(P + (size_t)(f * ns + su) * Np2 + prow_off, nv, R.t);
-        if (su >= 1) {
-          fft_fwd<NP>(nv, R.tw, R.t);
-          if (act) store_row<NP>(W + f * Np2 + prow_off, nv, R.t);
-        }
-      }
+    if (act) store_row<NP>(P + (size_t)(f * ns + su) * Np2 + prow_off, nv, R.t);
+    if (su >= 1) {
+      fft_fwd<NP>(nv, R.tw, R.t);
+      if (act) store_row<NP>(W + f * Np2 + prow_off, nv, R.t);
     }
-  LINES_END
+    __syncthreads();
+  ITEMS_END
 }
 
 // ---------------------------------------------------------------------------
@@ -962,14 +853,14 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   pie_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if (D.a.propagator == PTY_ASP) {
-    tf_cols<NP>(R, D.nF, false);
+    tf_cols<NP>(R, false);
     if (!cluster_sync(R)) return false;
     asp_rows_constraint<NP>(R, k, j, ep);
     if (!cluster_sync(R)) return false;
-    tf_cols<NP>(R, D.nF, true);
+    tf_cols<NP>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
-    detector_cols_fraunhofer<NP>(R, k, j, ep, D.nF, D.fs);
+    detector_cols_fraunhofer<NP>(R, k, j, ep, D.fs);
     if (!cluster_sync(R)) return false;
   }
   pie_rows_update<NP>(R, k, slot);
@@ -983,17 +874,17 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
   e3_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   for (int s = 1; s < ns; ++s) {
-    tf_cols<NP>(R, D.nF, false);
+    tf_cols<NP>(R, false);
     if (!cluster_sync(R)) return false;
     e3_rows_forward<NP>(R, k, s, slot);
     if (!cluster_sync(R)) return false;
   }
-  detector_cols_fraunhofer<NP>(R, k, j, ep, D.nF, 1);
+  detector_cols_fraunhofer<NP>(R, k, j, ep, 1);
   if (!cluster_sync(R)) return false;
   e3_rows_update<NP>(R, k, ns - 1, slot, true);
   if (!cluster_sync(R)) return false;
   for (int s = ns - 1; s >= 1; --s) {
-    tf_cols<NP>(R, D.nF, true);
+    tf_cols<NP>(R, true);
     if (!cluster_sync(R)) return false;
     e3_rows_update<NP>(R, k, s - 1, slot, false);
     if (!cluster_sync(R)) return false;
@@ -1018,13 +909,13 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
     const int j1 = it == it1 - 1 ? (je < F ? je : F) : F;
     for (int j = j0; j < j1; ++j, ++q) {
       const int k = order[j];
-      unsigned* slot = D.ctrl + 16 + (q & 3u) * 32;
-      unsigned* zslot = D.ctrl + 16 + ((q + 2u) & 3u) * 32;
+      unsigned* slot = D.ctrl + CTRL_SLOTS + (q & 3u) * 32;
+      unsigned* zslot = D.ctrl + CTRL_SLOTS + ((q + 2u) & 3u) * 32;
       const bool ok = engine == PTY_E3PIE ? e3pie_position<NP>(R, k, j, slot, zslot, ep)
                                           : pie_position<NP>(R, k, j, slot, zslot, ep);
       if (!ok) return;
       if (momentum && D.a.momentum_mask[(size_t)it * F + j]) {
-        momentum_pass(R);
+        momentum_pass<NP>(R);
         if (!cluster_sync(R)) return;
       }
     }
@@ -1032,7 +923,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       // getErrorMetrics (BaseEngine.py:819-841): errorAtPos = sum|I - Iest| /
       // (energyAtPos + 1e-20) in FP64, error = sum over positions
       double acc = 0.0;
-      for (int jj = threadIdx.x; jj < F; jj += NTHREADS) {
+      for (int jj = threadIdx.x; jj < F; jj += Cfg<NP>::NT) {
         const int kk = order[jj];
         double s = 0.0;
         for (int g = 0; g < R.G; ++g) s += cloadd(ep + (size_t)jj * R.G + g);
@@ -1048,22 +939,26 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
 }
 
 // One kernel per probe size: each instantiation gets its own register
-// allocation.  A plan with several probe sizes launches one kernel per size,
-// concurrently on its own streams.
+// allocation and workgroup size.  A plan with several probe sizes launches
+// one kernel per size, concurrently on its own streams.
 template <int NP>
-__global__ __launch_bounds__(NTHREADS) void pty_iterate_kernel(const DevInst* __restrict__ insts,
-                                                               const int2* __restrict__ bmap, int it0, int it1,
-                                                               int jb, int je) {
+__global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst* __restrict__ insts,
+                                                                  const int2* __restrict__ bmap, int it0, int it1,
+                                                                  int jb, int je) {
   __shared__ float2 s_tw[256];
-  __shared__ double s_red[NWAVE];
-  __shared__ float s_redf[2 * NWAVE];
+  __shared__ double s_red[Cfg<NP>::NW];
+  __shared__ float2 s_xa[Cfg<NP>::IPR * NP];
+  __shared__ float2 s_xb[Cfg<NP>::IPR * NP];
   __shared__ int s_fail;
   const int2 bm = bmap[blockIdx.x];
+  if (bm.x < 0) return;
   const DevInst* D = insts + bm.x;
   Run R;
   R.D = D;
   R.g = bm.y;
   R.G = D->G;
+  R.nF = D->nF;
+  R.nFp = D->nFp;
   R.wave = threadIdx.x >> 6;
   R.lane = threadIdx.x & 63;
   R.t = R.lane & (Line<NP>::LPF - 1);
@@ -1071,16 +966,18 @@ __global__ __launch_bounds__(NTHREADS) void pty_iterate_kernel(const DevInst* __
   R.gen = 0u;
   R.tw = s_tw;
   R.red = s_red;
-  R.redf = s_redf;
+  R.nwave = Cfg<NP>::NW;
+  R.xa = s_xa;
+  R.xb = s_xb;
   R.fail = &s_fail;
-  for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
+  for (int i = threadIdx.x; i < NP / 2; i += Cfg<NP>::NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
   run_iterations<NP>(R, it0, it1, jb, je);
 }
 
 __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
   unsigned* c = insts[blockIdx.x].ctrl;
-  for (int i = threadIdx.x; i < 16 + 4 * 32; i += blockDim.x) c[i] = 0u;
+  for (int i = threadIdx.x; i < CTRL_WORDS; i += blockDim.x) c[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1306,10 +1203,13 @@ void free_plan(pty_plan* p) {
 
 int auto_workgroups(const pty_instance& a, int ninst) {
   if (ninst > 1) return 1;
-  const int sub = a.Np >= 64 ? 1 : 64 / a.Np;
-  int g = a.Np / (NWAVE * sub);
+  int nF = a.engine == PTY_E3PIE ? a.nlambda * a.npsm : a.nlambda * a.nosm * a.npsm * a.nslice;
+  int nFp = 1;
+  while (nFp < nF) nFp *= 2;
+  const int lines_per_round = slots_rt(a.Np) / nFp;
+  int g = a.Np / (lines_per_round > 0 ? lines_per_round : 1);
   if (g < 1) g = 1;
-  if (g > 64) g = 64;
+  if (g > 128) g = 128;
   return g;
 }
 
@@ -1328,7 +1228,7 @@ int validate(const pty_instance& a, int idx) {
   if (a.propagator == PTY_ASP && a.nlambda > 1)
     return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, set polychromeASP instead of ASP", idx);
   if (a.propagator == PTY_ASP && !a.tf) return set_err(PTY_EINVAL, "instance %d: ASP needs a transfer function", idx);
-  const int fc = fcap_rt(a.Np);
+  const int slots = slots_rt(a.Np);
   if (a.No % 2 != 0) return set_err(PTY_EINVAL, "instance %d: No=%d must be even", idx, a.No);
   if (a.engine == PTY_E3PIE) {
     if (a.nosm != 1) return set_err(PTY_EINVAL, "instance %d: e3PIE needs nosm == 1", idx);
@@ -1336,10 +1236,12 @@ int validate(const pty_instance& a, int idx) {
     if (a.propagator == PTY_ASP)
       return set_err(PTY_EINVAL, "instance %d: e3PIE supports the Fraunhofer detector propagator only", idx);
     if (!a.tf) return set_err(PTY_EINVAL, "instance %d: e3PIE needs the slice transfer function", idx);
-    if (a.nlambda * a.npsm > fc) return set_err(PTY_EINVAL, "instance %d: more than %d modes for Np=%d", idx, fc, a.Np);
+    if (a.nlambda * a.npsm > slots)
+      return set_err(PTY_EINVAL, "instance %d: more than %d modes for Np=%d", idx, slots, a.Np);
   } else {
     const int nF = a.nlambda * a.nosm * a.npsm * a.nslice;
-    if (nF > fc) return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed %d for Np=%d", idx, nF, fc, a.Np);
+    if (nF > slots)
+      return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed %d for Np=%d", idx, nF, slots, a.Np);
     if ((a.engine == PTY_MPIE || a.engine == PTY_MULTIPIE) && a.momentum_mask &&
         (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
       return set_err(PTY_EINVAL, "instance %d: momentum needs buffer and momentum arrays", idx);
@@ -1409,8 +1311,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     memset(&d, 0, sizeof d);
     d.a = a;
     int G = a.workgroups > 0 ? a.workgroups : auto_workgroups(a, ninst);
-    const int sub = a.Np >= 64 ? 1 : 64 / a.Np;
-    if (G > a.Np / sub) G = a.Np / sub;
+    if (G > a.Np) G = a.Np;
     if (G < 1) G = 1;
     while (a.Np % G) --G;
     d.G = G;
@@ -1423,6 +1324,8 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       d.nF = a.nlambda * a.nosm * a.npsm * a.nslice;
       d.fs = a.nslice;
     }
+    d.nFp = 1;
+    while (d.nFp < d.nF) d.nFp *= 2;
     d.nO = a.nlambda * a.nosm * a.nslice;
     d.nPr = a.nlambda * a.npsm * a.nslice;
     d.omAlphaO = (float)(1.0 - (double)a.alphaObject);
@@ -1455,7 +1358,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       if ((rc = dalloc(p, &mem, (size_t)a.nslice * d.nF * np2 * sizeof(float2)))) goto fail;
       d.ESW = (float2*)mem;
     }
-    if ((rc = dalloc(p, &mem, (16 + 4 * 32) * sizeof(unsigned)))) goto fail;
+    if ((rc = dalloc(p, &mem, CTRL_WORDS * sizeof(unsigned)))) goto fail;
     d.ctrl = (unsigned*)mem;
     if ((rc = dalloc(p, &mem, (size_t)2 * a.F * G * sizeof(double)))) goto fail;
     d.errpart = (double*)mem;
@@ -1483,7 +1386,11 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       ci = (int)p->classes.size() - 1;
     }
     if (G > 1) p->classes[ci].cooperative = true;
-    for (int g = 0; g < G; ++g) cmaps[ci].push_back(make_int2(i, g));
+    for (int g = 0; g < G; ++g) {
+      d.gsize[cmaps[ci].size() & 7] += 1;
+      cmaps[ci].push_back(make_int2(i, g));
+    }
+    for (int x = 0; x < 8; ++x) d.ngroups += d.gsize[x] > 0;
   }
   {
     void* mem = nullptr;
@@ -1503,7 +1410,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       HIP_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
       if (lc.cooperative) {
         int per_cu = 0;
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np), NTHREADS, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np), nthreads_rt(lc.np), 0));
         if (!coop) {
           rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
           goto fail;
@@ -1562,9 +1469,9 @@ static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb
     const int2* bm = lc.d_bmap;
     void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1};
     if (lc.cooperative)
-      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(NTHREADS), args, 0, ls));
+      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
     else
-      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(NTHREADS), args, 0, ls));
+      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
     if (fork) {
       HIP_TRY(hipEventRecord(lc.done, ls));
       HIP_TRY(hipStreamWaitEvent(st, lc.done, 0));
@@ -1579,7 +1486,7 @@ int pty_plan_check(pty_plan* p, void* hip_stream) {
   HIP_TRY(hipStreamSynchronize(st));
   for (size_t i = 0; i < p->h_insts.size(); ++i) {
     unsigned flag = 0;
-    HIP_TRY(hipMemcpy(&flag, p->h_insts[i].ctrl + 1, sizeof flag, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&flag, p->h_insts[i].ctrl + CTRL_FAIL, sizeof flag, hipMemcpyDeviceToHost));
     if (flag) return set_err(PTY_ETIMEDOUT, "instance %zu: a cluster barrier timed out", i);
   }
   return PTY_OK;
