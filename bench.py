@@ -45,6 +45,8 @@ def parse_args():
     ap.add_argument("--cpu-positions", type=int, default=1000)
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--F", type=int, default=1000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy before the timed region (profiler counter scale)")
     return ap.parse_args()
 
 
@@ -127,6 +129,12 @@ def main():
     plan, T = eng._begin()
     stream = torch.cuda.current_stream()
     F = data.numFrames
+    if args.calibrate_copy:
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for it in range(W):
         plan.run(it, it + 1)
     plan.check()

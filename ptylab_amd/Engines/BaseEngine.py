@@ -120,7 +120,7 @@ class BaseEngine:
 
     def _schedule(self, n_iter, momentum):
         F = self.experimentalData.numFrames
-        n_err = len(self.reconstruction.error)
+        n_err = len(getattr(self.reconstruction, "error", []))
         orders = np.empty((n_iter, F), dtype=np.int32)
         masks = np.zeros((n_iter, F), dtype=np.uint8)
         for it in range(n_iter):

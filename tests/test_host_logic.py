@@ -1,0 +1,131 @@
+"""Host-side logic of the product package against the oracle and the
+reference's documented behaviour (no GPU needed)."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+import ptylab_amd as P
+from ptyhelpers import make_data
+from regression_inputs import SEED, regression_dataset
+
+
+def test_params_defaults_match_reference():
+    p = P.Params()
+    # Params.py:41-195 defaults of the fields the hot path reads
+    assert p.propagatorType == "Fraunhofer" and p.positionOrder == "random"
+    assert p.intensityConstraint == "standard" and p.fftshiftSwitch is False
+    assert p.orthogonalizationSwitch is False and p.orthogonalizationFrequency == 10
+    assert p.saveMemory is False and p.probePowerCorrectionSwitch is False
+
+
+def test_geometry_positions_and_init_match_oracle():
+    d = regression_dataset()
+    data = make_data(d)
+    rec = P.Reconstruction(data, P.Params())
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"])
+    assert rec.No == geo.No and rec.Np == geo.Np
+    assert np.array_equal(rec.positions, geo.positions)
+    assert rec.dxp == geo.dxp
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    np.random.seed(SEED)
+    obj = O.initial_guess(geo.shape_O, "ones", geo)
+    pr = O.initial_guess(geo.shape_P, "circ", geo)
+    assert np.array_equal(rec.object, obj) and np.array_equal(rec.probe, pr)
+    assert np.array_equal(data.energyAtPos, O.energy_at_pos(d["ptychogram"]))
+
+
+def test_asp_transfer_function_matches_oracle():
+    from ptylab_amd.Operators import asp_tf_fftorder, aspw_transfer_function
+
+    for z in (5e-2, -1e-3, 0.0):
+        a = aspw_transfer_function(z, 632.8e-9, 64, 64 * 4e-6)
+        b = O.aspw_transfer(z, 632.8e-9, 64, 64 * 4e-6)
+        assert np.array_equal(a, b)
+    stack = O.asp_transfer_stack(1, 1, 64, 5e-2, 632.8e-9, 64 * 4e-6)
+    assert np.array_equal(asp_tf_fftorder(5e-2, 632.8e-9, 64, 64 * 4e-6),
+                          np.fft.ifftshift(stack, axes=(-2, -1))[0, 0, 0, 0])
+
+
+@pytest.mark.parametrize("engine", ["ePIE", "mPIE"])
+@pytest.mark.parametrize("order", ["sequential", "random"])
+def test_schedule_replays_reference_rng_consumption(engine, order):
+    """The host draws every iteration's order and the mPIE momentum triggers
+    up front; the draws and the RNG state afterwards equal the reference's
+    interleaved calls (one shuffle per iteration once two errors exist, one
+    rand(1) per position)."""
+    d = regression_dataset()
+    data = make_data(d)
+    params = P.Params()
+    params.positionOrder = order
+    rec = P.Reconstruction(data, params)
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    eng = getattr(P.Engines, engine)(rec, data, params, P.DummyMonitor())
+    F, n_iter = data.numFrames, 5
+    np.random.seed(7)
+    orders, masks = eng._schedule(n_iter, engine == "mPIE")
+    state_after = np.random.get_state()[1].copy()
+    np.random.seed(7)
+    for it in range(n_iter):
+        ref = O.position_order(order, F, it)
+        assert np.array_equal(orders[it], ref)
+        if engine == "mPIE":
+            trig = [bool(np.random.rand(1) > 0.95) for _ in range(F)]
+            assert np.array_equal(masks[it].astype(bool), trig)
+    assert np.array_equal(np.random.get_state()[1], state_after)
+
+
+def test_unsupported_switches_raise():
+    d = regression_dataset()
+    data = make_data(d)
+    params = P.Params()
+    params.backgroundModeSwitch = True
+    rec = P.Reconstruction(data, params)
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    eng = P.Engines.ePIE(rec, data, params, P.DummyMonitor())
+    with pytest.raises(NotImplementedError):
+        eng._prepareReconstruction()
+    params.backgroundModeSwitch = False
+    params.propagatorType = "ASP"
+    params.fftshiftSwitch = True
+    with pytest.raises(ValueError):
+        eng._prepareReconstruction()
+    params.fftshiftSwitch = False
+    params.propagatorType = "Fresnel"
+    with pytest.raises(NotImplementedError):
+        eng._prepareReconstruction()
+    params.propagatorType = "Fraunhofer"
+    params.intensityConstraint = "bogus"
+    with pytest.raises(ValueError):
+        eng._prepareReconstruction()
+
+
+def test_reconstruct_without_gpu_fails_loudly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    d = regression_dataset()
+    data = make_data(d)
+    params = P.Params()
+    rec = P.Reconstruction(data, params)
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    eng = P.Engines.ePIE(rec, data, params, P.DummyMonitor())
+    eng.numIterations = 1
+    with pytest.raises(RuntimeError, match="GPU"):
+        eng.reconstruct()
+
+
+def test_synthetic_encoder_round_trip():
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=64, No=256, F=100, seed=0)
+    data = make_data(w.as_dict())
+    rec = P.Reconstruction(data, P.Params())
+    rec.No = w.No
+    assert np.array_equal(rec.positions, w.positions)
+    assert w.positions.min() >= 0 and w.positions.max() + 64 <= 256

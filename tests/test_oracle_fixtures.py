@@ -1,0 +1,116 @@
+"""The oracle against the committed fixtures and against what the
+mathematics fixes (closed forms, invariants, brute force on tiny inputs)."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from golden.make_goldens import engine_case, propagator_case
+from regression_inputs import ENGINE_CASES
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.parametrize("name", list(ENGINE_CASES))
+def test_oracle_engine_matches_committed_fixture(name):
+    got = engine_case(name)
+    ref = np.load(GOLDEN / f"oracle_{name}.npz")
+    for k in ("object", "probe", "error"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-6, atol=1e-9, err_msg=f"{name}:{k}")
+
+
+@pytest.mark.parametrize("prop", ["fraunhofer", "asp"])
+def test_oracle_propagator_matches_committed_fixture(prop):
+    got = propagator_case(prop)
+    ref = np.load(GOLDEN / f"oracle_propagator_{prop}.npz")
+    for k in ("forward", "back"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-6, atol=1e-9)
+
+
+def _dft2_brute(x):
+    n = x.shape[-1]
+    k = np.arange(n)
+    Wm = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    return Wm @ x @ Wm.T
+
+
+@pytest.mark.parametrize("n", [4, 8, 12])
+def test_fft2c_is_the_shifted_unitary_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    want = np.fft.fftshift(_dft2_brute(np.fft.ifftshift(x))) / n
+    np.testing.assert_allclose(O.fft2c(x), want, atol=1e-12)
+    np.testing.assert_allclose(O.ifft2c(O.fft2c(x)), x, atol=1e-12)
+    assert np.isclose(np.sum(abs(O.fft2c(x)) ** 2), np.sum(abs(x) ** 2))
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_checkerboard_identity_used_by_the_kernels(n):
+    """For n % 4 == 0 the centred transform equals c * F(c * x) with
+    c = (-1)^(y+x): the shifts fold into sign flips, so the GPU path never
+    permutes data for fft2c / ifft2c."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    c = (-1.0) ** (yy + xx)
+    np.testing.assert_allclose(O.fft2c(x), c * np.fft.fft2(c * x, norm="ortho"), atol=1e-12)
+    np.testing.assert_allclose(O.ifft2c(x), c * np.fft.ifft2(c * x, norm="ortho"), atol=1e-12)
+
+
+def test_asp_at_zero_distance_is_identity_and_band_limit_closed_form():
+    N, L, wl = 64, 64 * 4e-6, 632.8e-9
+    H0 = O.aspw_transfer(0.0, wl, N, L)
+    np.testing.assert_allclose(H0, np.ones((N, N)), atol=0)
+    z = 5e-2
+    H = O.aspw_transfer(z, wl, N, L)
+    f = np.arange(-N / 2, N / 2) / L
+    Fx, Fy = np.meshgrid(f, f)
+    fmax = L / (wl * np.sqrt(L ** 2 + 4 * z ** 2))
+    inside = Fx ** 2 + Fy ** 2 < fmax ** 2
+    assert np.array_equal(np.abs(H) > 0.5, inside)
+    np.testing.assert_allclose(np.abs(H[inside]), 1.0, atol=1e-12)
+    np.testing.assert_allclose(O.aspw_transfer(-z, wl, N, L), H.conj(), atol=0)
+
+
+def test_multipie_equals_mpie_without_wavelength_or_object_modes():
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=32, No=96, F=12, seed=3, step_frac=0.3)
+    d = w.as_dict()
+    runs = []
+    for eng in ("mPIE", "multiPIE"):
+        geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"],
+                         d["entrancePupilDiameter"], No=w.No, npsm=2)
+        np.random.seed(4)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        r = O.OracleRun(geo, obj, pr, engine=eng, numIterations=3)
+        np.random.seed(5)
+        runs.append(r.reconstruct())
+    assert np.array_equal(runs[0].object, runs[1].object)
+    assert np.array_equal(runs[0].probe, runs[1].probe)
+
+
+def test_orthogonalize_modes_invariants():
+    rng = np.random.default_rng(0)
+    p = (rng.standard_normal((4, 16, 16)) + 1j * rng.standard_normal((4, 16, 16)))
+    modes, ev, V = O.orthogonalize_modes(p)
+    m2 = modes.reshape(4, -1)
+    G = m2.conj() @ m2.T
+    assert np.abs(G - np.diag(np.diag(G))).max() < 1e-9 * np.abs(np.diag(G)).max()
+    d = np.diag(G).real
+    assert np.all(np.diff(d) <= 1e-9 * d.max())
+    np.testing.assert_allclose(ev.sum(), 1.0)
+    p2 = p.reshape(4, -1)
+    Q, _ = np.linalg.qr(m2.T)
+    np.testing.assert_allclose(Q @ (Q.conj().T @ p2.T), p2.T, atol=1e-9)
+
+
+def test_nrmse_closed_form():
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal((20, 20)) + 1j * rng.standard_normal((20, 20))
+    assert O.nrmse(t * (0.3 - 2j), t) < 1e-12
+    noise = 1e-3 * (rng.standard_normal((20, 20)))
+    assert 1e-5 < O.nrmse(t + noise, t) < 1e-2
