@@ -96,6 +96,11 @@ def load(path: os.PathLike | None = None):
         raise RuntimeError(
             f"{p} is missing: build it with `python -c 'import ptylab_amd.build as b; b.build()'`"
         )
+    # torch ships its own libamdhip64.so.7; loading torch first makes the
+    # library bind to that same HIP runtime (one runtime per process), so
+    # device pointers and streams from torch are valid in the library
+    import torch  # noqa: F401
+
     lib = ctypes.CDLL(str(p))
     for name in EXPORTED:
         getattr(lib, name)  # raises AttributeError if absent

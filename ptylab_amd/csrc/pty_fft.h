@@ -1,11 +1,14 @@
 // Wavefront-resident FFT building blocks for gfx950 (wave64).
 //
-// A length-N line (N = L * LPF, LPF = min(N, 64) lanes per line, L = N / LPF
-// complex values per lane) lives in registers: element n = l * LPF + t sits in
-// register l of lane t of its lane group.  Butterflies whose span is a
-// multiple of LPF are done inside a lane; smaller spans exchange values
-// between lanes with one ds_bpermute per float (__shfl_xor).  No LDS and no
-// workgroup barrier is needed inside a transform.
+// A length-N line lives in registers of a group of LPF lanes: element
+// n = l * LPF + t sits in register l of the lane whose in-line index is t
+// (L = N / LPF registers per lane).  Consecutive in-line indices are LS
+// lanes apart in the wave (LS = 1: a line occupies adjacent lanes; LS = CW:
+// CW lines interleaved lane by lane, as the column passes use).
+// Butterflies whose span is a multiple of LPF are done inside a lane; smaller
+// spans exchange values between lanes with one ds_bpermute per float
+// (__shfl_xor with mask span * LS).  No LDS and no workgroup barrier is
+// needed inside a transform.
 //
 // fft_fwd: radix-2 decimation in frequency, natural order in, bit-reversed
 //          order out, kernel exp(-2 pi i nk / N), unnormalised.
@@ -36,16 +39,20 @@ __device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
 
 template <int N>
 struct Line {
-  static constexpr int LPF = N >= 64 ? 64 : N;  // lanes per line
+  static constexpr int LPF = N >= 64 ? 64 : N;  // lanes per line (row layout)
   static constexpr int L = N / LPF;             // values per lane
   static constexpr int SUB = 64 / LPF;          // lines per wave
   static constexpr int LOG2 = N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6 : N == 128 ? 7 : N == 256 ? 8 : 9;
 };
 
-template <int N>
-__device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
-  constexpr int L = Line<N>::L;
-  constexpr int LPF = Line<N>::LPF;
+template <int N, int LPF, int LS>
+__device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t) {
+  constexpr int L = N / LPF;
+  // Opaque zero offset: keeps the compiler from hoisting the (loop-invariant)
+  // twiddle loads out of the caller's loops into registers.
+  unsigned off = 0;
+  __asm__ volatile("" : "+s"(off));
+  const float2* __restrict__ tw = tw_ + off;
 #pragma unroll
   for (int sr = L / 2; sr >= 1; sr >>= 1) {
 #pragma unroll
@@ -66,16 +73,20 @@ __device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* _
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const float2 x = v[l];
-      const float2 p = shfl_xor2(x, s);
+      const float2 p = shfl_xor2(x, s * LS);
       v[l] = up ? cmul(csub(p, x), w) : cadd(x, p);
     }
   }
 }
 
-template <int N>
-__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
-  constexpr int L = Line<N>::L;
-  constexpr int LPF = Line<N>::LPF;
+template <int N, int LPF, int LS>
+__device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t) {
+  constexpr int L = N / LPF;
+  // Opaque zero offset: keeps the compiler from hoisting the (loop-invariant)
+  // twiddle loads out of the caller's loops into registers.
+  unsigned off = 0;
+  __asm__ volatile("" : "+s"(off));
+  const float2* __restrict__ tw = tw_ + off;
 #pragma unroll
   for (int s = 1; s < LPF; s <<= 1) {
     const bool up = (t & s) != 0;
@@ -83,7 +94,7 @@ __device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* _
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const float2 x = up ? cmul(v[l], w) : v[l];
-      const float2 p = shfl_xor2(x, s);
+      const float2 p = shfl_xor2(x, s * LS);
       v[l] = up ? csub(p, x) : cadd(x, p);
     }
   }
@@ -101,6 +112,16 @@ __device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* _
       }
     }
   }
+}
+
+// row layout: a line on LPF adjacent lanes
+template <int N>
+__device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
+  fft_fwd_g<N, Line<N>::LPF, 1>(v, tw, t);
+}
+template <int N>
+__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
+  fft_inv_g<N, Line<N>::LPF, 1>(v, tw, t);
 }
 
 __device__ __forceinline__ int brev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }

@@ -55,6 +55,11 @@ using namespace pty;
 #define CTRL_RELEASE 416   // published barrier generation
 #define CTRL_WORDS 448
 
+// pass families of the persistent kernel
+#define FAM_PIE_FRAUNHOFER 0
+#define FAM_PIE_ASP 1
+#define FAM_E3PIE 2
+
 namespace {
 
 thread_local std::string g_err;
@@ -291,6 +296,12 @@ struct Cfg {
   static constexpr int NW = NT / 64;
   static constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
   static constexpr int IPR = NW * SUB;  // item slots per round
+  // LDS exchange: row passes use two (IPR, NP) complex buffers, column
+  // passes one (NW, values per lane, 64) float buffer (at most NW * NP * 8
+  // floats for 8-column groups)
+  static constexpr int XROW = 2 * IPR * NP;
+  static constexpr int XCOL = NW * NP * 4;
+  static constexpr int XBUF = XROW > XCOL ? XROW : XCOL;  // float2 elements
 };
 
 #define ITEMS_BEGIN(R, NP, line, f, act, islot, lslot, frw)                     \
@@ -402,77 +413,138 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   }
 }
 
+// Column passes: a wave owns a group of CW adjacent columns of one field
+// (lane = rho * CW + c, rows n = l * LPC + rho with LPC = 64 / CW lanes per
+// column), so every load and store instruction moves contiguous row
+// segments of CW * 8 bytes; CW = 1 is a column on 64 adjacent lanes.
+// Workgroup g owns column groups [g * cpg, (g + 1) * cpg) of the Np / CW
+// groups (workgroups past the last group idle in column passes); its waves
+// take (group, field) items, groups per round = waves / nFp.
+template <int NP>
+struct Col {
+#ifndef PTY_COLUMN_WIDTH
+  static constexpr int CW0 = NP >= 256 ? 2 : 1;
+#else
+  static constexpr int CW0 = PTY_COLUMN_WIDTH;
+#endif
+  static constexpr int CW = CW0 * NP >= 64 ? CW0 : 64 / NP;  // a column spans at most NP lanes
+  static constexpr int LPC = 64 / CW;                         // lanes per column
+  static constexpr int CL = NP / LPC;                         // values per lane
+  static constexpr int NG = NP / CW;                          // column groups
+};
+
+#define CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)                         \
+  {                                                                             \
+    const int ng_ = Col<NP>::NG;                                                \
+    const int cpg_ = (ng_ + (R).G - 1) / (R).G;                                 \
+    const int cg0_ = (R).g * cpg_;                                              \
+    const int cgn_ = cg0_ < ng_ ? (ng_ - cg0_ < cpg_ ? ng_ - cg0_ : cpg_) : 0; \
+    const int gpr_ = Cfg<NP>::NW / (R).nFp;                                     \
+    const int islot = (R).wave;                                                 \
+    const int lg_ = islot / (R).nFp;                                            \
+    const int frw = islot - lg_ * (R).nFp;                                      \
+    const int lslot = lg_ * (R).nFp;                                            \
+    const int f = frw < (R).nF ? frw : (R).nF - 1;                              \
+    for (int base_ = 0; base_ < cgn_; base_ += gpr_) {                          \
+      const int gi_ = base_ + lg_;                                              \
+      const bool gok_ = gi_ < cgn_;                                             \
+      const bool act = gok_ && frw < (R).nF;                                    \
+      const int p = (cg0_ + (gok_ ? gi_ : cgn_ - 1)) * Col<NP>::CW + (R).lane % Col<NP>::CW;
+#define CITEMS_END \
+  }                \
+  }
+
+template <int NP>
+__device__ __forceinline__ int col_rho(const Run& R) { return R.lane / Col<NP>::CW; }
+
+template <int NP>
+__device__ __forceinline__ void load_colg(float2 (&v)[Col<NP>::CL], const float2* f, int p, int rho) {
+#pragma unroll
+  for (int l = 0; l < Col<NP>::CL; ++l) v[l] = cload(f + (size_t)(l * Col<NP>::LPC + rho) * NP + p);
+}
+template <int NP>
+__device__ __forceinline__ void store_colg(float2* f, const float2 (&v)[Col<NP>::CL], int p, int rho) {
+#pragma unroll
+  for (int l = 0; l < Col<NP>::CL; ++l) cstore(f + (size_t)(l * Col<NP>::LPC + rho) * NP + p, v[l]);
+}
+template <int NP>
+__device__ __forceinline__ void fft_fwd_colg(float2 (&v)[Col<NP>::CL], const float2* tw, int rho) {
+  fft_fwd_g<NP, Col<NP>::LPC, Col<NP>::CW>(v, tw, rho);
+}
+template <int NP>
+__device__ __forceinline__ void fft_inv_colg(float2 (&v)[Col<NP>::CL], const float2* tw, int rho) {
+  fft_inv_g<NP, Col<NP>::LPC, Col<NP>::CW>(v, tw, rho);
+}
+
 // Fraunhofer detector column pass: forward column transform, unitary scale,
 // intensity constraint with the measured frame, inverse column transform.
 // Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
 // BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
 template <int NP>
 __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  constexpr int CL = Col<NP>::CL, LPC = Col<NP>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
   const float invN = 1.0f / (float)NP;
   const int nF = D.nF;
+  const int rho = col_rho<NP>(R);
+  float* xs = (float*)R.xa;
   double err = 0.0;
-  ITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
-    float2 v[L];
-    float im[L];
-    load_col<NP>(v, W + f * Np2, p, R.t);
+  CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
+    float2 v[CL];
+    load_colg<NP>(v, W + f * Np2, p, rho);
+    fft_fwd_colg<NP>(v, R.tw, rho);
 #pragma unroll
-    for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)p * NP + l * LPF + R.t);
-    fft_fwd<NP>(v, R.tw, R.t);
-    float2 q[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < CL; ++l) {
       v[l] = cscale(v[l], invN);
-      q[l] = make_float2(act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f, 0.f);
+      xs[(islot * CL + l) * 64 + R.lane] = act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f;
     }
-    lds_put<NP>(R.xa, islot, q, R.t);
     __syncthreads();
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
+    for (int l = 0; l < CL; ++l) {
+      const float im = ldg(Ik + (size_t)(l * LPC + rho) * NP + p);
       float iest = 0.f;
-      for (int g = 0; g < nF; ++g) iest += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
-      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest);
-      v[l] = cscale(v[l], sqrtf(im[l] / (iest + 1e-10f)));
+      for (int g = 0; g < nF; ++g) iest += xs[((lslot + g) * CL + l) * 64 + R.lane];
+      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im - iest);
+      v[l] = cscale(v[l], sqrtf(im / (iest + 1e-10f)));
     }
-    fft_inv<NP>(v, R.tw, R.t);
-    if (act) store_col<NP>(W + f * Np2, v, p, R.t);
+    fft_inv_colg<NP>(v, R.tw, rho);
+    if (act) store_colg<NP>(W + f * Np2, v, p, rho);
     __syncthreads();
-  ITEMS_END
+  CITEMS_END
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
 }
 
 // Column pass of an angular-spectrum propagation: forward column transform,
-// multiply by the (transposed, bit-reversed, 1/Np^2-scaled) transfer function
-// or its conjugate, inverse column transform.  Fields are independent.
+// multiply by the (bit-reversed, 1/Np^2-scaled) transfer function or its
+// conjugate, inverse column transform.  Fields are independent.
 template <int NP>
 __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
-  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  constexpr int CL = Col<NP>::CL, LPC = Col<NP>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float2* __restrict__ HT = D.HT;
-  ITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
+  const int rho = col_rho<NP>(R);
+  CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     (void)islot;
     (void)lslot;
     (void)frw;
-    float2 v[L], h[L];
-    load_col<NP>(v, W + f * Np2, p, R.t);
+    float2 v[CL];
+    load_colg<NP>(v, W + f * Np2, p, rho);
+    fft_fwd_colg<NP>(v, R.tw, rho);
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-      h[l] = ldg2(HT + (size_t)p * NP + l * LPF + R.t);
-      if (conj) h[l] = conjf2(h[l]);
+    for (int l = 0; l < CL; ++l) {
+      float2 h = ldg2(HT + (size_t)(l * LPC + rho) * NP + p);
+      if (conj) h = conjf2(h);
+      v[l] = cmul(v[l], h);
     }
-    fft_fwd<NP>(v, R.tw, R.t);
-#pragma unroll
-    for (int l = 0; l < L; ++l) v[l] = cmul(v[l], h[l]);
-    fft_inv<NP>(v, R.tw, R.t);
-    if (act) store_col<NP>(W + f * Np2, v, p, R.t);
-  ITEMS_END
+    fft_inv_colg<NP>(v, R.tw, rho);
+    if (act) store_colg<NP>(W + f * Np2, v, p, rho);
+  CITEMS_END
 }
 
 // ASP detector row pass: inverse row transform gives the detector field, the
@@ -847,12 +919,12 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 // per-position drivers
 // ---------------------------------------------------------------------------
-template <int NP>
+template <int NP, bool ASP>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
   const DevInst& D = *R.D;
   pie_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
-  if (D.a.propagator == PTY_ASP) {
+  if constexpr (ASP) {
     tf_cols<NP>(R, false);
     if (!cluster_sync(R)) return false;
     asp_rows_constraint<NP>(R, k, j, ep);
@@ -895,7 +967,7 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
 // Iterations [it0, it1); the first iteration starts at position jb and the
 // last one stops before position je (full iterations: jb = 0, je >= F).  The
 // error of an iteration is reduced when its last position has run.
-template <int NP>
+template <int NP, int FAM>
 __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
   const DevInst& D = *R.D;
   const int F = D.a.F;
@@ -911,10 +983,13 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       const int k = order[j];
       unsigned* slot = D.ctrl + CTRL_SLOTS + (q & 3u) * 32;
       unsigned* zslot = D.ctrl + CTRL_SLOTS + ((q + 2u) & 3u) * 32;
-      const bool ok = engine == PTY_E3PIE ? e3pie_position<NP>(R, k, j, slot, zslot, ep)
-                                          : pie_position<NP>(R, k, j, slot, zslot, ep);
+      bool ok;
+      if constexpr (FAM == FAM_E3PIE)
+        ok = e3pie_position<NP>(R, k, j, slot, zslot, ep);
+      else
+        ok = pie_position<NP, FAM == FAM_PIE_ASP>(R, k, j, slot, zslot, ep);
       if (!ok) return;
-      if (momentum && D.a.momentum_mask[(size_t)it * F + j]) {
+      if (FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j]) {
         momentum_pass<NP>(R);
         if (!cluster_sync(R)) return;
       }
@@ -938,17 +1013,18 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
   }
 }
 
-// One kernel per probe size: each instantiation gets its own register
-// allocation and workgroup size.  A plan with several probe sizes launches
-// one kernel per size, concurrently on its own streams.
-template <int NP>
+// One kernel per probe size and pass family (PIE with the Fraunhofer
+// detector, PIE with ASP, e3PIE): each instantiation carries only its own
+// passes and gets its own register allocation and workgroup size.  A plan
+// with several classes launches one kernel per class, concurrently on its
+// own streams.
+template <int NP, int FAM>
 __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst* __restrict__ insts,
                                                                   const int2* __restrict__ bmap, int it0, int it1,
                                                                   int jb, int je) {
   __shared__ float2 s_tw[256];
   __shared__ double s_red[Cfg<NP>::NW];
-  __shared__ float2 s_xa[Cfg<NP>::IPR * NP];
-  __shared__ float2 s_xb[Cfg<NP>::IPR * NP];
+  __shared__ float2 s_x[Cfg<NP>::XBUF];
   __shared__ int s_fail;
   const int2 bm = bmap[blockIdx.x];
   if (bm.x < 0) return;
@@ -967,12 +1043,12 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   R.tw = s_tw;
   R.red = s_red;
   R.nwave = Cfg<NP>::NW;
-  R.xa = s_xa;
-  R.xb = s_xb;
+  R.xa = s_x;
+  R.xb = s_x + Cfg<NP>::IPR * NP;
   R.fail = &s_fail;
   for (int i = threadIdx.x; i < NP / 2; i += Cfg<NP>::NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
-  run_iterations<NP>(R, it0, it1, jb, je);
+  run_iterations<NP, FAM>(R, it0, it1, jb, je);
 }
 
 __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
@@ -983,7 +1059,8 @@ __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
 // ---------------------------------------------------------------------------
 // layout preparation kernels (plan creation only)
 // ---------------------------------------------------------------------------
-// dst[f][p][n] = src[f][brev(n)][brev(p)]  (measured frame for the column pass)
+// dst[f][n][p] = src[f][brev(n)][brev(p)]: the measured frame in the order
+// the column pass holds the transformed field (rows and columns bit-reversed)
 __global__ void pty_permute_frames_kernel(const float* __restrict__ src, float* __restrict__ dst, int F, int np,
                                           int lg) {
   const size_t np2 = (size_t)np * np;
@@ -991,17 +1068,17 @@ __global__ void pty_permute_frames_kernel(const float* __restrict__ src, float* 
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
     const size_t f = i / np2;
     const int rem = (int)(i % np2);
-    const int p = rem / np, n = rem % np;
+    const int n = rem / np, p = rem % np;
     dst[i] = src[f * np2 + (size_t)brev(n, lg) * np + brev(p, lg)];
   }
 }
 
-// HT[p][n] = tf[brev(n)][brev(p)] / np^2 (np^2 is a power of two: exact)
+// ht[n][p] = tf[brev(n)][brev(p)] / np^2 (np^2 is a power of two: exact)
 __global__ void pty_permute_tf_kernel(const float2* __restrict__ tf, float2* __restrict__ ht, int np, int lg) {
   const int total = np * np;
   const float sc = 1.0f / ((float)np * (float)np);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-    const int p = i / np, n = i % np;
+    const int n = i / np, p = i % np;
     ht[i] = cscale(tf[brev(n, lg) * np + brev(p, lg)], sc);
   }
 }
@@ -1129,6 +1206,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_rows_inv_kernel(float2* __restri
 // instantiation, one block map, one launch per run.
 struct LaunchClass {
   int np = 0;
+  int fam = 0;
   int nblocks = 0;
   bool cooperative = false;
   int2* d_bmap = nullptr;
@@ -1147,16 +1225,31 @@ struct pty_plan {
   std::vector<void*> allocs;
 };
 
-static const void* iterate_kernel_ptr(int np) {
-  switch (np) {
-    case 16: return (const void*)pty_iterate_kernel<16>;
-    case 32: return (const void*)pty_iterate_kernel<32>;
-    case 64: return (const void*)pty_iterate_kernel<64>;
-    case 128: return (const void*)pty_iterate_kernel<128>;
-    case 256: return (const void*)pty_iterate_kernel<256>;
-    case 512: return (const void*)pty_iterate_kernel<512>;
+template <int NP>
+static const void* iterate_kernel_fam(int fam) {
+  switch (fam) {
+    case FAM_PIE_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_PIE_FRAUNHOFER>;
+    case FAM_PIE_ASP: return (const void*)pty_iterate_kernel<NP, FAM_PIE_ASP>;
+    case FAM_E3PIE: return (const void*)pty_iterate_kernel<NP, FAM_E3PIE>;
   }
   return nullptr;
+}
+
+static const void* iterate_kernel_ptr(int np, int fam) {
+  switch (np) {
+    case 16: return iterate_kernel_fam<16>(fam);
+    case 32: return iterate_kernel_fam<32>(fam);
+    case 64: return iterate_kernel_fam<64>(fam);
+    case 128: return iterate_kernel_fam<128>(fam);
+    case 256: return iterate_kernel_fam<256>(fam);
+    case 512: return iterate_kernel_fam<512>(fam);
+  }
+  return nullptr;
+}
+
+static int family_of(const pty_instance& a) {
+  if (a.engine == PTY_E3PIE) return FAM_E3PIE;
+  return a.propagator == PTY_ASP ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
 }
 
 namespace {
@@ -1377,11 +1470,13 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       d.HT = (const float2*)mem;
     }
     int ci = -1;
+    const int fam = family_of(a);
     for (size_t c = 0; c < p->classes.size(); ++c)
-      if (p->classes[c].np == a.Np) ci = (int)c;
+      if (p->classes[c].np == a.Np && p->classes[c].fam == fam) ci = (int)c;
     if (ci < 0) {
       p->classes.emplace_back();
       p->classes.back().np = a.Np;
+      p->classes.back().fam = fam;
       cmaps.emplace_back();
       ci = (int)p->classes.size() - 1;
     }
@@ -1410,7 +1505,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       HIP_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
       if (lc.cooperative) {
         int per_cu = 0;
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np), nthreads_rt(lc.np), 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np, lc.fam), nthreads_rt(lc.np), 0));
         if (!coop) {
           rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
           goto fail;
@@ -1469,9 +1564,9 @@ static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb
     const int2* bm = lc.d_bmap;
     void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1};
     if (lc.cooperative)
-      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
+      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
     else
-      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
+      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
     if (fork) {
       HIP_TRY(hipEventRecord(lc.done, ls));
       HIP_TRY(hipStreamWaitEvent(st, lc.done, 0));
