@@ -45,6 +45,11 @@ def parse_args():
     ap.add_argument("--cpu-positions", type=int, default=1000)
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--F", type=int, default=1000)
+    ap.add_argument("--workload", choices=("c1", "batched"), default="c1",
+                    help="c1: single mPIE ASP reconstruction (default); batched: 64 independent reconstructions")
+    ap.add_argument("--engine", choices=("mPIE", "ePIE"), default="mPIE", help="engine of the batched workload")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--no-graph", action="store_true", help="batched: direct launches instead of graph replays")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy before the timed region (profiler counter scale)")
     return ap.parse_args()
@@ -95,6 +100,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent reconstructions "
               "shard across GPUs without communication, which this benchmark does not measure", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        return main_batched(args)
     import torch
 
     import ptylab_amd as P
@@ -191,6 +198,156 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_positions)
+    print(json.dumps(out))
+
+
+def cpu_baseline_batched(workloads, engine, budget_s=15.0):
+    """The NumPy oracle on one host core over the batch's instances in order,
+    one iteration each, until about budget_s seconds of work."""
+    import oracle as O
+
+    done, t_all = 0, 0.0
+    names = []
+    for w in workloads:
+        geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+        np.random.seed(1)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        run = O.OracleRun(geo, obj, pr, engine=engine, propagator="Fraunhofer", numIterations=1)
+        t0 = time.perf_counter()
+        run.reconstruct()
+        t_all += time.perf_counter() - t0
+        done += w.ptychogram.shape[0]
+        names.append(f"Np={w.Np}/F={w.ptychogram.shape[0]}")
+        if t_all >= budget_s:
+            break
+    return {
+        "value": done / t_all,
+        "unit": "position updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy oracle, {engine} Fraunhofer, one iteration of the first {len(names)} batch instances "
+                  f"({', '.join(names)}; {done} positions, {t_all:.1f} s)",
+    }
+
+
+def main_batched(args):
+    """Configuration "batched": independent mPIE (or ePIE) reconstructions,
+    Np uniform in {64, 128, 256}, No = 4 Np, F uniform in [100, 800], inputs
+    as in the plain-ePIE family, one launch per iteration replayed as a HIP
+    graph, cluster sizes from the autotuner.  A step is one iteration of
+    every instance; value = K * sum(F) / elapsed."""
+    import torch
+
+    import ptylab_amd as P
+    from ptylab_amd import autotune
+    from ptylab_amd import synthetic as S
+
+    # inputs first: the generator forks worker processes, which must happen
+    # before this process initialises the GPU
+    t_gen = time.perf_counter()
+    ws = S.batched_instances(n=args.instances, workers=min(16, os.cpu_count() or 1))
+    print(f"[bench] {len(ws)} workloads generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    if not torch.cuda.is_available():
+        print("no HIP GPU available", file=sys.stderr)
+        sys.exit(1)
+    K, W = args.steps, args.warmup
+    engines = []
+    for i, w in enumerate(ws):
+        data = P.ExperimentalData()
+        for k, v in w.as_dict().items():
+            setattr(data, k, v)
+        data._setData()
+        params = P.Params()
+        params.propagatorType = "Fraunhofer"
+        params.positionOrder = "random"
+        rec = P.Reconstruction(data, params)
+        rec.No = w.No
+        np.random.seed(1000 + i)
+        rec.initializeObjectProbe()
+        eng = getattr(P.Engines, args.engine)(rec, data, params, P.DummyMonitor())
+        eng.numIterations = W + K
+        engines.append(eng)
+    np.random.seed(3)
+    for e in engines:
+        e._prepareReconstruction()
+    staged = [e._stage() for e in engines]
+    insts = [s_[0] for s_ in staged]
+    for inst in insts:
+        inst.flags = P._native.PTY_FLAG_BATCHED
+    t_tune = time.perf_counter()
+    G = autotune.tune(insts)
+    print(f"[bench] autotuned cluster sizes in {time.perf_counter() - t_tune:.1f} s: total {sum(G)} workgroups",
+          file=sys.stderr, flush=True)
+    for inst, g in zip(insts, G):
+        inst.workgroups = g
+    stream = torch.cuda.current_stream()
+    plan = P._native.Plan(insts, [s_[1] for s_ in staged], stream)
+    plan.use_graph(not args.no_graph)
+    Fs = np.array([e.experimentalData.numFrames for e in engines])
+    for it in range(W):
+        plan.run(it, it + 1)
+    plan.check()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    t0 = time.perf_counter()
+    for i in range(K):
+        ev[i][0].record(stream)
+        plan.run(W + i, W + i + 1)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    plan.check()
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    n_wg = plan.workgroups()
+    momentum = args.engine == "mPIE"
+    bytes_iter = 0.0
+    for e, w in zip(engines, ws):
+        frac = float(e._masks[W:W + K].sum()) / (K * e.experimentalData.numFrames) if momentum else 0.0
+        bytes_iter += e.experimentalData.numFrames * bytes_per_position(w.Np, w.No, 1, 1, 1, 1, frac)
+    for e in engines:
+        e._collect(W + K)
+    plan.close()
+    mean_launch_s = float(np.mean(launch_ms)) / 1e3
+    achieved = bytes_iter / mean_launch_s / 1e9
+    nps = {int(n): int(sum(1 for w in ws if w.Np == n)) for n in sorted(set(w.Np for w in ws))}
+    out = {
+        "metric": METRIC,
+        "value": K * float(Fs.sum()) / elapsed,
+        "unit": "position updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(ws)} independent {args.engine} reconstructions, Fraunhofer, Np in {nps}, No=4Np, "
+                        f"F uniform in [100, 800] (sum {int(Fs.sum())}), one launch per iteration"
+                        + (" (graph replay)" if not args.no_graph else " (direct launch)"),
+            "positions_per_step": int(Fs.sum()),
+            "workgroups": n_wg,
+            "cluster_sizes": {str(n): sorted(set(int(g) for g, w in zip(G, ws) if w.Np == n)) for n in nps},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "pty_batch_kernel",
+            "bytes_per_step": bytes_iter,
+            "mean_launch_ms": mean_launch_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU oracle baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_batched(ws, args.engine)
     print(json.dumps(out))
 
 

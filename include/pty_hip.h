@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 1
+#define PTY_ABI_VERSION 2
 
 /* error codes */
 #define PTY_OK 0
@@ -90,8 +90,13 @@ typedef struct pty_instance {
   double* err_at_pos;
   double* err_iter;
   int32_t workgroups;  /* workgroups cooperating on this instance; 0 = auto */
-  int32_t reserved;
+  int32_t flags;       /* PTY_FLAG_* bits, 0 by default */
 } pty_instance;
+
+/* Run the instance in the batched launch (the one kernel that serves every
+ * Np <= 256 of a pass family) even when it is alone in its plan.  In a plan
+ * of several instances every instance with Np <= 256 runs there anyway. */
+#define PTY_FLAG_BATCHED 1
 
 typedef struct pty_plan pty_plan;
 
@@ -103,7 +108,13 @@ const char* pty_last_error(void);
 
 /*
  * Build a plan for `ninst` independent reconstructions run by one persistent
- * launch per iteration.  Replaces the set-up half of engine.reconstruct():
+ * launch per iteration.  Several instances of uneven Np and position counts
+ * share one launch per pass family (the batched kernel); an instance's
+ * `workgroups` sets its cluster size (choose it with the autotuner, see
+ * pty_coresident_workgroups).  There is no batching API in the reference:
+ * a batch is what a script looping over independent engine.reconstruct()
+ * calls (ePIE.py:53-112, mPIE.py:111-180) would do one after another.
+ * Replaces the set-up half of engine.reconstruct():
  * BaseEngine._prepareReconstruction / _checkGPU (BaseEngine.py:96-111,
  * 613-635) and the engines' initializeReconstructionParams.  Validates every
  * shape, allocates the per-instance workspace, permutes each ptychogram and
@@ -133,6 +144,24 @@ int pty_plan_run_positions(pty_plan* plan, int32_t it, int32_t j_begin, int32_t 
  * Returns PTY_ETIMEDOUT if a cluster barrier gave up.
  */
 int pty_plan_check(pty_plan* plan, void* hip_stream);
+
+/*
+ * Run every later pty_plan_run / pty_plan_run_positions of this plan as one
+ * replay per iteration of an instantiated HIP graph (control-block reset
+ * plus the class kernels, with the iteration arguments rewritten per
+ * replay) instead of direct launches.  On by default for plans with a
+ * batched launch.  No reference counterpart (a launch-mode switch).
+ */
+int pty_plan_use_graph(pty_plan* plan, int32_t enable);
+
+/*
+ * Workgroups of the kernel that would run an instance of this probe size,
+ * engine and propagator (batched != 0: in the batched launch) that the
+ * current device can hold at once: the upper bound for the sum of the
+ * instances' `workgroups` in one launch.  Host query for the autotuner; no
+ * reference counterpart.
+ */
+int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count);
 
 /* Number of workgroups the plan launches per iteration (host query). */
 int pty_plan_workgroups(const pty_plan* plan, int32_t* total);

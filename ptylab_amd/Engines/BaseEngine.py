@@ -54,6 +54,8 @@ class BaseEngine:
         # workgroups cooperating on one reconstruction (0: chosen by the
         # library from Np); an MI355X-specific knob with no reference twin
         self.workgroups = 0
+        self._T = None  # device tensors while a plan runs this engine
+        self._plan = None
         self.logger = logging.getLogger(self.ENGINE or "BaseEngine")
 
     # -- preparation -------------------------------------------------------
@@ -215,8 +217,9 @@ class BaseEngine:
     def _momentum_enabled(self):
         return False
 
-    def _begin(self):
-        """Upload, schedule, plan.  Returns (plan, tensors)."""
+    def _stage(self):
+        """Upload the state and describe the instance; no plan yet.
+        Returns (pty_instance, tensors)."""
         import torch
 
         if not torch.cuda.is_available():
@@ -225,21 +228,28 @@ class BaseEngine:
         momentum = self._momentum_enabled()
         T = self._upload(n_iter, momentum)
         inst = self._instance(T, n_iter, momentum)
-        stream = torch.cuda.current_stream()
-        plan = _native.Plan([inst], T, stream)
-        self._T, self._plan = T, plan
+        self._T = T
         r = self.reconstruction
         # expose the live device state like the reference's GPU path does
         r.object, r.probe = T["object"], T["probe"]
         if momentum:
             r.objectBuffer, r.probeBuffer = T["objectBuffer"], T["probeBuffer"]
             r.objectMomentum, r.probeMomentum = T["objectMomentum"], T["probeMomentum"]
+        return inst, T
+
+    def _begin(self):
+        """Upload, schedule, plan.  Returns (plan, tensors)."""
+        import torch
+
+        inst, T = self._stage()
+        plan = _native.Plan([inst], T, torch.cuda.current_stream())
+        self._plan = plan
         return plan, T
 
-    def _end(self, n_done):
-        """Check the device flag, then errors and state back to NumPy."""
-        plan, T = self._plan, self._T
-        plan.check()
+    def _collect(self, n_done):
+        """Errors and state back to NumPy (ePIE.py:123-126) after the plan
+        that ran this engine has been checked."""
+        T = self._T
         r = self.reconstruction
         errs = T["err_iter"][:n_done].cpu().numpy()
         r.error = np.append(np.asarray(r.error, dtype=np.float64), errs)
@@ -249,9 +259,15 @@ class BaseEngine:
         for k in ("object", "probe", "objectBuffer", "probeBuffer", "objectMomentum", "probeMomentum"):
             if k in T:
                 setattr(r, k, T[k].cpu().numpy())
+        self._T = None
+
+    def _end(self, n_done):
+        """Check the device flag, then errors and state back to NumPy."""
+        plan = self._plan
+        plan.check()
+        self._collect(n_done)
         plan.close()
         self._plan = None
-        self._T = None
 
     # -- per-iteration constraints (BaseEngine.applyConstraints :1440-1612) -----
     def applyConstraints(self, loop):

@@ -13,7 +13,8 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libptyhip.so"
 
-PTY_ABI_VERSION = 1
+PTY_ABI_VERSION = 2
+PTY_FLAG_BATCHED = 1
 PTY_EPIE, PTY_MPIE, PTY_MULTIPIE, PTY_E3PIE = 0, 1, 2, 3
 PTY_FRAUNHOFER, PTY_FRAUNHOFER_SHIFTED, PTY_ASP = 0, 1, 2
 PTY_ETIMEDOUT = -110
@@ -28,6 +29,8 @@ EXPORTED = (
     "pty_plan_run_positions",
     "pty_plan_check",
     "pty_plan_workgroups",
+    "pty_plan_use_graph",
+    "pty_coresident_workgroups",
     "pty_plan_destroy",
     "pty_propagate",
     "pty_fft2",
@@ -71,7 +74,7 @@ class PtyInstance(ctypes.Structure):
         ("err_at_pos", ctypes.c_void_p),
         ("err_iter", ctypes.c_void_p),
         ("workgroups", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
 
 
@@ -112,6 +115,8 @@ def load(path: os.PathLike | None = None):
     lib.pty_plan_run_positions.argtypes = [vp, i32, i32, i32, vp]
     lib.pty_plan_check.argtypes = [vp, vp]
     lib.pty_plan_workgroups.argtypes = [vp, ctypes.POINTER(i32)]
+    lib.pty_plan_use_graph.argtypes = [vp, i32]
+    lib.pty_coresident_workgroups.argtypes = [i32, i32, i32, i32, ctypes.POINTER(i32)]
     lib.pty_plan_destroy.argtypes = [vp]
     lib.pty_propagate.argtypes = [vp, i32, i32, i32, i32, vp, vp]
     lib.pty_fft2.argtypes = [vp, i32, i32, i32, vp]
@@ -169,6 +174,14 @@ def fft2(fields, sign: int, stream=None):
     return fields
 
 
+def coresident_workgroups(Np, engine, propagator, batched):
+    """Workgroups of the kernel serving such an instance that fit on the
+    device at once (pty_coresident_workgroups)."""
+    n = ctypes.c_int32()
+    check(load().pty_coresident_workgroups(int(Np), int(engine), int(propagator), int(bool(batched)), ctypes.byref(n)))
+    return n.value
+
+
 class Plan:
     """RAII wrapper of a pty_plan; keeps the tensors it points to alive."""
 
@@ -195,6 +208,10 @@ class Plan:
         n = ctypes.c_int32()
         check(self._lib.pty_plan_workgroups(self._h, ctypes.byref(n)))
         return n.value
+
+    def use_graph(self, enable=True):
+        """Replay a per-iteration HIP graph instead of direct launches."""
+        check(self._lib.pty_plan_use_graph(self._h, int(bool(enable))))
 
     def close(self):
         if getattr(self, "_h", None):

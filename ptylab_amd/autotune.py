@@ -1,0 +1,239 @@
+"""Cluster-size autotuner for batched plans.
+
+A batched plan runs many independent reconstructions in one persistent
+launch per iteration; instance i runs on a cluster of G_i workgroups that
+splits every line and column pass of a position G_i ways.  Larger clusters
+cut an instance's time per position but cost more workgroups and more
+barrier traffic, and the whole launch must stay co-resident
+(sum G_i <= pty_coresident_workgroups).  An iteration of the batch lasts as
+long as its slowest instance, so the tuner
+
+1. measures the time per position t(G) of the batched kernel for every
+   instance shape in the batch (pass family, engine, Np, No, field count) and
+   every candidate cluster size G (powers of two), on synthetic data of that
+   shape, with HIP events; results are cached in a JSON file;
+2. assigns cluster sizes by repeatedly doubling the cluster of the instance
+   with the longest predicted run time (n_iter x F x t(G)) while the
+   capacity allows and the doubling shortens that instance.
+
+The reference has no batching and no tuner; this module chooses launch
+parameters only and never changes results (every cluster size gives
+bit-identical reconstructions).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
+BATCH_MAX_NP = 256  # largest probe size of the batched kernel
+_DEFAULT_CACHE = Path(os.environ.get("PTYLAB_AMD_TUNE_CACHE", Path.home() / ".cache" / "ptylab_amd" / "tune.json"))
+
+
+@dataclass(frozen=True)
+class Shape:
+    """What the per-position cost of an instance depends on."""
+
+    engine: int
+    propagator: int
+    Np: int
+    No: int
+    nF: int  # exit-wave fields (nlambda * nosm * npsm * nslice)
+
+    def key(self, G):
+        return f"e{self.engine}p{self.propagator}/Np{self.Np}/No{self.No}/nF{self.nF}/G{G}"
+
+
+@dataclass
+class Job:
+    shape: Shape
+    F: int
+    n_iter: int
+
+
+def family(engine, propagator):
+    """Pass family of the batched kernel an instance runs in."""
+    if engine == _native.PTY_E3PIE:
+        return 2
+    return 1 if propagator == _native.PTY_ASP else 0
+
+
+def candidate_sizes(shape: Shape, capacity: int):
+    """Powers of two that divide Np, at most the capacity."""
+    return [g for g in CANDIDATES if g <= capacity and shape.Np % g == 0]
+
+
+def assign(jobs, cost, capacity):
+    """Cluster sizes minimising the batch makespan.
+
+    ``cost(shape, G)`` is the time per position (any unit); ``capacity`` the
+    workgroups that fit at once.  Returns a list of cluster sizes, one per
+    job, with sum <= capacity (ValueError if even all-ones does not fit).
+    """
+    n = len(jobs)
+    if n > capacity:
+        raise ValueError(f"{n} instances need at least {n} workgroups; {capacity} fit on the device")
+    G = [1] * n
+    T = [j.n_iter * j.F * cost(j.shape, 1) for j in jobs]
+    used = n
+    blocked = [False] * n
+    while True:
+        order = sorted((i for i in range(n) if not blocked[i]), key=lambda i: -T[i])
+        if not order:
+            break
+        i = order[0]
+        if T[i] < max(T):
+            break  # the longest instance cannot be shortened further
+        g2 = 2 * G[i]
+        if g2 not in candidate_sizes(jobs[i].shape, capacity) or used + G[i] > capacity:
+            blocked[i] = True
+            continue
+        t2 = jobs[i].n_iter * jobs[i].F * cost(jobs[i].shape, g2)
+        if t2 >= T[i]:
+            blocked[i] = True
+            continue
+        used += G[i]
+        G[i], T[i] = g2, t2
+    return G
+
+
+def predicted_makespan(jobs, cost, G):
+    return max(j.n_iter * j.F * cost(j.shape, g) for j, g in zip(jobs, G))
+
+
+class CostTable:
+    """Measured microseconds per position, keyed by Shape.key(G), JSON-cached."""
+
+    def __init__(self, path: os.PathLike | None = None, positions: int = 48):
+        self.path = Path(path) if path is not None else _DEFAULT_CACHE
+        self.positions = positions
+        self.table = {}
+        if self.path.exists():
+            try:
+                self.table = json.loads(self.path.read_text())
+            except (OSError, ValueError):
+                self.table = {}
+
+    def save(self):
+        try:
+            self.path.parent.mkdir(parents=True, exist_ok=True)
+            self.path.write_text(json.dumps(self.table, indent=1, sort_keys=True))
+        except OSError:
+            pass
+
+    def __call__(self, shape: Shape, G: int) -> float:
+        k = shape.key(G)
+        if k not in self.table:
+            self.table[k] = measure(shape, G, self.positions)
+            self.save()
+        return self.table[k]
+
+
+def measure(shape: Shape, G: int, positions: int = 48, seed: int = 0) -> float:
+    """Microseconds per position of the batched kernel for one instance of
+    this shape on a cluster of G workgroups (synthetic data, momentum
+    triggers at the mPIE rate for the momentum engines)."""
+    import torch
+
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(seed)
+    Np, No, nF = shape.Np, shape.No, shape.nF
+    e3 = shape.engine == _native.PTY_E3PIE
+    nslice = 2 if e3 else 1
+    npsm = nF
+    F = positions
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+
+    def crand(*s):
+        return torch.complex(torch.rand(*s, generator=gen) + 0.5, torch.rand(*s, generator=gen) - 0.5).to(dev)
+
+    T = {}
+    T["object"] = crand(1, 1, 1, nslice, No, No)
+    T["probe"] = crand(1, 1, npsm, nslice, Np, Np)
+    T["ptychogram"] = torch.rand(F, Np, Np, generator=gen).to(dev)
+    pos = rng.integers(0, No - Np + 1, size=(F, 2)).astype(np.int32)
+    T["positions"] = torch.as_tensor(pos, device=dev)
+    n_iter = 2
+    T["order"] = torch.as_tensor(np.tile(np.arange(F, dtype=np.int32), (n_iter, 1)), device=dev)
+    momentum = shape.engine in (_native.PTY_MPIE, _native.PTY_MULTIPIE)
+    T["mask"] = torch.as_tensor((rng.random((n_iter, F)) > 0.95).astype(np.uint8), device=dev)
+    T["energy"] = torch.full((F,), float(Np * Np), device=dev)
+    T["err_at_pos"] = torch.zeros((n_iter, F), dtype=torch.float64, device=dev)
+    T["err_iter"] = torch.zeros((n_iter,), dtype=torch.float64, device=dev)
+    if shape.propagator == _native.PTY_ASP or e3:
+        ph = torch.rand(Np, Np, generator=gen) * 6.283
+        T["tf"] = torch.polar(torch.ones(Np, Np), ph).to(torch.complex64).to(dev)
+    if momentum:
+        for k in ("object", "probe"):
+            T[k + "Buffer"] = T[k].clone()
+            T[k + "Momentum"] = torch.zeros_like(T[k])
+    inst = _native.PtyInstance()
+    inst.engine, inst.propagator = shape.engine, shape.propagator
+    inst.nlambda, inst.nosm, inst.npsm, inst.nslice = 1, 1, npsm, nslice
+    inst.Np, inst.No, inst.F, inst.n_iter = Np, No, F, n_iter
+    inst.betaObject = inst.betaProbe = 0.25
+    inst.alphaObject = inst.alphaProbe = 0.1
+    inst.momFriction, inst.momFeedback, inst.probeWeight, inst.sliceBeta = 0.7, 0.3, 1.0, 0.9
+    p = lambda k: T[k].data_ptr() if k in T else None  # noqa: E731
+    inst.object, inst.probe = p("object"), p("probe")
+    inst.objectBuffer, inst.objectMomentum = p("objectBuffer"), p("objectMomentum")
+    inst.probeBuffer, inst.probeMomentum = p("probeBuffer"), p("probeMomentum")
+    inst.ptychogram, inst.positions, inst.order = p("ptychogram"), p("positions"), p("order")
+    inst.momentum_mask = p("mask") if momentum else None
+    inst.energy, inst.tf = p("energy"), p("tf")
+    inst.err_at_pos, inst.err_iter = p("err_at_pos"), p("err_iter")
+    inst.workgroups = G
+    inst.flags = _native.PTY_FLAG_BATCHED
+    stream = torch.cuda.current_stream()
+    plan = _native.Plan([inst], T, stream)
+    try:
+        plan.use_graph(False)
+        plan.run(0, 1)
+        plan.check()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plan.run(1, 2)
+        e1.record(stream)
+        plan.check()
+        return e0.elapsed_time(e1) * 1e3 / F
+    finally:
+        plan.close()
+
+
+def shape_of(inst) -> Shape:
+    e3 = inst.engine == _native.PTY_E3PIE
+    nF = inst.nlambda * inst.npsm if e3 else inst.nlambda * inst.nosm * inst.npsm * inst.nslice
+    return Shape(int(inst.engine), int(inst.propagator), int(inst.Np), int(inst.No), int(nF))
+
+
+def launch_key(inst):
+    """Instances with the same key share one launch: the batched kernel of
+    a pass family for Np <= 256, a per-size kernel otherwise."""
+    fam = family(inst.engine, inst.propagator)
+    return (fam, 0) if inst.Np <= BATCH_MAX_NP else (fam, int(inst.Np))
+
+
+def tune(instances, table: CostTable | None = None):
+    """Cluster sizes for a list of pty_instance (one batched plan).  The
+    launches of a plan run one after another, each alone on the device, so
+    every launch is assigned against its own co-resident capacity."""
+    table = table if table is not None else CostTable()
+    out = [1] * len(instances)
+    launches = {}
+    for i, inst in enumerate(instances):
+        launches.setdefault(launch_key(inst), []).append(i)
+    for (fam, np_), idx in launches.items():
+        a = instances[idx[0]]
+        cap = _native.coresident_workgroups(np_ or BATCH_MAX_NP, a.engine, a.propagator, np_ == 0)
+        jobs = [Job(shape_of(instances[i]), int(instances[i].F), int(instances[i].n_iter)) for i in idx]
+        G = assign(jobs, table, cap)
+        for i, g in zip(idx, G):
+            out[i] = g
+    return out

@@ -119,15 +119,16 @@ __host__ __device__ static int log2_np(int np) {
   return -1;
 }
 
-// item slots per round of the persistent kernel for probe size np
-// (Cfg<NP>::IPR): waves per workgroup x lines per wave
-static int slots_rt(int np) { return (np >= 128 ? 8 : 16) * (np >= 64 ? 1 : 64 / np); }
+// threads per workgroup of the per-size kernel (Cfg<NP>::NT) and item slots
+// per round for a workgroup of nt threads: waves x lines per wave
 static int nthreads_rt(int np) { return np >= 128 ? 512 : 1024; }
+static int slots_rt(int np, int nt) { return (nt / 64) * (np >= 64 ? 1 : 64 / np); }
 
 struct Run {
   const DevInst* D;
   int g, G, wave, t, sub, lane;
   int nF, nFp, nwave;
+  int nt, ipr;  // threads per workgroup, (line, field) item slots per round
   unsigned gen;
   const float2* tw;
   double* red;
@@ -291,16 +292,28 @@ __device__ __forceinline__ float2 ldg2(const float2* p) {
 // the maxima) go through LDS and are added in increasing field order, the C
 // order of NumPy's axis sums, so results are deterministic.
 template <int NP>
+struct Col {
+#ifndef PTY_COLUMN_WIDTH
+  static constexpr int CW0 = NP >= 256 ? 2 : 1;
+#else
+  static constexpr int CW0 = PTY_COLUMN_WIDTH;
+#endif
+  static constexpr int CW = CW0 * NP >= 64 ? CW0 : 64 / NP;  // a column spans at most NP lanes
+  static constexpr int LPC = 64 / CW;                         // lanes per column
+  static constexpr int CL = NP / LPC;                         // values per lane
+  static constexpr int NG = NP / CW;                          // column groups
+};
+
+template <int NP>
 struct Cfg {
   static constexpr int NT = NP >= 128 ? 512 : 1024;  // threads per workgroup
   static constexpr int NW = NT / 64;
   static constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
   static constexpr int IPR = NW * SUB;  // item slots per round
   // LDS exchange: row passes use two (IPR, NP) complex buffers, column
-  // passes one (NW, values per lane, 64) float buffer (at most NW * NP * 8
-  // floats for 8-column groups)
+  // passes one (NW, values per lane, 64) float buffer
   static constexpr int XROW = 2 * IPR * NP;
-  static constexpr int XCOL = NW * NP * 4;
+  static constexpr int XCOL = NW * NP * Col<NP>::CW / 2;
   static constexpr int XBUF = XROW > XCOL ? XROW : XCOL;  // float2 elements
 };
 
@@ -308,7 +321,7 @@ struct Cfg {
   {                                                                            \
     const int per_ = NP / (R).G;                                               \
     const int line0_ = (R).g * per_;                                           \
-    const int lpr_ = Cfg<NP>::IPR / (R).nFp;                                   \
+    const int lpr_ = (R).ipr / (R).nFp;                                        \
     const int islot = (R).wave * Line<NP>::SUB + (R).sub;                      \
     const int lil_ = islot / (R).nFp;                                          \
     const int frw = islot - lil_ * (R).nFp;                                    \
@@ -420,26 +433,13 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
 // Workgroup g owns column groups [g * cpg, (g + 1) * cpg) of the Np / CW
 // groups (workgroups past the last group idle in column passes); its waves
 // take (group, field) items, groups per round = waves / nFp.
-template <int NP>
-struct Col {
-#ifndef PTY_COLUMN_WIDTH
-  static constexpr int CW0 = NP >= 256 ? 2 : 1;
-#else
-  static constexpr int CW0 = PTY_COLUMN_WIDTH;
-#endif
-  static constexpr int CW = CW0 * NP >= 64 ? CW0 : 64 / NP;  // a column spans at most NP lanes
-  static constexpr int LPC = 64 / CW;                         // lanes per column
-  static constexpr int CL = NP / LPC;                         // values per lane
-  static constexpr int NG = NP / CW;                          // column groups
-};
-
 #define CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)                         \
   {                                                                             \
     const int ng_ = Col<NP>::NG;                                                \
     const int cpg_ = (ng_ + (R).G - 1) / (R).G;                                 \
     const int cg0_ = (R).g * cpg_;                                              \
     const int cgn_ = cg0_ < ng_ ? (ng_ - cg0_ < cpg_ ? ng_ - cg0_ : cpg_) : 0; \
-    const int gpr_ = Cfg<NP>::NW / (R).nFp;                                     \
+    const int gpr_ = (R).nwave / (R).nFp;                                       \
     const int islot = (R).wave;                                                 \
     const int lg_ = islot / (R).nFp;                                            \
     const int frw = islot - lg_ * (R).nFp;                                      \
@@ -702,7 +702,7 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
     const size_t chunk = (n + R.G - 1) / R.G;
     const size_t b = chunk * R.g;
     const size_t e = b + chunk < n ? b + chunk : n;
-    for (size_t i = b + threadIdx.x; i < e; i += Cfg<NP>::NT) {
+    for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
       const float2 x = cload(X + i), bb = cload(B + i), m = cload(M + i);
       const float2 mn = make_float2((bb.x - x.x) + m.x * eta, (bb.y - x.y) + m.y * eta);
       const float2 xn = make_float2(x.x - mn.x * nu, x.y - mn.y * nu);
@@ -973,6 +973,8 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
   const int F = D.a.F;
   const int engine = D.a.engine;
   const bool momentum = (engine == PTY_MPIE || engine == PTY_MULTIPIE) && D.a.momentum_mask != nullptr;
+  // instances of a batch may have fewer iterations than the launch covers
+  if (it1 > D.a.n_iter) it1 = D.a.n_iter;
   unsigned q = 0;
   for (int it = it0; it < it1; ++it) {
     const int* order = D.a.order + (size_t)it * F;
@@ -998,7 +1000,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       // getErrorMetrics (BaseEngine.py:819-841): errorAtPos = sum|I - Iest| /
       // (energyAtPos + 1e-20) in FP64, error = sum over positions
       double acc = 0.0;
-      for (int jj = threadIdx.x; jj < F; jj += Cfg<NP>::NT) {
+      for (int jj = threadIdx.x; jj < F; jj += R.nt) {
         const int kk = order[jj];
         double s = 0.0;
         for (int g = 0; g < R.G; ++g) s += cloadd(ep + (size_t)jj * R.G + g);
@@ -1013,25 +1015,13 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
   }
 }
 
-// One kernel per probe size and pass family (PIE with the Fraunhofer
-// detector, PIE with ASP, e3PIE): each instantiation carries only its own
-// passes and gets its own register allocation and workgroup size.  A plan
-// with several classes launches one kernel per class, concurrently on its
-// own streams.
-template <int NP, int FAM>
-__global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst* __restrict__ insts,
-                                                                  const int2* __restrict__ bmap, int it0, int it1,
-                                                                  int jb, int je) {
-  __shared__ float2 s_tw[256];
-  __shared__ double s_red[Cfg<NP>::NW];
-  __shared__ float2 s_x[Cfg<NP>::XBUF];
-  __shared__ int s_fail;
-  const int2 bm = bmap[blockIdx.x];
-  if (bm.x < 0) return;
-  const DevInst* D = insts + bm.x;
-  Run R;
+// Per-workgroup set-up for probe size NP on a workgroup of NT threads:
+// lane roles, LDS views, the twiddles of length NP.
+template <int NP, int NT>
+__device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2* s_tw, double* s_red, float2* s_x,
+                                         int* s_fail) {
   R.D = D;
-  R.g = bm.y;
+  R.g = g;
   R.G = D->G;
   R.nF = D->nF;
   R.nFp = D->nFp;
@@ -1042,13 +1032,89 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   R.gen = 0u;
   R.tw = s_tw;
   R.red = s_red;
-  R.nwave = Cfg<NP>::NW;
+  R.nt = NT;
+  R.nwave = NT / 64;
+  R.ipr = (NT / 64) * Line<NP>::SUB;
   R.xa = s_x;
-  R.xb = s_x + Cfg<NP>::IPR * NP;
-  R.fail = &s_fail;
-  for (int i = threadIdx.x; i < NP / 2; i += Cfg<NP>::NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
+  R.xb = s_x + R.ipr * NP;
+  R.fail = s_fail;
+  for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
+}
+
+// One kernel per probe size and pass family (PIE with the Fraunhofer
+// detector, PIE with ASP, e3PIE): each instantiation carries only its own
+// passes and gets its own register allocation and workgroup size.
+template <int NP, int FAM>
+__global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst* __restrict__ insts,
+                                                                  const int2* __restrict__ bmap, int it0, int it1,
+                                                                  int jb, int je) {
+  __shared__ float2 s_tw[256];
+  __shared__ double s_red[Cfg<NP>::NW];
+  __shared__ float2 s_x[Cfg<NP>::XBUF];
+  __shared__ int s_fail;
+  const int2 bm = bmap[blockIdx.x];
+  if (bm.x < 0) return;
+  Run R;
+  init_run<NP, Cfg<NP>::NT>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, &s_fail);
   run_iterations<NP, FAM>(R, it0, it1, jb, je);
+}
+
+// Batched launch: every instance of one pass family with Np <= 256 runs in
+// one persistent launch of BATCH_NT-thread workgroups, whatever its probe
+// size; each workgroup branches once on its instance's Np.  One launch of
+// uniform workgroups is what makes the co-residency of every cluster a
+// property of the launch size alone (nblocks <= co-resident capacity).
+#define BATCH_NT 512
+#define BATCH_MAX_NP 256
+
+template <int NP>
+struct BatchCfg {
+  static constexpr int NW = BATCH_NT / 64;
+  static constexpr int IPR = NW * Line<NP>::SUB;
+  static constexpr int XROW = 2 * IPR * NP;
+  static constexpr int XCOL = NW * NP * Col<NP>::CW / 2;
+  static constexpr int XBUF = XROW > XCOL ? XROW : XCOL;
+};
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int BATCH_XBUF =
+    cmax(cmax(cmax(BatchCfg<16>::XBUF, BatchCfg<32>::XBUF), cmax(BatchCfg<64>::XBUF, BatchCfg<128>::XBUF)),
+         BatchCfg<256>::XBUF);
+
+template <int FAM>
+__global__ __launch_bounds__(BATCH_NT) void pty_batch_kernel(const DevInst* __restrict__ insts,
+                                                             const int2* __restrict__ bmap, int it0, int it1, int jb,
+                                                             int je) {
+  __shared__ float2 s_tw[256];
+  __shared__ double s_red[BATCH_NT / 64];
+  __shared__ float2 s_x[BATCH_XBUF];
+  __shared__ int s_fail;
+  const int2 bm = bmap[blockIdx.x];
+  if (bm.x < 0) return;
+  const DevInst* D = insts + bm.x;
+  Run R;
+  switch (D->a.Np) {
+    case 16:
+      init_run<16, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      run_iterations<16, FAM>(R, it0, it1, jb, je);
+      break;
+    case 32:
+      init_run<32, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      run_iterations<32, FAM>(R, it0, it1, jb, je);
+      break;
+    case 64:
+      init_run<64, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      run_iterations<64, FAM>(R, it0, it1, jb, je);
+      break;
+    case 128:
+      init_run<128, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      run_iterations<128, FAM>(R, it0, it1, jb, je);
+      break;
+    case 256:
+      init_run<256, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      run_iterations<256, FAM>(R, it0, it1, jb, je);
+      break;
+  }
 }
 
 __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
@@ -1202,27 +1268,35 @@ __global__ __launch_bounds__(NTHREADS) void pty_rows_inv_kernel(float2* __restri
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// Instances with the same probe size form a launch class: one kernel
-// instantiation, one block map, one launch per run.
+// Instances with the same probe size and pass family form a launch class:
+// one kernel instantiation, one block map.  In a plan of several instances
+// (or for an instance flagged PTY_FLAG_BATCHED) every instance with
+// Np <= BATCH_MAX_NP joins the batched class of its family (np = 0), whose
+// kernel serves every probe size.  Classes run one after another on the
+// caller's stream, so each launch alone must fit the device.
 struct LaunchClass {
-  int np = 0;
+  int np = 0;  // probe size; 0 for the batched class
   int fam = 0;
   int nblocks = 0;
-  bool cooperative = false;
+  int nthreads = 0;
+  bool cooperative = false;  // some instance of the class has G > 1
   int2* d_bmap = nullptr;
-  hipStream_t stream = nullptr;  // own stream when the plan has several classes
-  hipEvent_t done = nullptr;
+  hipGraphNode_t node = nullptr;
 };
 
 struct pty_plan {
   int device = 0;
   int ninst = 0;
   int nblocks = 0;
+  int max_iter = 0;
   DevInst* d_insts = nullptr;
   std::vector<DevInst> h_insts;
   std::vector<LaunchClass> classes;
-  hipEvent_t fork = nullptr;
   std::vector<void*> allocs;
+  // per-iteration graph: reset node, then one kernel node per class
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 template <int NP>
@@ -1237,6 +1311,13 @@ static const void* iterate_kernel_fam(int fam) {
 
 static const void* iterate_kernel_ptr(int np, int fam) {
   switch (np) {
+    case 0:
+      switch (fam) {
+        case FAM_PIE_FRAUNHOFER: return (const void*)pty_batch_kernel<FAM_PIE_FRAUNHOFER>;
+        case FAM_PIE_ASP: return (const void*)pty_batch_kernel<FAM_PIE_ASP>;
+        case FAM_E3PIE: return (const void*)pty_batch_kernel<FAM_E3PIE>;
+      }
+      return nullptr;
     case 16: return iterate_kernel_fam<16>(fam);
     case 32: return iterate_kernel_fam<32>(fam);
     case 64: return iterate_kernel_fam<64>(fam);
@@ -1283,30 +1364,34 @@ int dalloc(pty_plan* p, void** out, size_t bytes) {
 }
 
 void free_plan(pty_plan* p) {
-  for (LaunchClass& c : p->classes) {
-    if (c.stream) (void)hipStreamDestroy(c.stream);
-    if (c.done) (void)hipEventDestroy(c.done);
-  }
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
   p->classes.clear();
-  if (p->fork) (void)hipEventDestroy(p->fork);
-  p->fork = nullptr;
   for (void* a : p->allocs) (void)hipFree(a);
   p->allocs.clear();
 }
 
-int auto_workgroups(const pty_instance& a, int ninst) {
+// whether an instance runs in the batched class of its family
+bool joins_batch(const pty_instance& a, int ninst) {
+  return a.Np <= BATCH_MAX_NP && (ninst > 1 || (a.flags & PTY_FLAG_BATCHED));
+}
+
+int auto_workgroups(const pty_instance& a, int ninst, int nt) {
   if (ninst > 1) return 1;
   int nF = a.engine == PTY_E3PIE ? a.nlambda * a.npsm : a.nlambda * a.nosm * a.npsm * a.nslice;
   int nFp = 1;
   while (nFp < nF) nFp *= 2;
-  const int lines_per_round = slots_rt(a.Np) / nFp;
+  const int lines_per_round = slots_rt(a.Np, nt) / nFp;
   int g = a.Np / (lines_per_round > 0 ? lines_per_round : 1);
   if (g < 1) g = 1;
   if (g > 128) g = 128;
   return g;
 }
 
-int validate(const pty_instance& a, int idx) {
+int validate(const pty_instance& a, int idx, int nt) {
+  if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
   if (a.engine < PTY_EPIE || a.engine > PTY_E3PIE) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
   if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_ASP)
     return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
@@ -1321,7 +1406,18 @@ int validate(const pty_instance& a, int idx) {
   if (a.propagator == PTY_ASP && a.nlambda > 1)
     return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, set polychromeASP instead of ASP", idx);
   if (a.propagator == PTY_ASP && !a.tf) return set_err(PTY_EINVAL, "instance %d: ASP needs a transfer function", idx);
-  const int slots = slots_rt(a.Np);
+  const int slots = slots_rt(a.Np, nt);
+  {
+    // a column pass gives every field of a column group its own wave in one
+    // round (the mode sums go through LDS), so the padded field count may
+    // not exceed the waves of a workgroup
+    const int nF = a.engine == PTY_E3PIE ? a.nlambda * a.npsm : a.nlambda * a.nosm * a.npsm * a.nslice;
+    int nFp = 1;
+    while (nFp < nF) nFp *= 2;
+    if (nFp > nt / 64)
+      return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed the %d waves of a workgroup for Np=%d", idx, nF,
+                     nt / 64, a.Np);
+  }
   if (a.No % 2 != 0) return set_err(PTY_EINVAL, "instance %d: No=%d must be even", idx, a.No);
   if (a.engine == PTY_E3PIE) {
     if (a.nosm != 1) return set_err(PTY_EINVAL, "instance %d: e3PIE needs nosm == 1", idx);
@@ -1394,7 +1490,9 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
   std::vector<std::vector<int2>> cmaps;  // per launch class
   for (int i = 0; i < ninst; ++i) {
     const pty_instance& a = insts[i];
-    rc = validate(a, i);
+    const bool batched = joins_batch(a, ninst);
+    const int nt = batched ? BATCH_NT : nthreads_rt(a.Np);
+    rc = validate(a, i, nt);
     if (rc) {
       free_plan(p);
       delete p;
@@ -1403,7 +1501,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     DevInst& d = p->h_insts[i];
     memset(&d, 0, sizeof d);
     d.a = a;
-    int G = a.workgroups > 0 ? a.workgroups : auto_workgroups(a, ninst);
+    int G = a.workgroups > 0 ? a.workgroups : auto_workgroups(a, ninst, nt);
     if (G > a.Np) G = a.Np;
     if (G < 1) G = 1;
     while (a.Np % G) --G;
@@ -1469,14 +1567,18 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       HIP_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
     }
+    if (a.n_iter > p->max_iter) p->max_iter = a.n_iter;
     int ci = -1;
     const int fam = family_of(a);
+    const int key = batched ? 0 : a.Np;
     for (size_t c = 0; c < p->classes.size(); ++c)
-      if (p->classes[c].np == a.Np && p->classes[c].fam == fam) ci = (int)c;
+      if (p->classes[c].np == key && p->classes[c].fam == fam) ci = (int)c;
     if (ci < 0) {
       p->classes.emplace_back();
-      p->classes.back().np = a.Np;
+      p->classes.back().np = key;
       p->classes.back().fam = fam;
+      p->classes.back().nthreads = nt;
+      p->use_graph = p->use_graph || batched;
       cmaps.emplace_back();
       ci = (int)p->classes.size() - 1;
     }
@@ -1504,24 +1606,20 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       lc.d_bmap = (int2*)mem;
       HIP_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
       if (lc.cooperative) {
+        // clusters spin on each other: the whole launch must be co-resident
         int per_cu = 0;
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np, lc.fam), nthreads_rt(lc.np), 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np, lc.fam), lc.nthreads, 0));
         if (!coop) {
           rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
           goto fail;
         }
         if (lc.nblocks > per_cu * cus) {
-          rc = set_err(PTY_EINVAL, "%d workgroups for Np=%d exceed the %d that can be co-resident", lc.nblocks, lc.np,
-                       per_cu * cus);
+          rc = set_err(PTY_EINVAL, "%d workgroups for %s%d exceed the %d that can be co-resident", lc.nblocks,
+                       lc.np ? "Np=" : "the batched launch of family ", lc.np ? lc.np : lc.fam, per_cu * cus);
           goto fail;
         }
       }
-      if (p->classes.size() > 1) {
-        HIP_TRY(hipStreamCreateWithFlags(&lc.stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&lc.done, hipEventDisableTiming));
-      }
     }
-    if (p->classes.size() > 1) HIP_TRY(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
     HIP_TRY(hipStreamSynchronize(st));
   }
   *out = p;
@@ -1545,33 +1643,113 @@ int pty_plan_run_positions(pty_plan* p, int32_t it, int32_t j_begin, int32_t j_e
   return plan_launch(p, it, it + 1, j_begin, j_end, hip_stream);
 }
 
+// Per-iteration graph of a plan: the control-block reset (a fill kernel, no
+// memset node), then the class kernels in sequence.  Kernel nodes are plain
+// launches: the co-residency that a cooperative launch would verify is
+// checked once at plan creation.  Each replay rewrites the iteration
+// arguments of the kernel nodes, then launches the instantiated graph.
+static int graph_build(pty_plan* p) {
+  HIP_TRY(hipGraphCreate(&p->graph, 0));
+  const DevInst* di = p->d_insts;
+  int zero = 0;
+  hipKernelNodeParams kp = {};
+  void* rargs[] = {(void*)&di};
+  kp.func = (void*)pty_reset_ctrl_kernel;
+  kp.gridDim = dim3(p->ninst);
+  kp.blockDim = dim3(64);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = rargs;
+  kp.extra = nullptr;
+  hipGraphNode_t prev = nullptr;
+  HIP_TRY(hipGraphAddKernelNode(&prev, p->graph, nullptr, 0, &kp));
+  for (LaunchClass& lc : p->classes) {
+    const int2* bm = lc.d_bmap;
+    void* args[] = {(void*)&di, (void*)&bm, (void*)&zero, (void*)&zero, (void*)&zero, (void*)&zero};
+    kp.func = (void*)iterate_kernel_ptr(lc.np, lc.fam);
+    kp.gridDim = dim3(lc.nblocks);
+    kp.blockDim = dim3(lc.nthreads);
+    kp.kernelParams = args;
+    HIP_TRY(hipGraphAddKernelNode(&lc.node, p->graph, &prev, 1, &kp));
+    prev = lc.node;
+  }
+  HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  return PTY_OK;
+}
+
+static int graph_replay(pty_plan* p, int it0, int it1, int jb, int je, hipStream_t st) {
+  if (!p->exec) {
+    const int rc = graph_build(p);
+    if (rc) return rc;
+  }
+  const DevInst* di = p->d_insts;
+  for (LaunchClass& lc : p->classes) {
+    const int2* bm = lc.d_bmap;
+    void* args[] = {(void*)&di, (void*)&bm, (void*)&it0, (void*)&it1, (void*)&jb, (void*)&je};
+    hipKernelNodeParams kp = {};
+    kp.func = (void*)iterate_kernel_ptr(lc.np, lc.fam);
+    kp.gridDim = dim3(lc.nblocks);
+    kp.blockDim = dim3(lc.nthreads);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    HIP_TRY(hipGraphExecKernelNodeSetParams(p->exec, lc.node, &kp));
+  }
+  HIP_TRY(hipGraphLaunch(p->exec, st));
+  return PTY_OK;
+}
+
 static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb, int32_t je, void* hip_stream) {
   if (!p) return set_err(PTY_EINVAL, "pty_plan_run: null plan");
   if (it_begin < 0 || it_end < it_begin) return set_err(PTY_EINVAL, "pty_plan_run: bad iteration range");
-  for (const DevInst& d : p->h_insts)
-    if (it_end > d.a.n_iter) return set_err(PTY_EINVAL, "pty_plan_run: it_end %d > n_iter %d", it_end, d.a.n_iter);
+  if (it_end > p->max_iter) return set_err(PTY_EINVAL, "pty_plan_run: it_end %d > n_iter %d", it_end, p->max_iter);
   if (it_end == it_begin) return PTY_OK;
   hipStream_t st = (hipStream_t)hip_stream;
+  if (p->use_graph) {
+    // one replay per iteration
+    for (int it = it_begin; it < it_end; ++it) {
+      const int rc = graph_replay(p, it, it + 1, it == it_begin ? jb : 0, it == it_end - 1 ? je : 0x7fffffff, st);
+      if (rc) return rc;
+    }
+    return PTY_OK;
+  }
   pty_reset_ctrl_kernel<<<p->ninst, 64, 0, st>>>(p->d_insts);
   HIP_TRY(hipGetLastError());
   const DevInst* di = p->d_insts;
   int a0 = it_begin, a1 = it_end, b0 = jb, b1 = je;
-  const bool fork = p->classes.size() > 1;
-  if (fork) HIP_TRY(hipEventRecord(p->fork, st));
   for (LaunchClass& lc : p->classes) {
-    hipStream_t ls = fork ? lc.stream : st;
-    if (fork) HIP_TRY(hipStreamWaitEvent(ls, p->fork, 0));
     const int2* bm = lc.d_bmap;
     void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1};
     if (lc.cooperative)
-      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
+      HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(lc.nthreads), args, 0, st));
     else
-      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(nthreads_rt(lc.np)), args, 0, ls));
-    if (fork) {
-      HIP_TRY(hipEventRecord(lc.done, ls));
-      HIP_TRY(hipStreamWaitEvent(st, lc.done, 0));
-    }
+      HIP_TRY(hipLaunchKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(lc.nthreads), args, 0, st));
   }
+  return PTY_OK;
+}
+
+int pty_plan_use_graph(pty_plan* p, int32_t enable) {
+  if (!p) return set_err(PTY_EINVAL, "pty_plan_use_graph: null plan");
+  p->use_graph = enable != 0;
+  return PTY_OK;
+}
+
+int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count) {
+  if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_E3PIE || propagator < PTY_FRAUNHOFER ||
+      propagator > PTY_ASP)
+    return set_err(PTY_EINVAL, "pty_coresident_workgroups: bad arguments");
+  pty_instance a;
+  memset(&a, 0, sizeof a);
+  a.Np = Np;
+  a.engine = engine;
+  a.propagator = propagator;
+  a.flags = batched ? PTY_FLAG_BATCHED : 0;
+  const bool b = joins_batch(a, 1);
+  const int nt = b ? BATCH_NT : nthreads_rt(Np);
+  int dev = 0, cus = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(b ? 0 : Np, family_of(a)), nt, 0));
+  *count = per_cu * cus;
   return PTY_OK;
 }
 

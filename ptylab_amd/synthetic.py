@@ -202,17 +202,27 @@ def cpm_multislice(Np=128, No=512, F=400, nslice=4, dz=5e-6, seed=20, peak=1e4, 
                     slices[None, None, None], probe0[None, None, None, None], corners, No, Np)
 
 
-def batched_instances(n=64, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800):
-    """Config "batched": per instance Np uniform in `nps`, No = 4 Np,
-    F uniform in [fmin, fmax], inputs as in the plain-ePIE family."""
-    out = []
-    for i in range(n):
-        rng = np.random.default_rng(seed0 + i)
-        Np = int(rng.choice(nps))
-        F = int(rng.integers(fmin, fmax + 1))
-        No = 4 * Np
-        lim = (No - Np) // 2 - 1
-        step = min(0.25 * Np, 0.9 * lim * np.sqrt(np.pi / F))
-        w = cpm_fraunhofer(Np=Np, No=No, F=F, seed=seed0 + i, step_frac=step / Np)
-        out.append(w)
-    return out
+def batched_instance(i, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800):
+    """Instance i of the batched configuration: Np uniform in `nps`,
+    No = 4 Np, F uniform in [fmin, fmax], inputs as in the plain-ePIE
+    family, all drawn from seed seed0 + i."""
+    rng = np.random.default_rng(seed0 + i)
+    Np = int(rng.choice(nps))
+    F = int(rng.integers(fmin, fmax + 1))
+    No = 4 * Np
+    lim = (No - Np) // 2 - 1
+    step = min(0.25 * Np, 0.9 * lim * np.sqrt(np.pi / F))
+    return cpm_fraunhofer(Np=Np, No=No, F=F, seed=seed0 + i, step_frac=step / Np)
+
+
+def batched_instances(n=64, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, workers=1):
+    """Config "batched": n instances (batched_instance(i), i < n).  With
+    workers > 1 they are generated by a pool of forked processes; call it
+    before the process initialises the GPU."""
+    args = [(i, seed0, nps, fmin, fmax) for i in range(n)]
+    if workers <= 1:
+        return [batched_instance(*a) for a in args]
+    import multiprocessing as mp
+
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.starmap(batched_instance, args, chunksize=1)

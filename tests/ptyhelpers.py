@@ -27,8 +27,9 @@ def make_data(d):
     return data
 
 
-def run_package(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=None,
-                refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None):
+def build_package(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=None,
+                  refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None):
+    """Engine set up as the reference's scripts do, not yet reconstructed."""
     nl, no, npm, ns = modes
     data = make_data(d)
     params = P.Params()
@@ -53,9 +54,16 @@ def run_package(d, engine, propagator, modes, iters, seed, order="sequential", N
     eng.workgroups = workgroups
     for k, v in (coeffs or {}).items():
         setattr(eng, k, v)
+    return eng
+
+
+def run_package(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=None,
+                refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None):
+    eng = build_package(d, engine, propagator, modes, iters, seed, order=order, No=No, dz=dz,
+                        refrIndex=refrIndex, init=init, workgroups=workgroups, orth=orth, coeffs=coeffs)
     np.random.seed(seed)
     eng.reconstruct()
-    return rec
+    return eng.reconstruction
 
 
 def run_oracle(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=0.0,

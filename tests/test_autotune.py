@@ -1,0 +1,83 @@
+"""Cluster-size allocation of the autotuner (host logic, no GPU): the
+greedy doubling against brute force on small batches, capacity and
+divisibility constraints, and the error for batches that cannot fit."""
+
+import itertools
+
+import pytest
+
+from ptylab_amd import autotune as A
+
+
+def _cost(shape, G):
+    # work splits G ways; every extra workgroup adds barrier overhead
+    work = shape.Np * shape.Np * shape.nF * 1e-3
+    return work / G + 0.8 * (G > 1) + 0.05 * G
+
+
+def _jobs(specs):
+    return [A.Job(A.Shape(1, 0, Np, 4 * Np, nF), F, it) for Np, nF, F, it in specs]
+
+
+def _brute(jobs, cap):
+    best = None
+    choices = [A.candidate_sizes(j.shape, cap) for j in jobs]
+    for G in itertools.product(*choices):
+        if sum(G) > cap:
+            continue
+        m = A.predicted_makespan(jobs, _cost, G)
+        best = m if best is None else min(best, m)
+    return best
+
+
+@pytest.mark.parametrize("cap", [3, 4, 6, 9, 16, 40])
+def test_greedy_matches_brute_force(cap):
+    jobs = _jobs([(64, 1, 300, 10), (128, 1, 120, 10), (32, 2, 700, 10)])
+    G = A.assign(jobs, _cost, cap)
+    assert sum(G) <= cap
+    assert all(j.shape.Np % g == 0 and g in A.CANDIDATES for j, g in zip(jobs, G))
+    got = A.predicted_makespan(jobs, _cost, G)
+    assert got == pytest.approx(_brute(jobs, cap), rel=1e-12)
+
+
+def test_never_worse_than_one_workgroup_each():
+    jobs = _jobs([(256, 4, 800, 100), (64, 1, 100, 100), (128, 1, 450, 50), (64, 1, 800, 100)])
+    G = A.assign(jobs, _cost, 256)
+    assert A.predicted_makespan(jobs, _cost, G) <= A.predicted_makespan(jobs, _cost, [1] * len(jobs))
+    assert sum(G) <= 256
+    # the most expensive instance gets the largest cluster
+    assert G[0] == max(G)
+
+
+def test_capacity_equal_to_batch_size_gives_ones():
+    jobs = _jobs([(64, 1, 100, 1)] * 5)
+    assert A.assign(jobs, _cost, 5) == [1] * 5
+
+
+def test_too_many_instances_raise():
+    with pytest.raises(ValueError):
+        A.assign(_jobs([(64, 1, 100, 1)] * 5), _cost, 4)
+
+
+def test_launch_keys_split_by_family_and_size():
+    from ptylab_amd import _native
+
+    a = _native.PtyInstance()
+    a.engine, a.propagator, a.Np = _native.PTY_MPIE, _native.PTY_FRAUNHOFER, 128
+    b = _native.PtyInstance()
+    b.engine, b.propagator, b.Np = _native.PTY_EPIE, _native.PTY_FRAUNHOFER, 64
+    c = _native.PtyInstance()
+    c.engine, c.propagator, c.Np = _native.PTY_MPIE, _native.PTY_ASP, 64
+    d = _native.PtyInstance()
+    d.engine, d.propagator, d.Np = _native.PTY_EPIE, _native.PTY_FRAUNHOFER, 512
+    keys = [A.launch_key(x) for x in (a, b, c, d)]
+    assert keys[0] == keys[1] != keys[2]
+    assert keys[3] not in keys[:3]
+
+
+def test_cost_table_round_trips_json(tmp_path):
+    t = A.CostTable(tmp_path / "t.json")
+    s = A.Shape(1, 0, 64, 256, 1)
+    t.table[s.key(2)] = 12.5
+    t.save()
+    assert A.CostTable(tmp_path / "t.json")(s, 2) == 12.5

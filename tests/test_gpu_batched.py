@@ -1,0 +1,123 @@
+"""Batched launch: many independent reconstructions of uneven Np, position
+count and iteration count in one launch per pass family, replayed as a
+per-iteration HIP graph.
+
+A batch must leave every engine exactly where its own reconstruct() would
+have: the per-position arithmetic is the same whichever kernel, workgroup
+size or cluster size runs it, so objects and probes are compared
+bit-for-bit against separate runs (the FP64 error sums may differ in the
+last bits: their partial sums group differently with the workgroup shape).
+One member is also checked against the complex64 oracle at the one-iteration
+bar (relative error <= 1e-5)."""
+
+import numpy as np
+import pytest
+
+from ptyhelpers import build_package, rel, run_oracle
+from ptylab_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+# (engine, propagator, modes, Np, No, F, iterations, order, seed)
+CASES = [
+    ("ePIE", "Fraunhofer", (1, 1, 1, 1), 64, 256, 40, 1, "sequential", 1),
+    ("mPIE", "Fraunhofer", (1, 1, 1, 1), 32, 128, 60, 4, "random", 2),
+    ("mPIE", "Fraunhofer", (1, 1, 1, 1), 128, 512, 30, 3, "random", 3),
+    ("ePIE", "Fraunhofer", (1, 1, 1, 1), 256, 640, 12, 2, "random", 4),
+    ("mPIE", "ASP", (1, 1, 2, 1), 64, 256, 25, 4, "random", 5),
+    ("ePIE", "Fraunhofer", (1, 1, 1, 1), 512, 1024, 4, 2, "sequential", 6),
+]
+
+
+@pytest.fixture(scope="module")
+def datasets():
+    out = []
+    for engine, prop, modes, Np, No, F, iters, order, seed in CASES:
+        if prop == "ASP":
+            w = S.cpm_asp_modes(Np=Np, No=No, F=F, npsm=modes[2], seed=seed, probe_radius_px=Np // 4)
+        else:
+            w = S.cpm_fraunhofer(Np=Np, No=No, F=F, seed=seed)
+        out.append(w)
+    return out
+
+
+def _engines(datasets):
+    engs = []
+    for (engine, prop, modes, Np, No, F, iters, order, seed), w in zip(CASES, datasets):
+        engs.append(build_package(w.as_dict(), engine, prop, modes, iters, seed, order=order, No=w.No))
+    return engs
+
+
+def _separate(datasets):
+    engs = _engines(datasets)
+    np.random.seed(77)
+    for e in engs:
+        e.reconstruct()
+    return [e.reconstruction for e in engs]
+
+
+@pytest.fixture(scope="module")
+def separate(datasets):
+    return _separate(datasets)
+
+
+def _batched(datasets, **kw):
+    import ptylab_amd as P
+
+    engs = _engines(datasets)
+    np.random.seed(77)
+    b = P.Engines.Batched(engs, **kw).reconstruct()
+    return [e.reconstruction for e in engs], b
+
+
+def _same(a, b, tag):
+    for ra, rb, case in zip(a, b, CASES):
+        name = f"{tag} {case[0]} Np={case[3]}"
+        assert np.array_equal(np.asarray(ra.object), np.asarray(rb.object)), name
+        assert np.array_equal(np.asarray(ra.probe), np.asarray(rb.probe)), name
+        assert len(ra.error) == len(rb.error) == case[6], name
+        np.testing.assert_allclose(np.asarray(ra.error), np.asarray(rb.error), rtol=1e-12, err_msg=name)
+
+
+def test_batched_graph_equals_separate_runs(datasets, separate):
+    got, b = _batched(datasets, workgroups=[1, 1, 2, 4, 2, 1], graph=True)
+    _same(separate, got, "graph")
+
+
+def test_batched_direct_launch_equals_separate_runs(datasets, separate):
+    got, _ = _batched(datasets, workgroups=[2, 1, 1, 8, 1, 2], graph=False)
+    _same(separate, got, "direct")
+
+
+def test_batched_autotuned_equals_separate_runs(datasets, separate, tmp_path):
+    from ptylab_amd import autotune
+
+    table = autotune.CostTable(tmp_path / "tune.json", positions=16)
+    got, b = _batched(datasets, tuner=table)
+    G = b.last_workgroups
+    print("autotuned cluster sizes", G, flush=True)
+    assert all(g >= 1 and (g & (g - 1)) == 0 and c[3] % g == 0 for g, c in zip(G, CASES))
+    assert all(v > 0 for v in table.table.values())
+    _same(separate, got, "autotuned")
+
+
+def test_batched_member_against_oracle(datasets):
+    got, _ = _batched(datasets, workgroups=[4, 1, 1, 1, 1, 1])
+    engine, prop, modes, Np, No, F, iters, order, seed = CASES[0]
+    ref = run_oracle(datasets[0].as_dict(), engine, prop, modes, iters, seed, order=order, No=No)
+    eo, ep = rel(got[0].object, ref.object), rel(got[0].probe, ref.probe)
+    print(f"batched member vs oracle: object {eo:.2e} probe {ep:.2e}", flush=True)
+    assert eo <= 1e-5 and ep <= 1e-5
+    np.testing.assert_allclose(got[0].error, ref.error, rtol=1e-5)
+
+
+def test_too_many_fields_for_a_workgroup_raise(datasets):
+    """16 probe modes need 16 waves per column round; the batched kernel's
+    workgroups have 8, so plan creation rejects the instance."""
+    import ptylab_amd as P
+    from ptylab_amd import _native
+
+    w = datasets[0]
+    eng = build_package(w.as_dict(), "ePIE", "Fraunhofer", (1, 1, 16, 1), 1, 1, No=w.No)
+    with pytest.raises(_native.NativeError, match="exceed the 8 waves"):
+        P.Engines.Batched([eng], workgroups=[1]).reconstruct()
