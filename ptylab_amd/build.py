@@ -8,6 +8,8 @@ HERE = Path(__file__).resolve().parent
 REPO = HERE.parent
 SRC = HERE / "csrc" / "pty_hip.hip"
 OUT = HERE / "libptyhip.so"
+NET_SRC = HERE / "csrc" / "pty_net.hip"
+NET_OUT = HERE / "libptynet.so"
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -24,17 +26,29 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose=False):
+def _compile(src, out, verbose):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, f"-I{REPO / 'include'}", "-o", str(tmp), str(SRC)]
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [hipcc, *HIPCC_FLAGS, f"-I{REPO / 'include'}", "-o", str(tmp), str(src)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-4000:]}")
-    os.replace(tmp, OUT)
+        raise RuntimeError(f"hipcc failed on {src.name} ({res.returncode}):\n{res.stderr[-4000:]}")
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT}")
-    return OUT
+        print(f"built {out}")
+    return out
+
+
+def build(verbose=False):
+    """Compile both libraries: the reconstruction hot path (libptyhip.so)
+    and the learned initialiser's network kernels (libptynet.so)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(2) as ex:
+        f1 = ex.submit(_compile, SRC, OUT, verbose)
+        f2 = ex.submit(_compile, NET_SRC, NET_OUT, verbose)
+        f2.result()
+        return f1.result()
 
 
 if __name__ == "__main__":

@@ -37,3 +37,48 @@ def test_struct_layout_matches_header():
 
     # 10 int32 + 8 float + 14 pointers + 2 int32, naturally aligned
     assert ctypes.sizeof(_native.PtyInstance) == 10 * 4 + 8 * 4 + 14 * 8 + 2 * 4
+
+
+def _declared_net():
+    text = (ROOT / "include" / "pty_net.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(pty_net_\w+)\(", text, re.M)))
+
+
+def test_network_library_exports_every_declared_symbol():
+    from ptylab_amd import unet
+
+    names = _declared_net()
+    assert "pty_net_conv_fwd" in names and "pty_net_conv_wgrad" in names
+    if not unet.LIB_PATH.exists():
+        pytest.skip("libptynet.so not built")
+    lib = ctypes.CDLL(str(unet.LIB_PATH))
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(unet.EXPORTED) == set(names)
+    assert unet.load().pty_net_kpad(3, 8) == unet.kpad(3, 8) == 96
+
+
+def _param_counts(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?\w+\*?\s+\*?(pty_\w+)\(([^)]*)\);", text, re.M | re.S):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+@pytest.mark.parametrize("header,module", [("pty_hip.h", "_native"), ("pty_net.h", "unet")])
+def test_ctypes_signatures_match_header_parameter_counts(header, module):
+    import importlib
+
+    mod = importlib.import_module(f"ptylab_amd.{module}")
+    if not mod.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = mod.load()
+    for name, n in _param_counts(header).items():
+        at = getattr(lib, name).argtypes
+        if at is None:
+            assert n == 0, name
+        else:
+            assert len(at) == n, (name, len(at), n)
