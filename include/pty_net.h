@@ -74,11 +74,13 @@ int pty_net_up_concat_bwd(const void* dy, int ca_s, int cb_s, void* dxa, const v
  * contributions at a ReLU output and applies the ReLU derivative. */
 int pty_net_add_gate(const void* g1, const void* g2, const void* act, void* y, size_t n, void* stream);
 
-/* Mean squared error of n fp32 outputs (pairs: real, imag) against tgt:
- * dout (n/2 pixels, c_s channels) bf16 = 2 (out - tgt) / n; partial[b] is
- * the FP64 sum of squares of block b (nblocks blocks). */
-int pty_net_mse(const float* out, const float* tgt, size_t n, int c_s, void* dout, double* partial, int nblocks,
-                void* stream);
+/* Mean squared error of the crop [oy, oy+hc) x [ox, ox+wc) of the fp32
+ * outputs out (B,Ho,Wo,2) (real, imaginary) against tgt (B,hc,wc,2):
+ * dout (B,Ho,Wo,c_s) bf16 = 2 (out - tgt) / n inside the crop
+ * (n = B*hc*wc*2; entries outside are not written); partial[b] is the FP64
+ * sum of squares of block b (nblocks blocks). */
+int pty_net_mse(const float* out, const float* tgt, int B, int Ho, int Wo, int oy, int ox, int hc, int wc, int c_s,
+                void* dout, double* partial, int nblocks, void* stream);
 
 /* One Adam step (step >= 1) on n fp32 weights; refreshes the bf16 copy w16
  * (NULL: none). */

@@ -27,13 +27,14 @@ import torch.nn.functional as Fn
 
 # (name, cin, cin_s, cout, ks, relu): the network of ptylab_amd/unet.py
 LAYERS = (
-    ("e0a", 1, 8, 32, 3, True), ("e0b", 32, 32, 32, 3, True),
+    ("e0a", 4, 8, 32, 3, True), ("e0b", 32, 32, 32, 3, True),
     ("e1a", 32, 32, 64, 3, True), ("e1b", 64, 64, 64, 3, True),
     ("e2a", 64, 64, 128, 3, True), ("e2b", 128, 128, 128, 3, True),
     ("e3a", 128, 128, 256, 3, True), ("e3b", 256, 256, 256, 3, True),
     ("d2a", 384, 384, 128, 3, True), ("d2b", 128, 128, 128, 3, True),
     ("d1a", 192, 192, 64, 3, True), ("d1b", 64, 64, 64, 3, True),
-    ("head", 64, 64, 2, 1, False),
+    ("d0a", 96, 96, 32, 3, True), ("d0b", 32, 32, 32, 3, True),
+    ("head", 32, 32, 2, 1, False),
 )
 SHAPE = {n: (cin, cin_s, cout, ks, relu) for n, cin, cin_s, cout, ks, relu in LAYERS}
 
@@ -57,7 +58,7 @@ def conv_nhwc(x, w, b, ks):
 
 def forward(params, x, mirror_bf16=True, requires_grad=False):
     """params {name: (w (cout, Kp), b (cout,))}; x (B, 128, 128, 8) array.
-    Returns (out (B, 64, 64, 2), activations dict, leaf weight tensors)."""
+    Returns (out (B, 128, 128, 2), activations dict, leaf weight tensors)."""
     r = bf16 if mirror_bf16 else (lambda t: t)
     leaves = {}
     for name, *_ in LAYERS:
@@ -86,6 +87,7 @@ def forward(params, x, mirror_bf16=True, requires_grad=False):
 
     x = torch.as_tensor(np.asarray(x, dtype=np.float64))
     h = conv("e0b", conv("e0a", x))
+    s0 = h
     h = conv("e1b", conv("e1a", pool(h)))
     s1 = h
     h = conv("e2b", conv("e2a", pool(h)))
@@ -93,16 +95,19 @@ def forward(params, x, mirror_bf16=True, requires_grad=False):
     h = conv("e3b", conv("e3a", pool(h)))
     h = conv("d2b", conv("d2a", torch.cat([up(h), s2], dim=3)))
     h = conv("d1b", conv("d1a", torch.cat([up(h), s1], dim=3)))
+    h = conv("d0b", conv("d0a", torch.cat([up(h), s0], dim=3)))
     out = conv("head", h)
     return out, act, leaves
 
 
 def loss_and_grads(params, x, target, mirror_bf16=True):
-    """MSE of the output against target (B, 64, 64, 2) and its gradients
-    {name: (dw (cout, Kp) rows, db (cout,))} in float64."""
+    """MSE of the output's central crop (the target's size, centred) against
+    target and the gradients {name: (dw (cout, Kp) rows, db (cout,))} in
+    float64."""
     out, act, leaves = forward(params, x, mirror_bf16=mirror_bf16, requires_grad=True)
     t = torch.as_tensor(np.asarray(target, np.float64))
-    loss = ((out - t) ** 2).mean()
+    o0 = (out.shape[1] - t.shape[1]) // 2
+    loss = ((out[:, o0:o0 + t.shape[1], o0:o0 + t.shape[2]] - t) ** 2).mean()
     loss.backward()
     grads = {}
     for name, (wt, bt) in leaves.items():

@@ -365,17 +365,27 @@ __global__ void add_gate_kernel(const bf16_t* __restrict__ g1, const bf16_t* __r
   }
 }
 
-// MSE over (B, HW, 2) fp32 outputs; dout (B, HW, c_s) bf16 = 2 (out - t) / N;
-// per-block FP64 partial sums of the squared error
-__global__ void mse_kernel(const float* __restrict__ out, const float* __restrict__ tgt, size_t n, int c_s,
-                           bf16_t* __restrict__ dout, double* __restrict__ partial) {
+// MSE over the crop [oy, oy + hc) x [ox, ox + wc) of (B, Ho, Wo, 2) fp32
+// outputs against tgt (B, hc, wc, 2); dout (B, Ho, Wo, c_s) bf16 gets
+// 2 (out - t) / n inside the crop (n = B hc wc 2; outside stays as it is,
+// zero); per-block FP64 partial sums of the squared error
+__global__ void mse_kernel(const float* __restrict__ out, const float* __restrict__ tgt, int B, int Ho, int Wo, int oy,
+                           int ox, int hc, int wc, int c_s, bf16_t* __restrict__ dout, double* __restrict__ partial) {
   __shared__ double red[256];
+  const size_t n = (size_t)B * hc * wc * 2;
   const float scale = 2.0f / (float)n;
   double acc = 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const float d = out[i] - tgt[i];
+    const int c = (int)(i & 1);
+    size_t r = i >> 1;
+    const int x = (int)(r % wc);
+    r /= wc;
+    const int y = (int)(r % hc);
+    const int b = (int)(r / hc);
+    const size_t pix = ((size_t)b * Ho + oy + y) * Wo + ox + x;
+    const float d = out[pix * 2 + c] - tgt[i];
     acc += (double)d * (double)d;
-    dout[(i >> 1) * c_s + (i & 1)] = f2bf(scale * d);
+    dout[pix * c_s + c] = f2bf(scale * d);
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -543,10 +553,12 @@ int pty_net_add_gate(const void* g1, const void* g2, const void* act, void* y, s
   return PTY_NET_OK;
 }
 
-int pty_net_mse(const float* out, const float* tgt, size_t n, int c_s, void* dout, double* partial, int nblocks,
-                void* stream) {
-  if (!out || !tgt || !dout || !partial || nblocks < 1 || n % 2 || c_s % 8) return set_err(PTY_NET_EINVAL, "mse: bad arguments");
-  mse_kernel<<<nblocks, 256, 0, (hipStream_t)stream>>>(out, tgt, n, c_s, (bf16_t*)dout, partial);
+int pty_net_mse(const float* out, const float* tgt, int B, int Ho, int Wo, int oy, int ox, int hc, int wc, int c_s,
+                void* dout, double* partial, int nblocks, void* stream) {
+  if (!out || !tgt || !dout || !partial || nblocks < 1 || c_s % 8 || B < 1 || hc < 1 || wc < 1 || oy < 0 || ox < 0 ||
+      oy + hc > Ho || ox + wc > Wo)
+    return set_err(PTY_NET_EINVAL, "mse: bad arguments");
+  mse_kernel<<<nblocks, 256, 0, (hipStream_t)stream>>>(out, tgt, B, Ho, Wo, oy, ox, hc, wc, c_s, (bf16_t*)dout, partial);
   HIP_TRY(hipGetLastError());
   return PTY_NET_OK;
 }

@@ -9,10 +9,12 @@ Workload (the learned-initialiser configuration): probe Np = 128 (the
 plain-ePIE family's aperture probe, radius 32 px, quadratic phase,
 propagated one Fresnel length), objects of the smoothed random
 amplitude/phase type, Fraunhofer data at 1e4 photons peak with Poisson
-noise.  Network input: log(1 + I) / 10 of a 128 x 128 pattern; output: the
-central 64 x 64 object patch (the illuminated disc) divided by its complex
-mean (the global phase and scale are not determined by one pattern), as
-real and imaginary channels.
+noise.  Network input (features()): the measured amplitudes back-propagated
+with the probe's far-field phase and divided by the probe, which puts a
+first object estimate in real space, aligned with the output; the probe
+intensity; log(1 + I) / 10.  Output: the central 64 x 64 object patch (the
+illuminated disc) divided by its complex mean (the global phase and scale
+are not determined by one pattern), as real and imaginary channels.
 
 Training data come from a GPU simulator that uses the hot path's own
 Fraunhofer propagator (pty_propagate) on random objects; the held-out scan
@@ -66,6 +68,7 @@ class Simulator:
         self.gen = torch.Generator(device=self.dev)
         self.gen.manual_seed(int(seed))
         self.P = torch.as_tensor(np.asarray(probe_np, np.complex64), device=self.dev)
+        self.feat = Features(probe_np)
         self.canvas, self.wpo, self.peak = canvas, windows_per_object, peak
         n = canvas + 8 * int(sigma) * 2
         self.pad = 8 * int(sigma)
@@ -114,9 +117,47 @@ class Simulator:
         I = exit_wave.real ** 2 + exit_wave.imag ** 2
         I = I * (self.peak / I.max())
         I = torch.poisson(I, generator=self.gen)
-        x = torch.zeros((B, NP, NP, 8), dtype=torch.bfloat16, device=self.dev)
-        x[..., 0] = (torch.log1p(I) / 10.0).to(torch.bfloat16)
-        return x, patch_targets(win)
+        return self.feat(I), patch_targets(win)
+
+
+class Features:
+    """Network input from measured intensities (B, 128, 128) on the GPU:
+
+    0, 1  real, imaginary of O_est = psi0 conj(P) / (|P|^2 + 0.05 max |P|^2),
+          psi0 = ifft2c(sqrt(I) exp(i arg fft2c(P))) / s, s^2 = sum I / sum |P|^2
+          (the measured amplitudes with the probe's far-field phase, back in
+          the object plane with the hot path's Fraunhofer propagator)
+    2     |P|^2 / max |P|^2
+    3     log(1 + I) / 10
+    """
+
+    def __init__(self, probe_np):
+        import torch
+
+        self.P = torch.as_tensor(np.asarray(probe_np, np.complex64), device="cuda")
+        far = self.P.clone().contiguous()
+        _native.propagate(far, _native.PTY_FRAUNHOFER, inverse=False)
+        self.phase = (far / far.abs().clamp_min(1e-30)).contiguous()
+        i2 = self.P.abs() ** 2
+        self.pw = float(i2.sum())
+        self.den = i2 + 0.05 * i2.max()
+        self.pint = (i2 / i2.max()).to(torch.bfloat16)
+
+    def __call__(self, I):
+        import torch
+
+        B = I.shape[0]
+        amp = torch.sqrt(I.clamp_min(0))
+        psi = (amp * self.phase[None]).to(torch.complex64).contiguous()
+        _native.propagate(psi, _native.PTY_FRAUNHOFER, inverse=True)
+        s = torch.sqrt(I.sum(dim=(1, 2)) / self.pw).clamp_min(1e-12)
+        est = psi * self.P.conj()[None] / self.den[None] / s[:, None, None]
+        x = torch.zeros((B, NP, NP, 8), dtype=torch.bfloat16, device=I.device)
+        x[..., 0] = est.real.to(torch.bfloat16)
+        x[..., 1] = est.imag.to(torch.bfloat16)
+        x[..., 2] = self.pint[None]
+        x[..., 3] = (torch.log1p(I) / 10.0).to(torch.bfloat16)
+        return x
 
 
 def patch_targets(win):
@@ -148,21 +189,12 @@ def train(net, steps, seed=0, lr=1e-3, lr_final=1e-4, log_every=0, sim=None):
 # ---------------------------------------------------------------------------
 # inference, stitching, NRMSE
 # ---------------------------------------------------------------------------
-def network_input(ptychogram):
-    """(F, 128, 128) intensities -> (F, 128, 128, 8) bf16 network input."""
-    import torch
-
-    I = torch.as_tensor(np.asarray(ptychogram, np.float32), device="cuda")
-    x = torch.zeros((I.shape[0], NP, NP, 8), dtype=torch.bfloat16, device="cuda")
-    x[..., 0] = (torch.log1p(I) / 10.0).to(torch.bfloat16)
-    return x
-
-
-def predict(net, ptychogram):
+def predict(net, ptychogram, probe_np):
     """Patch estimates (F, 64, 64) complex64 on the GPU, batches of net.B."""
     import torch
 
-    x = network_input(ptychogram)
+    I = torch.as_tensor(np.asarray(ptychogram, np.float32), device="cuda")
+    x = Features(probe_np)(I)
     F = x.shape[0]
     out = torch.empty((F, PATCH, PATCH), dtype=torch.complex64, device="cuda")
     B = net.B
@@ -171,39 +203,58 @@ def predict(net, ptychogram):
         xb = x[b0:b0 + n]
         if n < B:
             xb = torch.cat([xb, torch.zeros((B - n,) + tuple(x.shape[1:]), dtype=x.dtype, device=x.device)])
-        y = net.forward(xb)
-        out[b0:b0 + n] = torch.complex(y[:n, ..., 0], y[:n, ..., 1])
+        net.forward(xb)
+        out[b0:b0 + n] = net.prediction()[:n]
     return out
 
 
 def stitch(patches, positions, probe_np, No):
     """Probe-weighted average of the patches into a (No, No) object.
 
-    Patches are placed in order of distance from the scan centre; each is
-    first scaled by the complex gamma that best matches it (weighted least
-    squares) to what is already stitched, since every patch carries its own
-    arbitrary complex scale.  Pixels no patch covers keep the 'ones' value;
-    the result is scaled to unit mean amplitude on the covered region."""
+    Every patch carries its own arbitrary complex scale, so patches are
+    placed in rings of growing size around the scan centre (1, 4, 8, 16, 32,
+    then 64 at a time, by distance); each patch of a ring is first scaled by
+    the complex gamma that best matches it (weighted least squares) to what
+    the inner rings already stitched, then the ring is added with one
+    scatter-add.  Pixels no patch covers keep the 'ones' value; the result is
+    scaled to unit mean amplitude on the covered region."""
     import torch
 
     dev = patches.device
+    F = patches.shape[0]
     P = torch.as_tensor(np.asarray(probe_np, np.complex64), device=dev)
     w = (P.abs() ** 2)[OFF:OFF + PATCH, OFF:OFF + PATCH].to(torch.float32)
     w = w / w.max()
-    num = torch.zeros((No, No), dtype=torch.complex64, device=dev)
-    den = torch.zeros((No, No), dtype=torch.float32, device=dev)
     pos = np.asarray(positions, np.int64) + OFF
     c = pos.mean(axis=0)
     order = np.argsort(((pos - c) ** 2).sum(axis=1), kind="stable")
-    for j in order:
-        r, q = int(pos[j, 0]), int(pos[j, 1])
-        T = patches[j]
-        nv = num[r:r + PATCH, q:q + PATCH]
-        dv = den[r:r + PATCH, q:q + PATCH]
-        d = (w * dv * (T.abs() ** 2)).sum()
-        g = (w * T.conj() * nv).sum() / d if float(d) > 1e-6 else torch.ones((), dtype=torch.complex64, device=dev)
-        nv += w * g * T
-        dv += w
+    ar = torch.arange(PATCH, device=dev)
+    pt = torch.as_tensor(pos, device=dev)
+    idx = ((pt[:, 0, None, None] + ar[None, :, None]) * No + pt[:, 1, None, None] + ar[None, None, :])  # (F, 64, 64)
+    num = torch.zeros(No * No, dtype=torch.complex64, device=dev)
+    den = torch.zeros(No * No, dtype=torch.float32, device=dev)
+    sizes, start = [1, 4, 8, 16, 32], 0
+    groups = []
+    for n in sizes:
+        groups.append(order[start:start + n])
+        start += n
+    while start < F:
+        groups.append(order[start:start + 64])
+        start += 64
+    wf = w[None]
+    for g in groups:
+        if len(g) == 0:
+            continue
+        ids = torch.as_tensor(g, device=dev)
+        T = patches[ids]
+        ix = idx[ids]
+        nv, dv = num[ix], den[ix]
+        d = (wf * dv * (T.abs() ** 2)).sum(dim=(1, 2))
+        gam = (wf * T.conj() * nv).sum(dim=(1, 2)) / d.clamp_min(1e-12)
+        gam = torch.where(d > 1e-6, gam, torch.ones_like(gam))
+        num.index_add_(0, ix.reshape(-1), (wf * gam[:, None, None] * T).reshape(-1))
+        den.index_add_(0, ix.reshape(-1), wf.expand(len(g), -1, -1).reshape(-1))
+    num, den = num.view(No, No), den.view(No, No)
     cover = den > 0.05
     obj = torch.ones((No, No), dtype=torch.complex64, device=dev)
     obj[cover] = num[cover] / den[cover]
@@ -257,3 +308,46 @@ def run_to_target(engine, true_obj, roi, target=1e-2, max_iter=300):
         engine._end(done)
     torch.cuda.synchronize()
     return (done if hit is not None else None), (hit if hit is not None else time.perf_counter() - t0), curve
+
+
+def data_misfit(obj, ptychogram, positions, probe_np):
+    """sum_j ||sqrt(I_j) - |fft2c(P O_j)|||^2 / sum_j I_j for an object guess
+    (GPU, the hot path's Fraunhofer propagator): how well a candidate
+    initial object explains the measured amplitudes with the known probe."""
+    import torch
+
+    I = torch.as_tensor(np.asarray(ptychogram, np.float32), device=obj.device)
+    P = torch.as_tensor(np.asarray(probe_np, np.complex64), device=obj.device)
+    pos = torch.as_tensor(np.asarray(positions, np.int64), device=obj.device)
+    ar = torch.arange(NP, device=obj.device)
+    rows = (pos[:, 0, None] + ar[None])[:, :, None]
+    cols = (pos[:, 1, None] + ar[None])[:, None, :]
+    ex = (obj[rows, cols] * P[None]).contiguous()
+    _native.propagate(ex, _native.PTY_FRAUNHOFER, inverse=False)
+    s = torch.sqrt(I.sum() / (ex.abs() ** 2).sum())  # the data's global scale
+    return float(((torch.sqrt(I.clamp_min(0)) - s * ex.abs()) ** 2).sum() / I.sum())
+
+
+def initialize_object(reconstruction, experimentalData, net, probe_np):
+    """Replace reconstruction.object by the learned initial object when it
+    explains the data better than the uniform guess does (data_misfit with
+    the known probe); otherwise keep the uniform 'ones' object.  Returns a
+    dict: choice ('learned' or 'ones'), both misfits and the seconds spent.
+    The object keeps the reconstruction's 6D shape and complex64 dtype."""
+    import torch
+
+    t0 = time.perf_counter()
+    pos = np.asarray(reconstruction.positions, np.int64)
+    No = int(reconstruction.No)
+    I = experimentalData.ptychogram
+    patches = predict(net, I, probe_np)
+    obj = stitch(patches, pos, probe_np, No)
+    ones = torch.ones_like(obj)
+    e_l = data_misfit(obj, I, pos, probe_np)
+    e_o = data_misfit(ones, I, pos, probe_np)
+    choice = "learned" if e_l < e_o else "ones"
+    if choice == "learned":
+        o = obj.cpu().numpy().astype(np.complex64)
+        reconstruction.object = np.broadcast_to(o, reconstruction.shape_O).copy()
+    torch.cuda.synchronize()
+    return dict(choice=choice, misfit_learned=e_l, misfit_ones=e_o, seconds=time.perf_counter() - t0)

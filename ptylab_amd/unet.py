@@ -1,19 +1,25 @@
 """U-Net of the learned initialiser on the gfx950 kernels of libptynet.so.
 
-Architecture (input 128x128 diffraction pattern, output 64x64 object patch):
+Architecture (input: 4 feature channels on the 128 x 128 probe window;
+output: the object over the window, real and imaginary; the loss uses the
+central 64 x 64 crop, the illuminated disc):
 
     level  resolution  convolutions (3x3, ReLU)        then
-    e0     128^2       1 -> 32, 32 -> 32                pool 2x2
+    e0     128^2       4 -> 32, 32 -> 32    (skip s0)   pool 2x2
     e1     64^2        32 -> 64, 64 -> 64   (skip s1)   pool
     e2     32^2        64 -> 128, 128 -> 128 (skip s2)  pool
     e3     16^2        128 -> 256, 256 -> 256
     d2     32^2        up(e3) | s2 : 384 -> 128, 128 -> 128
     d1     64^2        up(d2) | s1 : 192 -> 64, 64 -> 64
-    head   64^2        1x1 conv 64 -> 2 (real, imaginary), no activation
+    d0     128^2       up(d1) | s0 : 96 -> 32, 32 -> 32
+    head   128^2       1x1 conv 32 -> 2, no activation
 
-1.91 M parameters.  Activations and gradients are bf16 NHWC, weights have
+1.95 M parameters.  Activations and gradients are bf16 NHWC, weights have
 fp32 masters and bf16 copies; all convolutions run as MFMA implicit GEMMs.
-The input channel 0 holds log(1 + I) / 10; channels 1..7 are zero padding.
+Input channels (see learned_init.features): the measured amplitudes
+back-propagated with the probe's far-field phase and divided by the probe
+(real, imaginary), the probe intensity, log(1 + I) / 10; channels 4..7 are
+zero padding.
 """
 
 from __future__ import annotations
@@ -53,7 +59,7 @@ def load(path=None):
         "pty_net_up_concat": [vp, i32, vp, i32, vp, i32, i32, i32, vp],
         "pty_net_up_concat_bwd": [vp, i32, i32, vp, vp, vp, i32, i32, i32, vp],
         "pty_net_add_gate": [vp, vp, vp, vp, sz, vp],
-        "pty_net_mse": [vp, vp, sz, i32, vp, vp, i32, vp],
+        "pty_net_mse": [vp, vp] + [i32] * 8 + [vp, vp, i32, vp],
         "pty_net_adam": [vp, vp, vp, vp, vp, sz, f32, f32, f32, f32, i32, vp],
         "pty_net_cast_bf16": [vp, vp, sz, vp],
     }
@@ -102,16 +108,18 @@ class ConvSpec:
         return kpad(self.ks, self.cin_s)
 
 
-IN_RES, OUT_RES, IN_CH = 128, 64, 8
+IN_RES, IN_CH, N_FEATURES = 128, 8, 4
+CROP_OFF, CROP = 32, 64  # the loss window inside the 128 x 128 output
 
 ARCH = (
-    ConvSpec("e0a", 1, 8, 32, 3, 128), ConvSpec("e0b", 32, 32, 32, 3, 128),
+    ConvSpec("e0a", N_FEATURES, 8, 32, 3, 128), ConvSpec("e0b", 32, 32, 32, 3, 128),
     ConvSpec("e1a", 32, 32, 64, 3, 64), ConvSpec("e1b", 64, 64, 64, 3, 64),
     ConvSpec("e2a", 64, 64, 128, 3, 32), ConvSpec("e2b", 128, 128, 128, 3, 32),
     ConvSpec("e3a", 128, 128, 256, 3, 16), ConvSpec("e3b", 256, 256, 256, 3, 16),
     ConvSpec("d2a", 384, 384, 128, 3, 32), ConvSpec("d2b", 128, 128, 128, 3, 32),
     ConvSpec("d1a", 192, 192, 64, 3, 64), ConvSpec("d1b", 64, 64, 64, 3, 64),
-    ConvSpec("head", 64, 64, 2, 1, 64, relu=False),
+    ConvSpec("d0a", 96, 96, 32, 3, 128), ConvSpec("d0b", 32, 32, 32, 3, 128),
+    ConvSpec("head", 32, 32, 2, 1, 128, relu=False),
 )
 SPEC = {s.name: s for s in ARCH}
 
@@ -175,12 +183,12 @@ class UNet:
         for s in ARCH[:-1]:
             A[s.name] = z16(B, s.res, s.res, s.cout_s)
         A["p0"], A["p1"], A["p2"] = z16(B, 64, 64, 32), z16(B, 32, 32, 64), z16(B, 16, 16, 128)
-        A["u2"], A["u1"] = z16(B, 32, 32, 384), z16(B, 64, 64, 192)
-        A["out"] = z32(B, 64, 64, 2)
+        A["u2"], A["u1"], A["u0"] = z16(B, 32, 32, 384), z16(B, 64, 64, 192), z16(B, 128, 128, 96)
+        A["out"] = z32(B, 128, 128, 2)
         self.A = A
         G = {k: torch.zeros_like(v) for k, v in A.items() if k not in ("x", "out")}
-        G["out"] = z16(B, 64, 64, 8)
-        G["skip1"], G["skip2"] = z16(B, 64, 64, 64), z16(B, 32, 32, 128)
+        G["out"] = z16(B, 128, 128, 8)
+        G["skip0"], G["skip1"], G["skip2"] = z16(B, 128, 128, 32), z16(B, 64, 64, 64), z16(B, 32, 32, 128)
         G["pool"] = z16(B, 128, 128, 32)
         self.G = G
         self.splits = {s.name: self._splits(s) for s in ARCH}
@@ -231,8 +239,18 @@ class UNet:
         _check(L.pty_net_up_concat(A["d2b"].data_ptr(), 128, A["e1b"].data_ptr(), 64, A["u1"].data_ptr(), B, 64, 64, h))
         self._conv("d1a", A["u1"], A["d1a"])
         self._conv("d1b", A["d1a"], A["d1b"])
-        self._conv("head", A["d1b"], A["out"], cout_s=2, out_f32=True)
+        _check(L.pty_net_up_concat(A["d1b"].data_ptr(), 64, A["e0b"].data_ptr(), 32, A["u0"].data_ptr(), B, 128, 128, h))
+        self._conv("d0a", A["u0"], A["d0a"])
+        self._conv("d0b", A["d0a"], A["d0b"])
+        self._conv("head", A["d0b"], A["out"], cout_s=2, out_f32=True)
         return A["out"]
+
+    def prediction(self):
+        """The central crop of the last output, (B, 64, 64) complex64."""
+        import torch
+
+        o = self.A["out"][:, CROP_OFF:CROP_OFF + CROP, CROP_OFF:CROP_OFF + CROP]
+        return torch.complex(o[..., 0], o[..., 1])
 
     # -- backward -----------------------------------------------------------------
     def _wgrad(self, name, x, dz):
@@ -250,18 +268,26 @@ class UNet:
                    relu=False)
 
     def loss_and_grad(self, target):
-        """MSE against target (B, 64, 64, 2) fp32; fills G['out'].  Returns
-        the loss as a 0-d FP64 tensor (no host synchronisation)."""
-        n = self.B * 64 * 64 * 2
+        """MSE of the central 64 x 64 crop against target (B, 64, 64, 2) fp32;
+        fills G['out'].  Returns the loss as a 0-d FP64 tensor (no host
+        synchronisation)."""
+        n = self.B * CROP * CROP * 2
         nb = min(512, max(1, n // 2048))
-        _check(self.lib.pty_net_mse(self.A["out"].data_ptr(), target.data_ptr(), n, 8, self.G["out"].data_ptr(),
-                                    self.partial.data_ptr(), nb, self._h()))
+        _check(self.lib.pty_net_mse(self.A["out"].data_ptr(), target.data_ptr(), self.B, IN_RES, IN_RES, CROP_OFF,
+                                    CROP_OFF, CROP, CROP, 8, self.G["out"].data_ptr(), self.partial.data_ptr(), nb,
+                                    self._h()))
         return self.partial[:nb].sum() / n
 
     def backward(self):
         A, G, L, h, B = self.A, self.G, self.lib, self._h(), self.B
-        self._wgrad("head", A["d1b"], G["out"])
-        self._dgrad("head", G["out"], G["d1b"], A["d1b"])
+        self._wgrad("head", A["d0b"], G["out"])
+        self._dgrad("head", G["out"], G["d0b"], A["d0b"])
+        self._wgrad("d0b", A["d0a"], G["d0b"])
+        self._dgrad("d0b", G["d0b"], G["d0a"], A["d0a"])
+        self._wgrad("d0a", A["u0"], G["d0a"])
+        self._dgrad("d0a", G["d0a"], G["u0"], A["u0"])
+        _check(L.pty_net_up_concat_bwd(G["u0"].data_ptr(), 64, 32, G["d1b"].data_ptr(), None, G["skip0"].data_ptr(),
+                                       B, 128, 128, h))
         self._wgrad("d1b", A["d1a"], G["d1b"])
         self._dgrad("d1b", G["d1b"], G["d1a"], A["d1a"])
         self._wgrad("d1a", A["u1"], G["d1a"])
@@ -296,7 +322,8 @@ class UNet:
         self._dgrad("e1a", G["e1a"], G["p0"], None)
         _check(L.pty_net_pool_bwd(G["p0"].data_ptr(), G["pool"].data_ptr(), B, 128, 128, 32, h))
         n = B * 128 * 128 * 32
-        _check(L.pty_net_add_gate(G["pool"].data_ptr(), None, A["e0b"].data_ptr(), G["e0b"].data_ptr(), n, h))
+        _check(L.pty_net_add_gate(G["pool"].data_ptr(), G["skip0"].data_ptr(), A["e0b"].data_ptr(), G["e0b"].data_ptr(),
+                                  n, h))
         self._wgrad("e0b", A["e0a"], G["e0b"])
         self._dgrad("e0b", G["e0b"], G["e0a"], A["e0a"])
         self._wgrad("e0a", A["x"], G["e0a"])
