@@ -42,6 +42,8 @@ extern "C" {
 #define PTY_FRAUNHOFER 0          /* fft2c with shifts, fftshiftSwitch=False  */
 #define PTY_FRAUNHOFER_SHIFTED 1  /* fft2 ortho, fftshiftSwitch=True          */
 #define PTY_ASP 2                 /* band-limited angular spectrum            */
+#define PTY_FRESNEL 3             /* fft2c(psi * Q), fftshiftSwitch=False (Operators.py:77-152) */
+#define PTY_POLYCHROME_ASP 4      /* ASP with one transfer function per wavelength (Operators.py:458-558) */
 
 /*
  * One reconstruction.  Shapes follow the reference's 6D layout
@@ -58,7 +60,11 @@ extern "C" {
  *   energy  (F) float32 energyAtPos (ExperimentalData.py:248)
  *   tf      (Np, Np) complex64 transfer function in FFT frequency order
  *             (ifftshift applied): the ASP detector TF (Operators.py:861-893)
- *             or the e3PIE slice TF (e3PIE.py:61-69); NULL for Fraunhofer ePIE
+ *             or the e3PIE slice TF (e3PIE.py:61-69); for PTY_POLYCHROME_ASP
+ *             (nlambda, Np, Np), one per wavelength of spectralDensity
+ *             (Operators.py:910-952); for PTY_FRESNEL the quadratic phase Q
+ *             (Np, Np) in natural order (_propagation_kernels.py:16-36);
+ *             NULL for Fraunhofer
  *   err_at_pos (n_iter, F) float64 and err_iter (n_iter) float64 outputs
  *             (getErrorMetrics, BaseEngine.py:819-841)
  * Buffers and momenta (mPIE/multiPIE only) have the object/probe shapes.

@@ -18,6 +18,9 @@ Parity pins (what checks each part; the tests live in ``tests/``):
   ``propagator_fraunhofer.npz``, the unitary round trip, a brute-force DFT.
 * ``aspw_transfer``/ASP: golden ``propagator_asp.npz``; z=0 identity; the
   closed-form band limit; |H|=1 inside the band.
+* Fresnel (``quad_phase``) and polychromeASP (``polychrome_transfer_stack``):
+  goldens ``propagator_fresnel.npz``, ``propagator_polychromeasp.npz``,
+  ``epie_fresnel.npz``, ``epie_polychrome.npz``.
 * ePIE/mPIE engines (single and mixed state): goldens ``epie_fraunhofer``,
   ``epie_asp``, ``epie_mixed_state``, ``mpie_single``, ``mpie_mixed_state``.
 * e3PIE slice chain: golden ``e3pie_multislice``.
@@ -40,7 +43,7 @@ import numpy as np
 
 __all__ = [
     "fft2c", "ifft2c", "circ", "aspw_transfer", "asp_transfer_stack",
-    "propagate", "pixel_positions", "Geometry", "initial_guess",
+    "quad_phase", "polychrome_transfer_stack", "propagate", "pixel_positions", "Geometry", "initial_guess",
     "energy_at_pos", "position_order", "orthogonalize_modes", "OracleRun",
     "nrmse", "GEMMA",
 ]
@@ -113,12 +116,41 @@ def asp_transfer_stack(nosm, npsm, Np, z, wavelength, Lp, nlambda=1):
     return stack
 
 
-def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None):
-    """object2detector / detector2object for the in-scope propagators
-    (Operators.py:27-74 Fraunhofer, :155-230 ASP, dispatch :561-584, :1082-1103)."""
+def quad_phase(z, wavelength, Np, dxp):
+    """exp(i pi / (lambda z) (x^2 + y^2)) on linspace(-Np/2, Np/2, Np) * dxp,
+    complex128 (_propagation_kernels.py:16-36)."""
+    x = np.linspace(-Np / 2, Np / 2, int(Np)) * dxp
+    X, Y = np.meshgrid(x, x)
+    return np.exp(1.0j * np.pi / (wavelength * z) * (X ** 2 + Y ** 2))
+
+
+def polychrome_transfer_stack(nosm, npsm, Np, z, spectral_density, Lp):
+    """The (nlambda, nosm, npsm, 1, Np, Np) stack of per-wavelength ASP
+    transfer functions, centred, complex128 as built (Operators.py:910-952)."""
+    nl = len(spectral_density)
+    stack = np.empty((nl, nosm, npsm, 1, Np, Np), dtype=np.complex128)
+    for il in range(nl):
+        stack[il] = aspw_transfer(z, spectral_density[il], Np, Lp)
+    return stack
+
+
+def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pure=False):
+    """object2detector / detector2object (Operators.py:561-584, dispatch
+    :1082-1103) for Fraunhofer (:27-74), Fresnel (:77-152), ASP (:155-230) and
+    polychromeASP (:458-558).  pure=True keeps complex64 fields complex64
+    (the quadratic phase cast to complex64, as a complex64 GPU path does);
+    the default follows NumPy's promotion with the reference's complex128
+    quadratic phase."""
     kind = kind.lower()
     if kind == "fraunhofer":
         return ifft2c(fields, shifted) if inverse else fft2c(fields, shifted)
+    if kind == "fresnel":
+        Q = quad_phase(geo.zo, geo.wavelength, geo.Np, geo.dxp)
+        if pure and fields.dtype == np.complex64:
+            Q = Q.astype(np.complex64)
+        if inverse:
+            return ifft2c(fields, shifted) * Q.conj()
+        return fft2c(fields * Q, shifted)
     if kind == "asp":
         if shifted:
             raise ValueError("ASP propagator only works with fftshiftswitch == False")
@@ -134,6 +166,21 @@ def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None):
         if fields.dtype == np.complex128:
             tf = tf.astype(np.complex128)
         return ifft2c(fft2c(fields, True) * tf, True)
+    if kind == "polychromeasp":
+        if shifted:
+            raise ValueError("ASP propagatorType works only with fftshiftSwitch = False!")
+        key = ("poly", geo.nosm, geo.npsm, geo.Np, geo.zo, tuple(geo.spectralDensity), geo.Lp)
+        tf = None if tf_cache is None else tf_cache.get(key)
+        if tf is None:
+            # built in complex128 and returned as is on the CPU path (:938-952)
+            tf = polychrome_transfer_stack(geo.nosm, geo.npsm, geo.Np, geo.zo, geo.spectralDensity, geo.Lp)
+            if tf_cache is not None:
+                tf_cache[key] = tf
+        if pure and fields.dtype == np.complex64:
+            tf = tf.astype(np.complex64)
+        if inverse:
+            tf = tf.conj()
+        return ifft2c(fft2c(fields) * tf)
     raise NotImplementedError(f"propagator {kind!r} is outside the oracle's scope")
 
 
@@ -161,8 +208,12 @@ class Geometry:
     nosm: int = 1
     npsm: int = 1
     nslice: int = 1
+    spectralDensity: Optional[np.ndarray] = None
 
     def __post_init__(self):
+        if self.spectralDensity is None:
+            # Reconstruction.py:166-170
+            self.spectralDensity = np.atleast_1d(self.wavelength)
         self.Nd = self.ptychogram.shape[-1]
         self.Ld = self.Nd * self.dxd
         self.dxp = self.wavelength * self.zo / self.Ld
@@ -308,10 +359,12 @@ class OracleRun:
 
     # -- shared pieces ------------------------------------------------------
     def _fwd(self, x):
-        return propagate(x, self.propagator, self.geo, False, self.fftshiftSwitch, self._tf)
+        return propagate(x, self.propagator, self.geo, False, self.fftshiftSwitch, self._tf,
+                         pure=self.precision == "c64pure")
 
     def _bwd(self, x):
-        return propagate(x, self.propagator, self.geo, True, self.fftshiftSwitch, self._tf)
+        return propagate(x, self.propagator, self.geo, True, self.fftshiftSwitch, self._tf,
+                         pure=self.precision == "c64pure")
 
     def _intensity_projection(self, k, esw):
         """BaseEngine.intensityProjection 'standard' branch (:870-1012) with

@@ -24,7 +24,7 @@ import logging
 import numpy as np
 
 from .. import _native
-from ..Operators import asp_tf_fftorder, aspw_transfer_function
+from ..Operators import asp_tf_fftorder, aspw_transfer_function, polychrome_tf_fftorder, quadratic_phase
 
 # switches whose non-default value changes the per-position math or the
 # per-iteration constraints; the GPU engines implement the defaults (and
@@ -71,10 +71,12 @@ class BaseEngine:
         if p.positionOrder not in ("sequential", "random", "NA"):
             raise ValueError("position order not properly set")
         kind = p.propagatorType.lower()
-        if kind not in ("fraunhofer", "asp"):
+        if kind not in ("fraunhofer", "asp", "fresnel", "polychromeasp"):
             raise NotImplementedError(f"propagatorType {p.propagatorType!r} is not implemented by the MI355X engines")
-        if p.propagatorType == "ASP" and p.fftshiftSwitch:
+        if kind in ("asp", "polychromeasp") and p.fftshiftSwitch:
             raise ValueError("ASP propagatorType works only with fftshiftSwitch = False")
+        if kind == "fresnel" and p.fftshiftSwitch:
+            raise NotImplementedError("Fresnel with fftshiftSwitch=True is not implemented by the MI355X engines")
         if self.experimentalData.operationMode != "CPM":
             raise NotImplementedError("only CPM is implemented")
 
@@ -137,15 +139,28 @@ class BaseEngine:
 
     def _transfer_function(self):
         r = self.reconstruction
-        if self.params.propagatorType.lower() == "asp":
+        kind = self.params.propagatorType.lower()
+        if kind == "asp":
             if r.nlambda > 1:
                 raise ValueError("For multi-wavelength, set polychromeASP instead of ASP")
             return asp_tf_fftorder(r.zo, r.wavelength, r.Np, r.Lp)
+        if kind == "polychromeasp":
+            sd = np.atleast_1d(r.spectralDensity)
+            if len(sd) != r.nlambda:
+                raise ValueError(f"polychromeASP: spectralDensity has {len(sd)} wavelengths, nlambda is {r.nlambda}")
+            return polychrome_tf_fftorder(r.zo, sd, r.Np, r.Lp)
+        if kind == "fresnel":
+            return quadratic_phase(r.zo, r.wavelength, r.Np, r.dxp)
         return None
 
     def _propagator_id(self):
-        if self.params.propagatorType.lower() == "asp":
+        kind = self.params.propagatorType.lower()
+        if kind == "asp":
             return _native.PTY_ASP
+        if kind == "polychromeasp":
+            return _native.PTY_POLYCHROME_ASP
+        if kind == "fresnel":
+            return _native.PTY_FRESNEL
         return _native.PTY_FRAUNHOFER_SHIFTED if self.params.fftshiftSwitch else _native.PTY_FRAUNHOFER
 
     # -- device side ------------------------------------------------------------

@@ -2,10 +2,13 @@
 
 ``object2detector`` / ``detector2object`` keep the reference's signature
 ``(fields, params, reconstruction) -> (esw, propagated)`` (Operators.py:561-584)
-and run the Fraunhofer and ASP propagators on the GPU through
-``pty_propagate``.  The ASP transfer function is built on the host in float64
-exactly as the reference builds it (Operators.py:631-684, 861-893) and cast to
-complex64 once; it is cached per geometry like the reference's lru_cache."""
+and run the Fraunhofer, Fresnel, ASP and polychromeASP propagators on the GPU
+through ``pty_propagate`` (Fresnel multiplies by the quadratic phase around a
+Fraunhofer transform, polychromeASP runs one ASP per wavelength).  Transfer
+functions and the quadratic phase are built on the host in float64 exactly as
+the reference builds them (Operators.py:631-684, 861-893, 910-952,
+_propagation_kernels.py:16-36) and cast to complex64 once; they are cached per
+geometry like the reference's lru_cache."""
 
 from functools import lru_cache
 
@@ -14,7 +17,7 @@ import numpy as np
 from . import _native
 from .utils import circ
 
-PROPAGATORS = {"fraunhofer", "asp"}
+PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp"}
 
 
 @lru_cache(maxsize=8)
@@ -53,6 +56,28 @@ def asp_tf_fftorder(z, wavelength, N, L):
     return np.ascontiguousarray(np.fft.ifftshift(aspw_transfer_function(z, wavelength, N, L).astype(np.complex64)))
 
 
+@lru_cache(maxsize=8)
+def _quad_phase_cached(z, wavelength, Np, dxp):
+    x = np.linspace(-Np / 2, Np / 2, int(Np)) * dxp
+    X, Y = np.meshgrid(x, x)
+    out = np.exp(1.0j * np.pi / (wavelength * z) * (X ** 2 + Y ** 2))
+    out.setflags(write=False)
+    return out
+
+
+def quadratic_phase(z, wavelength, Np, dxp):
+    """Fresnel quadratic phase exp(i pi (x^2 + y^2) / (lambda z)) on
+    linspace(-Np/2, Np/2, Np) * dxp, complex64, natural order
+    (_propagation_kernels.py:16-36; the reference keeps it complex128)."""
+    return np.ascontiguousarray(_quad_phase_cached(float(z), float(wavelength), int(Np), float(dxp)).astype(np.complex64))
+
+
+def polychrome_tf_fftorder(z, spectral_density, N, L):
+    """(nlambda, N, N) complex64 transfer functions, one per wavelength of the
+    spectral density, in FFT frequency order (Operators.py:910-952)."""
+    return np.ascontiguousarray(np.stack([asp_tf_fftorder(z, float(w), N, L) for w in np.atleast_1d(spectral_density)]))
+
+
 def _kind(params):
     kind = params.propagatorType.lower()
     if kind not in PROPAGATORS:
@@ -61,10 +86,12 @@ def _kind(params):
 
 
 def _native_kind(params, kind):
-    if kind == "asp":
+    if kind in ("asp", "polychromeasp"):
         if params.fftshiftSwitch:
             raise ValueError("ASP propagator only works with fftshiftswitch == False")
         return _native.PTY_ASP
+    if kind == "fresnel" and params.fftshiftSwitch:
+        raise NotImplementedError("Fresnel with fftshiftSwitch=True is not implemented on the GPU path")
     return _native.PTY_FRAUNHOFER_SHIFTED if params.fftshiftSwitch else _native.PTY_FRAUNHOFER
 
 
@@ -73,19 +100,36 @@ def _propagate(fields, params, reconstruction, inverse):
 
     kind = _kind(params)
     nk = _native_kind(params, kind)
-    if nk == _native.PTY_ASP and reconstruction.nlambda > 1:
+    r = reconstruction
+    if kind == "asp" and r.nlambda > 1:
         raise ValueError("For multi-wavelength, set polychromeASP instead of ASP")
     dev = torch.device("cuda")
     src_numpy = not isinstance(fields, torch.Tensor)
     t = torch.as_tensor(np.asarray(fields, dtype=np.complex64) if src_numpy else fields, device=dev)
     out = t.to(torch.complex64).contiguous().clone()
-    tf = None
-    if nk == _native.PTY_ASP:
-        tf = torch.as_tensor(
-            asp_tf_fftorder(reconstruction.zo, reconstruction.wavelength, reconstruction.Np, reconstruction.Lp),
-            device=dev,
-        )
-    _native.propagate(out, nk, inverse, tf, torch.cuda.current_stream())
+    stream = torch.cuda.current_stream()
+    if kind == "asp":
+        tf = torch.as_tensor(asp_tf_fftorder(r.zo, r.wavelength, r.Np, r.Lp), device=dev)
+        _native.propagate(out, nk, inverse, tf, stream)
+    elif kind == "polychromeasp":
+        # fields (nlambda, ...): wavelength l propagates with its own transfer function
+        tfs = torch.as_tensor(polychrome_tf_fftorder(r.zo, r.spectralDensity, r.Np, r.Lp), device=dev)
+        if out.shape[0] != tfs.shape[0]:
+            raise ValueError(f"polychromeASP: {out.shape[0]} wavelength planes, {tfs.shape[0]} wavelengths")
+        for il in range(tfs.shape[0]):
+            plane = out[il].contiguous()
+            _native.propagate(plane, nk, inverse, tfs[il].contiguous(), stream)
+            out[il] = plane
+    elif kind == "fresnel":
+        Q = torch.as_tensor(quadratic_phase(r.zo, r.wavelength, r.Np, r.dxp), device=dev)
+        if inverse:
+            _native.propagate(out, nk, True, None, stream)
+            out = (out * Q.conj()).contiguous()
+        else:
+            out = (out * Q).contiguous()
+            _native.propagate(out, nk, False, None, stream)
+    else:
+        _native.propagate(out, nk, inverse, None, stream)
     return out.cpu().numpy() if src_numpy else out
 
 

@@ -98,14 +98,21 @@ struct DevInst {
   float2* W;         // (nF, Np, Np)
   float2* ESW;       // e3PIE exit waves (nslice, nF, Np, Np)
   const float* Ik;   // ptychogram in kernel layout
-  const float2* HT;  // transfer function, transposed bit-reversed, / Np^2
+  const float2* HT;  // transfer function(s), bit-reversed layout, / Np^2; one per wavelength
+  const float2* Q;   // Fresnel quadratic phase (Np, Np), natural order, or null
+  int fpl;           // W fields per transfer function (fields of one wavelength)
   unsigned* ctrl;    // control block, CTRL_* layout
   int gsize[8];      // workgroups of this instance in each XCD group
   int ngroups;       // non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
 };
 
-__device__ float2 g_twiddle[6][256];  // exp(-2 pi i j / N), j < N/2, N = 16..512
+__device__ float2 g_twiddle[6][256];
+
+// propagators whose detector transform folds fft2c's shifts into a checkerboard
+__host__ __device__ static inline bool prop_checkerboard(int prop) { return prop == PTY_FRAUNHOFER || prop == PTY_FRESNEL; }
+// propagators of the angular-spectrum family (detector = ifft(fft(.) H))
+__host__ __device__ static inline bool prop_asp(int prop) { return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP; }  // exp(-2 pi i j / N), j < N/2, N = 16..512
 
 __host__ __device__ static int log2_np(int np) {
   switch (np) {
@@ -291,13 +298,14 @@ __device__ __forceinline__ float2 ldg2(const float2* p) {
 // Sums over fields (the modelled intensity, the object and probe increments,
 // the maxima) go through LDS and are added in increasing field order, the C
 // order of NumPy's axis sums, so results are deterministic.
-template <int NP>
+template <int NP, int CWR = 0>
 struct Col {
 #ifndef PTY_COLUMN_WIDTH
-  static constexpr int CW0 = NP >= 256 ? 2 : 1;
+  static constexpr int CWD = NP >= 256 ? 2 : 1;
 #else
-  static constexpr int CW0 = PTY_COLUMN_WIDTH;
+  static constexpr int CWD = PTY_COLUMN_WIDTH;
 #endif
+  static constexpr int CW0 = CWR > 0 ? CWR : CWD;
   static constexpr int CW = CW0 * NP >= 64 ? CW0 : 64 / NP;  // a column spans at most NP lanes
   static constexpr int LPC = 64 / CW;                         // lanes per column
   static constexpr int CL = NP / LPC;                         // values per lane
@@ -381,7 +389,8 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   const float2* __restrict__ O = (const float2*)D.a.object;
   const float2* __restrict__ P = (const float2*)D.a.probe;
   float2* __restrict__ W = D.W;
-  const bool cb = D.a.propagator == PTY_FRAUNHOFER;
+  const bool cb = prop_checkerboard(D.a.propagator);
+  const float2* __restrict__ Qp = D.Q;
   if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
   float pmax = 0.f, omax = 0.f;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
@@ -397,6 +406,8 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float2 e = cmul(of[l], pf[l]);
+      // Fresnel: fft2c(esw * Q) (Operators.py:110)
+      if (Qp) e = cmul(e, ldg2(Qp + (size_t)r * NP + l * LPF + R.t));
       if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
       v[l] = e;
     }
@@ -435,7 +446,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
 // take (group, field) items, groups per round = waves / nFp.
 #define CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)                         \
   {                                                                             \
-    const int ng_ = Col<NP>::NG;                                                \
+    const int ng_ = Col<NP, CWR>::NG;                                                \
     const int cpg_ = (ng_ + (R).G - 1) / (R).G;                                 \
     const int cg0_ = (R).g * cpg_;                                              \
     const int cgn_ = cg0_ < ng_ ? (ng_ - cg0_ < cpg_ ? ng_ - cg0_ : cpg_) : 0; \
@@ -449,53 +460,57 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       const int gi_ = base_ + lg_;                                              \
       const bool gok_ = gi_ < cgn_;                                             \
       const bool act = gok_ && frw < (R).nF;                                    \
-      const int p = (cg0_ + (gok_ ? gi_ : cgn_ - 1)) * Col<NP>::CW + (R).lane % Col<NP>::CW;
+      const int p = (cg0_ + (gok_ ? gi_ : cgn_ - 1)) * Col<NP, CWR>::CW + (R).lane % Col<NP, CWR>::CW;
 #define CITEMS_END \
   }                \
   }
 
-template <int NP>
-__device__ __forceinline__ int col_rho(const Run& R) { return R.lane / Col<NP>::CW; }
+template <int NP, int CWR>
+__device__ __forceinline__ int col_rho(const Run& R) { return R.lane / Col<NP, CWR>::CW; }
 
-template <int NP>
-__device__ __forceinline__ void load_colg(float2 (&v)[Col<NP>::CL], const float2* f, int p, int rho) {
+// registers of one lane in a column pass
+template <int NP, int CWR>
+using ColVals = float2[Col<NP, CWR>::CL];
+
+template <int NP, int CWR>
+__device__ __forceinline__ void load_colg(ColVals<NP, CWR>& v, const float2* f, int p, int rho) {
 #pragma unroll
-  for (int l = 0; l < Col<NP>::CL; ++l) v[l] = cload(f + (size_t)(l * Col<NP>::LPC + rho) * NP + p);
+  for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = cload(f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p);
 }
-template <int NP>
-__device__ __forceinline__ void store_colg(float2* f, const float2 (&v)[Col<NP>::CL], int p, int rho) {
+template <int NP, int CWR>
+__device__ __forceinline__ void store_colg(float2* f, const ColVals<NP, CWR>& v, int p, int rho) {
 #pragma unroll
-  for (int l = 0; l < Col<NP>::CL; ++l) cstore(f + (size_t)(l * Col<NP>::LPC + rho) * NP + p, v[l]);
+  for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p, v[l]);
 }
-template <int NP>
-__device__ __forceinline__ void fft_fwd_colg(float2 (&v)[Col<NP>::CL], const float2* tw, int rho) {
-  fft_fwd_g<NP, Col<NP>::LPC, Col<NP>::CW>(v, tw, rho);
+template <int NP, int CWR>
+__device__ __forceinline__ void fft_fwd_colg(ColVals<NP, CWR>& v, const float2* tw, int rho) {
+  fft_fwd_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, tw, rho);
 }
-template <int NP>
-__device__ __forceinline__ void fft_inv_colg(float2 (&v)[Col<NP>::CL], const float2* tw, int rho) {
-  fft_inv_g<NP, Col<NP>::LPC, Col<NP>::CW>(v, tw, rho);
+template <int NP, int CWR>
+__device__ __forceinline__ void fft_inv_colg(ColVals<NP, CWR>& v, const float2* tw, int rho) {
+  fft_inv_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, tw, rho);
 }
 
 // Fraunhofer detector column pass: forward column transform, unitary scale,
 // intensity constraint with the measured frame, inverse column transform.
 // Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
 // BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
-template <int NP>
+template <int NP, int CWR>
 __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs) {
-  constexpr int CL = Col<NP>::CL, LPC = Col<NP>::LPC;
+  constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
   const float invN = 1.0f / (float)NP;
   const int nF = D.nF;
-  const int rho = col_rho<NP>(R);
+  const int rho = col_rho<NP, CWR>(R);
   float* xs = (float*)R.xa;
   double err = 0.0;
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     float2 v[CL];
-    load_colg<NP>(v, W + f * Np2, p, rho);
-    fft_fwd_colg<NP>(v, R.tw, rho);
+    load_colg<NP, CWR>(v, W + f * Np2, p, rho);
+    fft_fwd_colg<NP, CWR>(v, R.tw, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
       v[l] = cscale(v[l], invN);
@@ -510,8 +525,8 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
       if (ITEM_LEAD(act, frw)) err += (double)fabsf(im - iest);
       v[l] = cscale(v[l], sqrtf(im / (iest + 1e-10f)));
     }
-    fft_inv_colg<NP>(v, R.tw, rho);
-    if (act) store_colg<NP>(W + f * Np2, v, p, rho);
+    fft_inv_colg<NP, CWR>(v, R.tw, rho);
+    if (act) store_colg<NP, CWR>(W + f * Np2, v, p, rho);
     __syncthreads();
   CITEMS_END
   const double s = block_sum(R, err);
@@ -521,29 +536,30 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
 // Column pass of an angular-spectrum propagation: forward column transform,
 // multiply by the (bit-reversed, 1/Np^2-scaled) transfer function or its
 // conjugate, inverse column transform.  Fields are independent.
-template <int NP>
+template <int NP, int CWR>
 __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
-  constexpr int CL = Col<NP>::CL, LPC = Col<NP>::LPC;
+  constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
-  const float2* __restrict__ HT = D.HT;
-  const int rho = col_rho<NP>(R);
+  const int rho = col_rho<NP, CWR>(R);
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     (void)islot;
     (void)lslot;
     (void)frw;
+    // the transfer function of field f's wavelength
+    const float2* __restrict__ HT = D.HT + (size_t)(f / D.fpl) * Np2;
     float2 v[CL];
-    load_colg<NP>(v, W + f * Np2, p, rho);
-    fft_fwd_colg<NP>(v, R.tw, rho);
+    load_colg<NP, CWR>(v, W + f * Np2, p, rho);
+    fft_fwd_colg<NP, CWR>(v, R.tw, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
       float2 h = ldg2(HT + (size_t)(l * LPC + rho) * NP + p);
       if (conj) h = conjf2(h);
       v[l] = cmul(v[l], h);
     }
-    fft_inv_colg<NP>(v, R.tw, rho);
-    if (act) store_colg<NP>(W + f * Np2, v, p, rho);
+    fft_inv_colg<NP, CWR>(v, R.tw, rho);
+    if (act) store_colg<NP, CWR>(W + f * Np2, v, p, rho);
   CITEMS_END
 }
 
@@ -605,7 +621,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   float2* __restrict__ P = (float2*)D.a.probe;
   const float2* __restrict__ W = D.W;
   const int prop = D.a.propagator;
-  const bool cb = prop == PTY_FRAUNHOFER;
+  const bool cb = prop_checkerboard(prop);
+  const float2* __restrict__ Qp = D.Q;
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
   const float Pmax = load_slot(slot + 0), Omax = load_slot(slot + 1);
@@ -631,9 +648,11 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float2 u = d[l];
-      if (prop != PTY_ASP) {
+      if (!prop_asp(prop)) {
         u = cscale(u, invN);
         if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
+        // Fresnel: ifft2c(.) * conj(Q) (Operators.py:150)
+        if (Qp) u = cmul(u, conjf2(ldg2(Qp + prow_off + l * LPF + R.t)));
       }
       const float2 dl = csub(u, cmul(of[l], pf[l]));
       float wo, wp;
@@ -919,44 +938,44 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 // per-position drivers
 // ---------------------------------------------------------------------------
-template <int NP, bool ASP>
+template <int NP, bool ASP, int CWR>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
   const DevInst& D = *R.D;
   pie_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
-    tf_cols<NP>(R, false);
+    tf_cols<NP, CWR>(R, false);
     if (!cluster_sync(R)) return false;
     asp_rows_constraint<NP>(R, k, j, ep);
     if (!cluster_sync(R)) return false;
-    tf_cols<NP>(R, true);
+    tf_cols<NP, CWR>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
-    detector_cols_fraunhofer<NP>(R, k, j, ep, D.fs);
+    detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs);
     if (!cluster_sync(R)) return false;
   }
   pie_rows_update<NP>(R, k, slot);
   return cluster_sync(R);
 }
 
-template <int NP>
+template <int NP, int CWR>
 __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
   const DevInst& D = *R.D;
   const int ns = D.a.nslice;
   e3_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   for (int s = 1; s < ns; ++s) {
-    tf_cols<NP>(R, false);
+    tf_cols<NP, CWR>(R, false);
     if (!cluster_sync(R)) return false;
     e3_rows_forward<NP>(R, k, s, slot);
     if (!cluster_sync(R)) return false;
   }
-  detector_cols_fraunhofer<NP>(R, k, j, ep, 1);
+  detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, 1);
   if (!cluster_sync(R)) return false;
   e3_rows_update<NP>(R, k, ns - 1, slot, true);
   if (!cluster_sync(R)) return false;
   for (int s = ns - 1; s >= 1; --s) {
-    tf_cols<NP>(R, true);
+    tf_cols<NP, CWR>(R, true);
     if (!cluster_sync(R)) return false;
     e3_rows_update<NP>(R, k, s - 1, slot, false);
     if (!cluster_sync(R)) return false;
@@ -967,7 +986,7 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
 // Iterations [it0, it1); the first iteration starts at position jb and the
 // last one stops before position je (full iterations: jb = 0, je >= F).  The
 // error of an iteration is reduced when its last position has run.
-template <int NP, int FAM>
+template <int NP, int FAM, int CWR>
 __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
   const DevInst& D = *R.D;
   const int F = D.a.F;
@@ -987,9 +1006,9 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       unsigned* zslot = D.ctrl + CTRL_SLOTS + ((q + 2u) & 3u) * 32;
       bool ok;
       if constexpr (FAM == FAM_E3PIE)
-        ok = e3pie_position<NP>(R, k, j, slot, zslot, ep);
+        ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
       else
-        ok = pie_position<NP, FAM == FAM_PIE_ASP>(R, k, j, slot, zslot, ep);
+        ok = pie_position<NP, FAM == FAM_PIE_ASP, CWR>(R, k, j, slot, zslot, ep);
       if (!ok) return;
       if (FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j]) {
         momentum_pass<NP>(R);
@@ -1057,7 +1076,7 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   if (bm.x < 0) return;
   Run R;
   init_run<NP, Cfg<NP>::NT>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, &s_fail);
-  run_iterations<NP, FAM>(R, it0, it1, jb, je);
+  run_iterations<NP, FAM, 0>(R, it0, it1, jb, je);
 }
 
 // Batched launch: every instance of one pass family with Np <= 256 runs in
@@ -1067,13 +1086,14 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
 // property of the launch size alone (nblocks <= co-resident capacity).
 #define BATCH_NT 512
 #define BATCH_MAX_NP 256
+#define BATCH_CW 1  // column width of the batched kernel (fewest registers)
 
 template <int NP>
 struct BatchCfg {
   static constexpr int NW = BATCH_NT / 64;
   static constexpr int IPR = NW * Line<NP>::SUB;
   static constexpr int XROW = 2 * IPR * NP;
-  static constexpr int XCOL = NW * NP * Col<NP>::CW / 2;
+  static constexpr int XCOL = NW * NP * Col<NP, BATCH_CW>::CW / 2;
   static constexpr int XBUF = XROW > XCOL ? XROW : XCOL;
 };
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -1082,7 +1102,7 @@ constexpr int BATCH_XBUF =
          BatchCfg<256>::XBUF);
 
 template <int FAM>
-__global__ __launch_bounds__(BATCH_NT) void pty_batch_kernel(const DevInst* __restrict__ insts,
+__global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) void pty_batch_kernel(const DevInst* __restrict__ insts,
                                                              const int2* __restrict__ bmap, int it0, int it1, int jb,
                                                              int je) {
   __shared__ float2 s_tw[256];
@@ -1096,23 +1116,23 @@ __global__ __launch_bounds__(BATCH_NT) void pty_batch_kernel(const DevInst* __re
   switch (D->a.Np) {
     case 16:
       init_run<16, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
-      run_iterations<16, FAM>(R, it0, it1, jb, je);
+      run_iterations<16, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 32:
       init_run<32, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
-      run_iterations<32, FAM>(R, it0, it1, jb, je);
+      run_iterations<32, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 64:
       init_run<64, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
-      run_iterations<64, FAM>(R, it0, it1, jb, je);
+      run_iterations<64, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 128:
       init_run<128, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
-      run_iterations<128, FAM>(R, it0, it1, jb, je);
+      run_iterations<128, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 256:
       init_run<256, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
-      run_iterations<256, FAM>(R, it0, it1, jb, je);
+      run_iterations<256, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
   }
 }
@@ -1330,7 +1350,7 @@ static const void* iterate_kernel_ptr(int np, int fam) {
 
 static int family_of(const pty_instance& a) {
   if (a.engine == PTY_E3PIE) return FAM_E3PIE;
-  return a.propagator == PTY_ASP ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
+  return prop_asp(a.propagator) ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
 }
 
 namespace {
@@ -1393,7 +1413,7 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 int validate(const pty_instance& a, int idx, int nt) {
   if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
   if (a.engine < PTY_EPIE || a.engine > PTY_E3PIE) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
-  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_ASP)
+  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
   if (log2_np(a.Np) < 0)
     return set_err(PTY_EINVAL, "instance %d: Np=%d unsupported (powers of two 16..512)", idx, a.Np);
@@ -1406,6 +1426,10 @@ int validate(const pty_instance& a, int idx, int nt) {
   if (a.propagator == PTY_ASP && a.nlambda > 1)
     return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, set polychromeASP instead of ASP", idx);
   if (a.propagator == PTY_ASP && !a.tf) return set_err(PTY_EINVAL, "instance %d: ASP needs a transfer function", idx);
+  if (a.propagator == PTY_POLYCHROME_ASP && !a.tf)
+    return set_err(PTY_EINVAL, "instance %d: polychromeASP needs one transfer function per wavelength", idx);
+  if (a.propagator == PTY_FRESNEL && !a.tf)
+    return set_err(PTY_EINVAL, "instance %d: Fresnel needs the quadratic phase", idx);
   const int slots = slots_rt(a.Np, nt);
   {
     // a column pass gives every field of a column group its own wave in one
@@ -1422,7 +1446,7 @@ int validate(const pty_instance& a, int idx, int nt) {
   if (a.engine == PTY_E3PIE) {
     if (a.nosm != 1) return set_err(PTY_EINVAL, "instance %d: e3PIE needs nosm == 1", idx);
     if (a.nslice > MAX_SLICES) return set_err(PTY_EINVAL, "instance %d: nslice > %d", idx, MAX_SLICES);
-    if (a.propagator == PTY_ASP)
+    if (a.propagator != PTY_FRAUNHOFER && a.propagator != PTY_FRAUNHOFER_SHIFTED)
       return set_err(PTY_EINVAL, "instance %d: e3PIE supports the Fraunhofer detector propagator only", idx);
     if (!a.tf) return set_err(PTY_EINVAL, "instance %d: e3PIE needs the slice transfer function", idx);
     if (a.nlambda * a.npsm > slots)
@@ -1553,7 +1577,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     d.ctrl = (unsigned*)mem;
     if ((rc = dalloc(p, &mem, (size_t)2 * a.F * G * sizeof(double)))) goto fail;
     d.errpart = (double*)mem;
-    if (a.propagator == PTY_ASP) {
+    if (prop_asp(a.propagator)) {
       d.Ik = a.ptychogram;  // natural order: the constraint runs in the row pass
     } else {
       if ((rc = dalloc(p, &mem, (size_t)a.F * np2 * sizeof(float)))) goto fail;
@@ -1561,9 +1585,16 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       HIP_TRY(hipGetLastError());
       d.Ik = (const float*)mem;
     }
-    if (a.tf) {
-      if ((rc = dalloc(p, &mem, np2 * sizeof(float2)))) goto fail;
-      pty_permute_tf_kernel<<<256, 256, 0, st>>>((const float2*)a.tf, (float2*)mem, np, log2_np(np));
+    d.fpl = d.nF;
+    if (a.propagator == PTY_FRESNEL) {
+      d.Q = (const float2*)a.tf;  // natural order, read by the row passes
+    } else if (a.tf) {
+      const int ntf = a.propagator == PTY_POLYCHROME_ASP ? a.nlambda : 1;
+      if (a.propagator == PTY_POLYCHROME_ASP) d.fpl = d.nF / a.nlambda;
+      if ((rc = dalloc(p, &mem, (size_t)ntf * np2 * sizeof(float2)))) goto fail;
+      for (int t = 0; t < ntf; ++t)
+        pty_permute_tf_kernel<<<256, 256, 0, st>>>((const float2*)a.tf + (size_t)t * np2, (float2*)mem + (size_t)t * np2,
+                                                   np, log2_np(np));
       HIP_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
     }
@@ -1735,7 +1766,7 @@ int pty_plan_use_graph(pty_plan* p, int32_t enable) {
 
 int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count) {
   if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_E3PIE || propagator < PTY_FRAUNHOFER ||
-      propagator > PTY_ASP)
+      propagator > PTY_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "pty_coresident_workgroups: bad arguments");
   pty_instance a;
   memset(&a, 0, sizeof a);

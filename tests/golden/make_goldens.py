@@ -18,14 +18,17 @@ sys.path.insert(0, str(HERE.parent.parent))
 sys.path.insert(0, str(HERE.parent))
 
 import oracle as O  # noqa: E402
-from regression_inputs import ENGINE_CASES, SEED, regression_dataset  # noqa: E402
+from regression_inputs import ENGINE_CASES, SEED, regression_dataset, spectral_density  # noqa: E402
+
+PROPAGATOR_CASES = {"fraunhofer": 1, "asp": 1, "fresnel": 1, "polychromeasp": 3}
 
 
 def engine_case(name):
     eng, prop, (nl, no, npm, ns), iters = ENGINE_CASES[name]
     d = regression_dataset()
     geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"],
-                     d["entrancePupilDiameter"], nlambda=nl, nosm=no, npsm=npm, nslice=ns)
+                     d["entrancePupilDiameter"], nlambda=nl, nosm=no, npsm=npm, nslice=ns,
+                     spectralDensity=spectral_density(d["wavelength"], nl) if nl > 1 else None)
     np.random.seed(SEED)
     obj = O.initial_guess(geo.shape_O, "ones", geo)
     pr = O.initial_guess(geo.shape_P, "circ", geo)
@@ -37,8 +40,10 @@ def engine_case(name):
 
 
 def propagator_case(prop):
+    nl = PROPAGATOR_CASES[prop]
     d = regression_dataset()
-    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"])
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
+                     nlambda=nl, spectralDensity=spectral_density(d["wavelength"], nl) if nl > 1 else None)
     np.random.seed(SEED)
     O.initial_guess(geo.shape_O, "ones", geo)
     probe = O.initial_guess(geo.shape_P, "circ", geo)
@@ -47,13 +52,17 @@ def propagator_case(prop):
     return {"forward": fwd, "back": back}
 
 
-def main():
+def main(only=None):
+    """Write every fixture, or only the named ones (engine case names or
+    propagator_<kind>)."""
     for name in ENGINE_CASES:
-        np.savez_compressed(HERE / f"oracle_{name}.npz", **engine_case(name))
-    for prop in ("fraunhofer", "asp"):
-        np.savez_compressed(HERE / f"oracle_propagator_{prop}.npz", **propagator_case(prop))
+        if only is None or name in only:
+            np.savez_compressed(HERE / f"oracle_{name}.npz", **engine_case(name))
+    for prop in PROPAGATOR_CASES:
+        if only is None or f"propagator_{prop}" in only:
+            np.savez_compressed(HERE / f"oracle_propagator_{prop}.npz", **propagator_case(prop))
     print("written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or None)

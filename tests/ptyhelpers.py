@@ -40,6 +40,9 @@ def build_package(d, engine, propagator, modes, iters, seed, order="sequential",
     if No is not None:
         rec.No = int(No)  # user override after construction (exampleReconstructionCPM.py:79)
     rec.nlambda, rec.nosm, rec.npsm, rec.nslice = nl, no, npm, ns
+    if nl > 1:
+        # test_engine_regression.py:75-77
+        rec.spectralDensity = float(np.atleast_1d(rec.wavelength)[0]) * np.linspace(0.98, 1.02, nl)
     if ns > 1:
         rec.dz = dz
         rec.refrIndex = refrIndex
@@ -69,8 +72,10 @@ def run_package(d, engine, propagator, modes, iters, seed, order="sequential", N
 def run_oracle(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=0.0,
                refrIndex=1.0, init=None, precision="c64", orth=False, coeffs=None):
     nl, no, npm, ns = modes
+    sd = float(d["wavelength"]) * np.linspace(0.98, 1.02, nl) if nl > 1 else None
     geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"],
-                     d.get("entrancePupilDiameter"), No=No, nlambda=nl, nosm=no, npsm=npm, nslice=ns)
+                     d.get("entrancePupilDiameter"), No=No, nlambda=nl, nosm=no, npsm=npm, nslice=ns,
+                     spectralDensity=sd)
     if init is None:
         np.random.seed(seed)
         obj = O.initial_guess(geo.shape_O, "ones", geo)

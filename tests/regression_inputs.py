@@ -24,11 +24,18 @@ REFERENCE_GOLDEN_DIR = Path("/root/reference/tests/regression/data")
 ENGINE_CASES = {
     "epie_fraunhofer": ("ePIE", "Fraunhofer", (1, 1, 1, 1), 3),
     "epie_asp": ("ePIE", "ASP", (1, 1, 1, 1), 3),
+    "epie_fresnel": ("ePIE", "Fresnel", (1, 1, 1, 1), 3),
+    "epie_polychrome": ("ePIE", "polychromeASP", (3, 1, 1, 1), 2),
     "epie_mixed_state": ("ePIE", "Fraunhofer", (1, 2, 3, 1), 3),
     "mpie_single": ("mPIE", "Fraunhofer", (1, 1, 1, 1), 3),
     "mpie_mixed_state": ("mPIE", "Fraunhofer", (1, 2, 3, 1), 3),
     "e3pie_multislice": ("e3PIE", "Fraunhofer", (1, 1, 1, 3), 2),
 }
+
+
+def spectral_density(wavelength, nlambda):
+    """Wavelengths of a polychromatic case (test_engine_regression.py:75-77)."""
+    return float(wavelength) * np.linspace(0.98, 1.02, nlambda)
 
 
 def regression_dataset(nd=ND, grid=GRID, seed=SEED):

@@ -57,3 +57,34 @@ def test_asp_propagator_matches_oracle(n):
     _native.propagate(t, _native.PTY_ASP, False, torch.as_tensor(tf, device="cuda"))
     want = O.ifft2c(O.fft2c(x.astype(np.complex128), True) * tf.astype(np.complex128), True)
     assert rel(t.cpu().numpy(), want) < 2e-6 * np.log2(n)
+
+
+@pytest.mark.parametrize("prop,nl", [("Fresnel", 1), ("polychromeASP", 3)])
+def test_fresnel_and_polychrome_propagators_match_oracle(prop, nl):
+    """Operators.object2detector / detector2object on the GPU against the
+    oracle's complex128 propagation of the same complex64 field (the
+    reference's own CPU path runs these two in complex128)."""
+    import ptylab_amd as P
+    from ptyhelpers import make_data
+    from regression_inputs import SEED, regression_dataset, spectral_density
+
+    d = regression_dataset()
+    data = make_data(d)
+    params = P.Params()
+    params.propagatorType = prop
+    rec = P.Reconstruction(data, params)
+    rec.nlambda = nl
+    if nl > 1:
+        rec.spectralDensity = spectral_density(d["wavelength"], nl)
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
+                     nlambda=nl, spectralDensity=rec.spectralDensity if nl > 1 else None)
+    field = np.asarray(rec.probe)
+    _, fwd = P.Operators.object2detector(field, params, rec)
+    _, back = P.Operators.detector2object(fwd, params, rec)
+    want_f = O.propagate(field.astype(np.complex128), prop, geo)
+    want_b = O.propagate(want_f, prop, geo, inverse=True)
+    ef, eb = rel(fwd, want_f), rel(back, want_b)
+    print(f"{prop}: forward {ef:.2e} back {eb:.2e}", flush=True)
+    assert ef < 1e-5 and eb < 1e-5

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from regression_inputs import ENGINE_CASES, REFERENCE_GOLDEN_DIR, SEED, regression_dataset
+from regression_inputs import ENGINE_CASES, REFERENCE_GOLDEN_DIR, SEED, regression_dataset, spectral_density
 
 needs_ref = pytest.mark.skipif(not REFERENCE_GOLDEN_DIR.exists(), reason="reference tree not mounted")
 
@@ -19,7 +19,8 @@ def _run(name):
     eng, prop, (nl, no, npm, ns), iters = ENGINE_CASES[name]
     d = regression_dataset()
     geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"],
-                     d["entrancePupilDiameter"], nlambda=nl, nosm=no, npsm=npm, nslice=ns)
+                     d["entrancePupilDiameter"], nlambda=nl, nosm=no, npsm=npm, nslice=ns,
+                     spectralDensity=spectral_density(d["wavelength"], nl) if nl > 1 else None)
     np.random.seed(SEED)
     obj = O.initial_guess(geo.shape_O, "ones", geo)
     pr = O.initial_guess(geo.shape_P, "circ", geo)
@@ -40,10 +41,11 @@ def test_oracle_engine_matches_reference_golden(name):
 
 
 @needs_ref
-@pytest.mark.parametrize("prop", ["fraunhofer", "asp"])
-def test_oracle_propagator_matches_reference_golden(prop):
+@pytest.mark.parametrize("prop,nl", [("fraunhofer", 1), ("asp", 1), ("fresnel", 1), ("polychromeasp", 3)])
+def test_oracle_propagator_matches_reference_golden(prop, nl):
     d = regression_dataset()
-    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"])
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
+                     nlambda=nl, spectralDensity=spectral_density(d["wavelength"], nl) if nl > 1 else None)
     np.random.seed(SEED)
     O.initial_guess(geo.shape_O, "ones", geo)
     probe = O.initial_guess(geo.shape_P, "circ", geo)
@@ -52,3 +54,4 @@ def test_oracle_propagator_matches_reference_golden(prop):
     g = np.load(REFERENCE_GOLDEN_DIR / f"propagator_{prop}.npz")
     np.testing.assert_allclose(fwd, g["forward"], rtol=1e-5, atol=1e-7)
     np.testing.assert_allclose(back, g["back"], rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose([np.linalg.norm(fwd.ravel()), np.linalg.norm(back.ravel())], g["error"], rtol=1e-6)
