@@ -32,11 +32,12 @@ extern "C" {
 #define PTY_EDEVICE (-5)      /* HIP runtime error (launch, copy, ...)      */
 #define PTY_ETIMEDOUT (-110)  /* a cluster barrier exceeded its spin limit  */
 
-/* engines (Engines/ePIE.py, mPIE.py, multiPIE.py, e3PIE.py) */
+/* engines (Engines/ePIE.py, mPIE.py, multiPIE.py, e3PIE.py, qNewton.py) */
 #define PTY_EPIE 0
 #define PTY_MPIE 1
 #define PTY_MULTIPIE 2
 #define PTY_E3PIE 3
+#define PTY_QNEWTON 4  /* Engines/qNewton.py; alphaObject/alphaProbe carry regObject/regProbe */
 
 /* propagators (Operators.py:1082-1103 dispatch keys) */
 #define PTY_FRAUNHOFER 0          /* fft2c with shifts, fftshiftSwitch=False  */

@@ -320,8 +320,10 @@ class OracleRun:
     fftshiftSwitch: bool = False
     positionOrder: str = "random"
     numIterations: int = 50
-    betaObject: float = 0.25
-    betaProbe: float = 0.25
+    betaObject: Optional[float] = None  # ePIE/mPIE 0.25, qNewton 1 (qNewton.py:45-53)
+    betaProbe: Optional[float] = None
+    regObject: float = 1.0   # qNewton regularisers (qNewton.py:51-52)
+    regProbe: float = 1.0
     alphaObject: float = 0.1
     alphaProbe: float = 0.1
     feedbackM: float = 0.3   # mPIE feedback / multiPIE betaM
@@ -338,6 +340,10 @@ class OracleRun:
         (complex64 everywhere, including e3PIE's slice transfer function, as
         the reference's GPU path casts it), "c128" (double precision)."""
         g = self.geo
+        if self.betaObject is None:
+            self.betaObject = 1 if self.engine == "qNewton" else 0.25
+        if self.betaProbe is None:
+            self.betaProbe = 1 if self.engine == "qNewton" else 0.25
         c = np.complex128 if self.precision == "c128" else np.complex64
         self.cdtype = c
         self.object = np.asarray(self.object).astype(c)
@@ -448,7 +454,16 @@ class OracleRun:
         esw_upd = self._intensity_projection(k, esw)
         DELTA = esw_upd - esw
         P = self.probe
-        if self.engine == "ePIE":
+        if self.engine == "qNewton":
+            # qNewton.py:106-140: the object step first, then the probe step with
+            # Omax taken over the whole, already updated, object
+            Pmax = np.max(np.sum(np.abs(P), axis=(0, 1, 2, 3)))
+            fracO = np.abs(P) / Pmax * P.conj() / (np.abs(P) ** 2 + self.regObject)
+            self.object[..., sy, sx] = patch + self.betaObject * np.sum(fracO * DELTA, axis=(0, 2, 3), keepdims=True)
+            Omax = np.max(np.sum(np.abs(self.object), axis=(0, 1, 2, 3)))
+            fracP = np.abs(patch) / Omax * patch.conj() / (np.abs(patch) ** 2 + self.regProbe)
+            self.probe = P + self.betaProbe * np.sum(fracP * DELTA, axis=(0, 1, 3), keepdims=True)
+        elif self.engine == "ePIE":
             # ePIE.py:128-160
             fracO = P.conj() / np.max(np.sum(np.abs(P) ** 2, axis=(0, 1, 2, 3)))
             self.object[..., sy, sx] = patch + self.betaObject * np.sum(fracO * DELTA, axis=(0, 2, 3), keepdims=True)
@@ -535,6 +550,8 @@ PINNED = {
     "OracleRun.mPIE": "goldens mpie_single, mpie_mixed_state",
     "OracleRun.multiPIE": "identity with mPIE at nlambda=nosm=1 (no golden exists)",
     "OracleRun.e3PIE": "golden e3pie_multislice",
+    "OracleRun.qNewton": "golden qnewton_single",
+    "propagate(fresnel/polychromeasp)": "goldens propagator_fresnel / propagator_polychromeasp, epie_fresnel, epie_polychrome",
     "orthogonalize_modes": "orthogonality + descending eigenvalues + span invariants",
     "nrmse": "closed form: invariant to a global complex scale; zero at gamma*true",
 }

@@ -60,9 +60,12 @@ class Job:
 
 def family(engine, propagator):
     """Pass family of the batched kernel an instance runs in."""
+    asp = propagator in (_native.PTY_ASP, _native.PTY_POLYCHROME_ASP)
     if engine == _native.PTY_E3PIE:
         return 2
-    return 1 if propagator == _native.PTY_ASP else 0
+    if engine == _native.PTY_QNEWTON:
+        return 4 if asp else 3
+    return 1 if asp else 0
 
 
 def candidate_sizes(shape: Shape, capacity: int):

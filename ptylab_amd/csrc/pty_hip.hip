@@ -59,6 +59,9 @@ using namespace pty;
 #define FAM_PIE_FRAUNHOFER 0
 #define FAM_PIE_ASP 1
 #define FAM_E3PIE 2
+#define FAM_QN_FRAUNHOFER 3  // qNewton: its own instantiations keep the PIE kernels' registers
+#define FAM_QN_ASP 4
+#define N_FAMILIES 5
 
 namespace {
 
@@ -101,18 +104,19 @@ struct DevInst {
   const float2* HT;  // transfer function(s), bit-reversed layout, / Np^2; one per wavelength
   const float2* Q;   // Fresnel quadratic phase (Np, Np), natural order, or null
   int fpl;           // W fields per transfer function (fields of one wavelength)
+  float2* OP;        // qNewton: the object patch before the position's update (nO, Np, Np)
   unsigned* ctrl;    // control block, CTRL_* layout
   int gsize[8];      // workgroups of this instance in each XCD group
   int ngroups;       // non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
 };
 
-__device__ float2 g_twiddle[6][256];
+__device__ float2 g_twiddle[6][256];  // exp(-2 pi i j / N), j < N/2, N = 16..512
 
 // propagators whose detector transform folds fft2c's shifts into a checkerboard
 __host__ __device__ static inline bool prop_checkerboard(int prop) { return prop == PTY_FRAUNHOFER || prop == PTY_FRESNEL; }
 // propagators of the angular-spectrum family (detector = ifft(fft(.) H))
-__host__ __device__ static inline bool prop_asp(int prop) { return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP; }  // exp(-2 pi i j / N), j < N/2, N = 16..512
+__host__ __device__ static inline bool prop_asp(int prop) { return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP; }
 
 __host__ __device__ static int log2_np(int np) {
   switch (np) {
@@ -378,7 +382,7 @@ __device__ __forceinline__ void lds_put(float2* buf, int islot, const float2 (&v
 // probe row, (checkerboard for the centred Fraunhofer transform), forward row
 // transform into W; per pixel sums of |P|^2 over the probe fields and of |O|^2
 // over the object fields give the maxima the updates divide by.
-template <int NP>
+template <int NP, bool QN, bool ASPF>
 __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -391,6 +395,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   float2* __restrict__ W = D.W;
   const bool cb = prop_checkerboard(D.a.propagator);
   const float2* __restrict__ Qp = D.Q;
+  constexpr bool qn = QN;
   if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
   float pmax = 0.f, omax = 0.f;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
@@ -399,15 +404,20 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
     load_row<NP>(of, O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
     load_row<NP>(pf, P + x.fp * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
-    for (int l = 0; l < L; ++l)
-      q[l] = make_float2(act && x.io == 0 ? cabs2(pf[l]) : 0.f, act && x.ip == 0 ? cabs2(of[l]) : 0.f);
+    for (int l = 0; l < L; ++l) {
+      // qNewton.py:114: Pmax = max(sum |P|); the PIE engines take sum |P|^2
+      const float pv = qn ? sqrtf(cabs2(pf[l])) : cabs2(pf[l]);
+      q[l] = make_float2(act && x.io == 0 ? pv : 0.f, act && x.ip == 0 ? cabs2(of[l]) : 0.f);
+    }
     lds_put<NP>(R.xa, islot, q, R.t);
     float2 v[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float2 e = cmul(of[l], pf[l]);
-      // Fresnel: fft2c(esw * Q) (Operators.py:110)
-      if (Qp) e = cmul(e, ldg2(Qp + (size_t)r * NP + l * LPF + R.t));
+      // Fresnel: fft2c(esw * Q) (Operators.py:110); never in ASP-family kernels
+      if constexpr (!ASPF) {
+        if (Qp) e = cmul(e, ldg2(Qp + (size_t)r * NP + l * LPF + R.t));
+      }
       if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
       v[l] = e;
     }
@@ -609,7 +619,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
 // and each object / probe field is written by one owner slot (the field with
 // psm index 0, resp. osm index 0).  Each object pixel of the patch has one
 // owner.
-template <int NP>
+template <int NP, bool QN, bool ASPF>
 __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -619,7 +629,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
   float2* __restrict__ O = (float2*)D.a.object;
   float2* __restrict__ P = (float2*)D.a.probe;
-  const float2* __restrict__ W = D.W;
+  float2* __restrict__ W = D.W;
   const int prop = D.a.propagator;
   const bool cb = prop_checkerboard(prop);
   const float2* __restrict__ Qp = D.Q;
@@ -648,7 +658,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float2 u = d[l];
-      if (!prop_asp(prop)) {
+      if constexpr (!ASPF) {
         u = cscale(u, invN);
         if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
         // Fresnel: ifft2c(.) * conj(Q) (Operators.py:150)
@@ -656,6 +666,18 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       }
       const float2 dl = csub(u, cmul(of[l], pf[l]));
       float wo, wp;
+      if constexpr (QN) {
+        // qNewton.py:116-121: (|P| / Pmax) conj(P) / (|P|^2 + regObject); the
+        // probe step waits for Omax over the updated object (qn_probe_rows)
+        const float ap = sqrtf(cabs2(pf[l]));
+        const float a = ap / Pmax;
+        const float2 b = make_float2(a * pf[l].x, -(a * pf[l].y));
+        const float sc = 1.0f / (cabs2(pf[l]) + D.a.alphaObject);
+        tO[l] = cmul(make_float2(b.x * sc, b.y * sc), dl);
+        tP[l] = make_float2(0.f, 0.f);
+        d[l] = dl;
+        continue;
+      }
       if (engine == PTY_EPIE) {
         wo = invP;
         wp = invO;
@@ -668,24 +690,29 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     }
     lds_put<NP>(R.xa, islot, tO, R.t);
     lds_put<NP>(R.xb, islot, tP, R.t);
+    constexpr bool qn = QN;
+    // qNewton keeps the difference for its probe step; only this item
+    // touches row r of field f in this pass
+    if (qn && act) store_row<NP>(W + f * Np2 + prow_off, d, R.t);
     __syncthreads();
     if (act && x.ip == 0) {
-      // object field x.fo: ePIE sums over (lambda, psm, slice), the m/multi
-      // engines over psm
+      // object field x.fo: ePIE and qNewton sum over (lambda, psm, slice),
+      // the m/multi engines over psm
       float2 nv[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 acc = make_float2(0.f, 0.f);
         for (int g = 0; g < nF; ++g) {
           const FieldIdx y = field_idx(g, no, npm, ns);
-          if (engine == PTY_EPIE ? y.io == x.io : y.fo == x.fo)
+          if (engine == PTY_EPIE || qn ? y.io == x.io : y.fo == x.fo)
             acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
         }
         nv[l] = cadd(of[l], cscale(acc, bO));
       }
+      if (qn) store_row<NP>(D.OP + x.fo * Np2 + prow_off, of, R.t);
       store_row<NP>(O + x.fo * No2 + orow_off, nv, R.t);
     }
-    if (act && x.io == 0) {
+    if (act && x.io == 0 && !qn) {
       // probe field x.fp: ePIE sums over (lambda, osm, slice), mPIE over osm,
       // multiPIE over (lambda, osm)
       float2 nv[L];
@@ -699,6 +726,74 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
           if (sel) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
         }
         nv[l] = cadd(pf[l], cscale(acc, engine == PTY_EPIE ? D.a.betaProbe : stepP));
+      }
+      store_row<NP>(P + x.fp * Np2 + prow_off, nv, R.t);
+    }
+    __syncthreads();
+  ITEMS_END
+}
+
+// qNewton, after the object step: Omax = max over the whole object of the
+// sum over its fields of |O| (qNewton.py:134), each workgroup over its own
+// chunk of pixels, fields added in C order.
+template <int NP>
+__device__ __forceinline__ void qn_omax_pass(Run& R, unsigned* slot) {
+  const DevInst& D = *R.D;
+  const size_t n = (size_t)D.a.No * D.a.No;
+  const float2* __restrict__ O = (const float2*)D.a.object;
+  const size_t chunk = (n + R.G - 1) / R.G;
+  const size_t b = chunk * R.g;
+  const size_t e = b + chunk < n ? b + chunk : n;
+  float m = 0.f;
+  for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
+    float s = 0.f;
+    for (int fo = 0; fo < D.nO; ++fo) s += sqrtf(cabs2(cload(O + fo * n + i)));
+    m = fmaxf(m, s);
+  }
+  m = wave_max(m);
+  if (R.lane == 0) atomic_fmax_slot(slot + 2, m);
+}
+
+// qNewton probe step (qNewton.py:129-140): probe += betaProbe sum over
+// (lambda, osm, slice) of (|O|/Omax) conj(O) / (|O|^2 + regProbe) * DELTA,
+// O the patch before this position's object step.
+template <int NP>
+__device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
+  const int nF = D.nF;
+  const size_t Np2 = (size_t)NP * NP;
+  float2* __restrict__ P = (float2*)D.a.probe;
+  const float Omax = load_slot(slot + 2);
+  const float reg = D.a.alphaProbe, bP = D.a.betaProbe;
+  ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
+    (void)frw;
+    const FieldIdx x = field_idx(f, no, npm, ns);
+    const size_t prow_off = (size_t)r * NP;
+    float2 d[L], op[L], t[L];
+    load_row<NP>(d, D.W + f * Np2 + prow_off, R.t);
+    load_row<NP>(op, D.OP + x.fo * Np2 + prow_off, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const float a = sqrtf(cabs2(op[l])) / Omax;
+      const float2 b = make_float2(a * op[l].x, -(a * op[l].y));
+      const float sc = 1.0f / (cabs2(op[l]) + reg);
+      t[l] = cmul(make_float2(b.x * sc, b.y * sc), d[l]);
+    }
+    lds_put<NP>(R.xb, islot, t, R.t);
+    __syncthreads();
+    if (act && x.io == 0) {
+      float2 pf[L], nv[L];
+      load_row<NP>(pf, P + x.fp * Np2 + prow_off, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int g = 0; g < nF; ++g) {
+          const FieldIdx y = field_idx(g, no, npm, ns);
+          if (y.ip == x.ip) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
+        }
+        nv[l] = cadd(pf[l], cscale(acc, bP));
       }
       store_row<NP>(P + x.fp * Np2 + prow_off, nv, R.t);
     }
@@ -938,10 +1033,10 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 // per-position drivers
 // ---------------------------------------------------------------------------
-template <int NP, bool ASP, int CWR>
+template <int NP, bool ASP, int CWR, bool QN>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
   const DevInst& D = *R.D;
-  pie_rows_first<NP>(R, k, slot, zslot);
+  pie_rows_first<NP, QN, ASP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
     tf_cols<NP, CWR>(R, false);
@@ -954,8 +1049,15 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs);
     if (!cluster_sync(R)) return false;
   }
-  pie_rows_update<NP>(R, k, slot);
-  return cluster_sync(R);
+  pie_rows_update<NP, QN, ASP>(R, k, slot);
+  if (!cluster_sync(R)) return false;
+  if constexpr (QN) {
+    qn_omax_pass<NP>(R, slot);
+    if (!cluster_sync(R)) return false;
+    qn_probe_rows<NP>(R, slot);
+    if (!cluster_sync(R)) return false;
+  }
+  return true;
 }
 
 template <int NP, int CWR>
@@ -1008,7 +1110,8 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       if constexpr (FAM == FAM_E3PIE)
         ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
       else
-        ok = pie_position<NP, FAM == FAM_PIE_ASP, CWR>(R, k, j, slot, zslot, ep);
+        ok = pie_position<NP, FAM == FAM_PIE_ASP || FAM == FAM_QN_ASP, CWR,
+                          FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP>(R, k, j, slot, zslot, ep);
       if (!ok) return;
       if (FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j]) {
         momentum_pass<NP>(R);
@@ -1325,6 +1428,8 @@ static const void* iterate_kernel_fam(int fam) {
     case FAM_PIE_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_PIE_FRAUNHOFER>;
     case FAM_PIE_ASP: return (const void*)pty_iterate_kernel<NP, FAM_PIE_ASP>;
     case FAM_E3PIE: return (const void*)pty_iterate_kernel<NP, FAM_E3PIE>;
+    case FAM_QN_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_QN_FRAUNHOFER>;
+    case FAM_QN_ASP: return (const void*)pty_iterate_kernel<NP, FAM_QN_ASP>;
   }
   return nullptr;
 }
@@ -1336,6 +1441,8 @@ static const void* iterate_kernel_ptr(int np, int fam) {
         case FAM_PIE_FRAUNHOFER: return (const void*)pty_batch_kernel<FAM_PIE_FRAUNHOFER>;
         case FAM_PIE_ASP: return (const void*)pty_batch_kernel<FAM_PIE_ASP>;
         case FAM_E3PIE: return (const void*)pty_batch_kernel<FAM_E3PIE>;
+        case FAM_QN_FRAUNHOFER: return (const void*)pty_batch_kernel<FAM_QN_FRAUNHOFER>;
+        case FAM_QN_ASP: return (const void*)pty_batch_kernel<FAM_QN_ASP>;
       }
       return nullptr;
     case 16: return iterate_kernel_fam<16>(fam);
@@ -1350,6 +1457,7 @@ static const void* iterate_kernel_ptr(int np, int fam) {
 
 static int family_of(const pty_instance& a) {
   if (a.engine == PTY_E3PIE) return FAM_E3PIE;
+  if (a.engine == PTY_QNEWTON) return prop_asp(a.propagator) ? FAM_QN_ASP : FAM_QN_FRAUNHOFER;
   return prop_asp(a.propagator) ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
 }
 
@@ -1412,7 +1520,7 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 
 int validate(const pty_instance& a, int idx, int nt) {
   if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
-  if (a.engine < PTY_EPIE || a.engine > PTY_E3PIE) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
+  if (a.engine < PTY_EPIE || a.engine > PTY_QNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
   if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
   if (log2_np(a.Np) < 0)
@@ -1572,6 +1680,10 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     if (a.engine == PTY_E3PIE) {
       if ((rc = dalloc(p, &mem, (size_t)a.nslice * d.nF * np2 * sizeof(float2)))) goto fail;
       d.ESW = (float2*)mem;
+    }
+    if (a.engine == PTY_QNEWTON) {
+      if ((rc = dalloc(p, &mem, (size_t)d.nO * np2 * sizeof(float2)))) goto fail;
+      d.OP = (float2*)mem;
     }
     if ((rc = dalloc(p, &mem, CTRL_WORDS * sizeof(unsigned)))) goto fail;
     d.ctrl = (unsigned*)mem;
@@ -1765,7 +1877,7 @@ int pty_plan_use_graph(pty_plan* p, int32_t enable) {
 }
 
 int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count) {
-  if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_E3PIE || propagator < PTY_FRAUNHOFER ||
+  if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_QNEWTON || propagator < PTY_FRAUNHOFER ||
       propagator > PTY_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "pty_coresident_workgroups: bad arguments");
   pty_instance a;

@@ -29,6 +29,7 @@ ENGINE_CASES = {
     "epie_mixed_state": ("ePIE", "Fraunhofer", (1, 2, 3, 1), 3),
     "mpie_single": ("mPIE", "Fraunhofer", (1, 1, 1, 1), 3),
     "mpie_mixed_state": ("mPIE", "Fraunhofer", (1, 2, 3, 1), 3),
+    "qnewton_single": ("qNewton", "Fraunhofer", (1, 1, 1, 1), 3),
     "e3pie_multislice": ("e3PIE", "Fraunhofer", (1, 1, 1, 3), 2),
 }
 

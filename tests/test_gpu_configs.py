@@ -159,3 +159,14 @@ def test_epie_stepwise_equals_reconstruct(c0):
     assert steps[0] == (0, 0) and steps[-1] == (1, c0.ptychogram.shape[0] - 1)
     assert np.array_equal(rec.object, full.object) and np.array_equal(rec.probe, full.probe)
     np.testing.assert_allclose(rec.error, full.error, rtol=1e-12)
+
+
+def test_qnewton_on_photon_scaled_inputs(c0):
+    """qNewton (qNewton.py:106-140): object step, then Omax over the whole
+    updated object, then the probe step -- against the oracle."""
+    d = c0.as_dict()
+    for iters, bar in ((1, 1e-5), (10, 1e-4)):
+        gpu = run_package(d, "qNewton", "Fraunhofer", (1, 1, 1, 1), iters, 3, order="random", No=c0.No)
+        ref = run_oracle(d, "qNewton", "Fraunhofer", (1, 1, 1, 1), iters, 3, order="random", No=c0.No)
+        eo, ep, ee = _report(f"C0 qNewton {iters} it", gpu, ref)
+        assert eo <= bar and ep <= bar and ee <= 1e-3
