@@ -192,6 +192,35 @@ int pty_propagate(void* fields, int32_t nfields, int32_t Np, int32_t propagator,
  */
 int pty_fft2(void* fields, int32_t nfields, int32_t Np, int32_t sign, void* hip_stream);
 
+/*
+ * Per-iteration constraints of BaseEngine.applyConstraints (BaseEngine.py:
+ * 1440-1612) on device state, between pty_plan_run calls.
+ */
+
+/* out[0] = sum of |x|^2 over n complex64 values, FP64, deterministic: the
+ * probe power of the power correction (BaseEngine.py:1462-1469, 227-235). */
+int pty_field_power(const void* x, size_t n, double* out, void* hip_stream);
+
+/* x *= target / sqrt(power[0]) (power read on the device): the probe power
+ * correction to ExperimentalData.maxProbePower (ExperimentalData.py:250). */
+int pty_scale_to_power(void* x, size_t n, const double* power, float target, void* hip_stream);
+
+/* x = (1 - aleph) x + aleph x w over nfields (N, N) fields, w an (N, N)
+ * float32 window: the absorbing probe boundary (aleph < 1,
+ * BaseEngine.py:1496-1506) and the probe boundary (aleph = 1, x *= w,
+ * :1493-1494) with the windows of _probeWindow (:237-266). */
+int pty_window_fields(void* x, int32_t nfields, int32_t N, const float* window, float aleph, void* hip_stream);
+
+/* out3 = (sum |x| X, sum |x| Y, sum |x|) over one (N, N) field, FP64: the
+ * centre of mass of comStabilization (BaseEngine.py:1724-1745). */
+int pty_field_moments(const void* x, int32_t N, const float* X, const float* Y, double* out3, void* hip_stream);
+
+/* numpy.roll of nfields (Ny, Nx) fields by (dy, dx) in place, through a
+ * scratch buffer of the same size: the probe/object shift of
+ * comStabilization (BaseEngine.py:1747-1776). */
+int pty_roll_fields(void* x, void* scratch, int32_t nfields, int32_t Ny, int32_t Nx, int32_t dy, int32_t dx,
+                    void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

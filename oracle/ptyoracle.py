@@ -332,6 +332,11 @@ class OracleRun:
     refrIndex: float = 1.0
     orthogonalizationSwitch: bool = False
     orthogonalizationFrequency: int = 10
+    probePowerCorrectionSwitch: bool = False
+    comStabilizationSwitch: object = False   # False or the period in iterations
+    probeBoundary: bool = False
+    absorbingProbeBoundary: bool = False
+    absorbingProbeBoundaryAleph: float = 5e-2
     precision: str = "c64"
     error: List[float] = field(default_factory=list)
 
@@ -389,9 +394,68 @@ class OracleRun:
         self.errorAtPos = self.err_at_pos_sum / (self.energy + 1e-20)
         self.error.append(float(np.sum(self.errorAtPos)))
 
+    @property
+    def max_probe_power(self):
+        """ExperimentalData.maxProbePower (ExperimentalData.py:250)."""
+        return np.sqrt(np.max(np.sum(self.geo.ptychogram, (-1, -2))))
+
+    def _probe_power_correction(self):
+        """BaseEngine.py:1462-1469 (and _initialProbePowerCorrection :227-235)."""
+        P = self.probe
+        self.probe = P / np.sqrt(np.sum(P * P.conj())) * self.max_probe_power
+
+    def _initial_probe_power_correction(self):
+        if self.probePowerCorrectionSwitch:
+            self._probe_power_correction()
+
+    def _probe_window(self):
+        """_probeWindow (BaseEngine.py:237-266): the super-Gaussian absorbing
+        window, replaced by a circle of 1.2 entrance-pupil diameters when
+        probeBoundary is on; float64 (float32 in c64pure)."""
+        g = self.geo
+        w = np.exp(-(((g.Xp ** 2 + g.Yp ** 2) / (2 * (3 / 4 * g.Np * g.dxp / 2.355) ** 2)) ** 10))
+        if self.probeBoundary:
+            w = circ(g.Xp, g.Yp, g.entrancePupilDiameter + g.entrancePupilDiameter * 0.2)
+        if self.precision == "c64pure":
+            w = np.asarray(w, dtype=np.float32)
+        self.probeWindow = w
+
+    def _com_stabilization(self):
+        """comStabilization (BaseEngine.py:1724-1776): centre of mass of |P|
+        of the first mode, last slice; roll probe and object (and the mPIE
+        buffers and momenta) by the rounded offset when it exceeds one pixel."""
+        g = self.geo
+        P2 = abs(self.probe[0, 0, 0, -1])
+        demon = np.sum(P2) * g.dxp
+        xc = int(np.around(np.sum(np.array(g.Xp, np.float32) * P2) / demon))
+        yc = int(np.around(np.sum(np.array(g.Yp, np.float32) * P2) / demon))
+        if xc ** 2 + yc ** 2 > 1:
+            roll = lambda a: np.roll(a, (-yc, -xc), axis=(-2, -1))  # noqa: E731
+            self.probe = roll(self.probe)
+            self.object = roll(self.object)
+            if self.engine in ("mPIE", "multiPIE"):
+                self.probeMomentum, self.probeBuffer = roll(self.probeMomentum), roll(self.probeBuffer)
+                self.objectMomentum, self.objectBuffer = roll(self.objectMomentum), roll(self.objectBuffer)
+        return yc, xc
+
     def _apply_constraints(self, loop):
-        """applyConstraints (BaseEngine.py:1440-1612): only orthogonalization
-        (:1614-1722) is in scope; every other switch is off by default."""
+        """applyConstraints (BaseEngine.py:1440-1612): orthogonalization
+        (:1614-1722), probe power correction (:1462-1469), COM stabilisation
+        (:1486-1491, 1724-1776), probe boundary (:1493-1494), absorbing probe
+        boundary (:1496-1506), in the reference's order."""
+        self._apply_orthogonalization(loop)
+        if self.probePowerCorrectionSwitch:
+            self._probe_power_correction()
+        if self.comStabilizationSwitch is not None and self.comStabilizationSwitch is not False:
+            if loop % int(self.comStabilizationSwitch) == 0:
+                self._com_stabilization()
+        if self.probeBoundary:
+            self.probe *= self.probeWindow
+        if self.absorbingProbeBoundary:
+            a = self.absorbingProbeBoundaryAleph
+            self.probe = (1 - a) * self.probe + a * self.probe * self.probeWindow
+
+    def _apply_orthogonalization(self, loop):
         if self.orthogonalizationSwitch and loop % self.orthogonalizationFrequency == 0:
             g = self.geo
             if g.npsm > 1:
@@ -418,6 +482,8 @@ class OracleRun:
         """Run ``numIterations`` iterations; ``positions_limit`` truncates each
         iteration's position loop (used only to time a bounded CPU sample)."""
         g = self.geo
+        self._initial_probe_power_correction()
+        self._probe_window()
         if self.engine in ("mPIE", "multiPIE"):
             # mPIE.py:104-105
             self.objectBuffer = self.object.copy()

@@ -27,14 +27,15 @@ from .. import _native
 from ..Operators import asp_tf_fftorder, aspw_transfer_function, polychrome_tf_fftorder, quadratic_phase
 
 # switches whose non-default value changes the per-position math or the
-# per-iteration constraints; the GPU engines implement the defaults (and
-# orthogonalization), as the reference raises NotImplementedError for the
-# switches it does not implement (BaseEngine.py:862, :1491)
+# per-iteration constraints; the GPU engines implement the defaults and the
+# constraints orthogonalization, probe power correction, COM stabilisation,
+# probe boundary and absorbing probe boundary; for the others they raise
+# NotImplementedError, as the reference does for the switches it does not
+# implement (BaseEngine.py:862, :1491)
 _UNSUPPORTED_TRUE = [
     "FourierMaskSwitch", "CPSCswitch", "backgroundModeSwitch", "adaptiveDenoisingSwitch",
     "positionCorrectionSwitch", "objectTVregSwitch", "OPRP", "saveMemory", "l2reg",
-    "modulusEnforcedProbeSwitch", "probePowerCorrectionSwitch", "probeSpectralPowerCorrectionSwitch",
-    "comStabilizationSwitch", "PSDestimationSwitch", "probeBoundary", "absorbingProbeBoundary",
+    "modulusEnforcedProbeSwitch", "probeSpectralPowerCorrectionSwitch", "PSDestimationSwitch",
     "probeSmoothenessSwitch", "objectSmoothenessSwitch", "absObjectSwitch", "absProbeSwitch",
     "objectContrastSwitch", "couplingSwitch", "binaryProbeSwitch", "TV_autofocus",
     "map_position_to_z_change", "weigh_probe_updates_by_intensity",
@@ -176,6 +177,10 @@ class BaseEngine:
 
         T["object"] = c64(r.object)
         T["probe"] = c64(r.probe)
+        if self.params.probePowerCorrectionSwitch:
+            # _initialProbePowerCorrection (BaseEngine.py:227-235), before the
+            # mPIE buffers copy the probe (mPIE.py:101-105)
+            self._probe_power_correction(T)
         if tuple(T["object"].shape) != tuple(r.shape_O):
             raise ValueError(f"object shape {tuple(T['object'].shape)} != {r.shape_O}")
         if tuple(T["probe"].shape) != tuple(r.shape_P):
@@ -198,6 +203,12 @@ class BaseEngine:
         tf = self._transfer_function()
         if tf is not None:
             T["tf"] = torch.as_tensor(np.ascontiguousarray(tf, dtype=np.complex64), device=dev)
+        p = self.params
+        if p.probeBoundary or p.absorbingProbeBoundary:
+            T["probeWindow"] = torch.as_tensor(self._probe_window(), device=dev)
+        if p.comStabilizationSwitch is not None and p.comStabilizationSwitch is not False:
+            T["Xp"] = torch.as_tensor(np.ascontiguousarray(r.Xp, dtype=np.float32), device=dev)
+            T["Yp"] = torch.as_tensor(np.ascontiguousarray(r.Yp, dtype=np.float32), device=dev)
         return T
 
     def _instance(self, T, n_iter, momentum):
@@ -285,9 +296,92 @@ class BaseEngine:
         self._plan = None
 
     # -- per-iteration constraints (BaseEngine.applyConstraints :1440-1612) -----
+    def _has_constraints(self):
+        p = self.params
+        return bool(p.orthogonalizationSwitch or p.probePowerCorrectionSwitch or p.probeBoundary
+                    or p.absorbingProbeBoundary
+                    or (p.comStabilizationSwitch is not None and p.comStabilizationSwitch is not False))
+
     def applyConstraints(self, loop):
-        if self.params.orthogonalizationSwitch and loop % self.params.orthogonalizationFrequency == 0:
+        """In the reference's order: orthogonalization, probe power
+        correction, COM stabilisation, probe boundary, absorbing probe
+        boundary (BaseEngine.py:1456-1506)."""
+        p = self.params
+        if p.orthogonalizationSwitch and loop % p.orthogonalizationFrequency == 0:
             self.orthogonalization()
+        if p.probePowerCorrectionSwitch:
+            self._probe_power_correction(self._T)
+        if p.comStabilizationSwitch is not None and p.comStabilizationSwitch is not False:
+            if loop % int(p.comStabilizationSwitch) == 0:
+                self.comStabilization()
+        if p.probeBoundary:
+            self._window_probe(1.0)
+        if p.absorbingProbeBoundary:
+            self._window_probe(float(p.absorbingProbeBoundaryAleph))
+
+    def _probe_window(self):
+        """_probeWindow (BaseEngine.py:237-266) in float64, cast to float32:
+        a super-Gaussian of FWHM 3/4 of the probe window (order 10), replaced
+        by a disc of 1.2 entrance-pupil diameters when probeBoundary is on."""
+        r = self.reconstruction
+        Xp, Yp = r.Xp, r.Yp
+        w = np.exp(-(((Xp ** 2 + Yp ** 2) / (2 * (3 / 4 * r.Np * r.dxp / 2.355) ** 2)) ** 10))
+        if self.params.probeBoundary:
+            d = self.experimentalData.entrancePupilDiameter
+            w = (Xp ** 2 + Yp ** 2) < ((d + d * 0.2) / 2) ** 2
+        return np.ascontiguousarray(w, dtype=np.float32)
+
+    def _probe_power_correction(self, T):
+        """probe / sqrt(sum |probe|^2) * maxProbePower (BaseEngine.py:
+        1462-1469) with the sum as an FP64 reduction on the GPU."""
+        import torch
+
+        probe = T["probe"]
+        power = torch.empty(1, dtype=torch.float64, device=probe.device)
+        lib, h = _native.load(), _native._stream_handle(torch.cuda.current_stream())
+        n = probe.numel()
+        _native.check(lib.pty_field_power(probe.data_ptr(), n, power.data_ptr(), h))
+        _native.check(lib.pty_scale_to_power(probe.data_ptr(), n, power.data_ptr(),
+                                             float(self.experimentalData.maxProbePower), h))
+
+    def _window_probe(self, aleph):
+        import torch
+
+        T, r = self._T, self.reconstruction
+        probe = T["probe"]
+        nf = probe.numel() // (r.Np * r.Np)
+        _native.check(_native.load().pty_window_fields(probe.data_ptr(), nf, int(r.Np), T["probeWindow"].data_ptr(),
+                                                       aleph, _native._stream_handle(torch.cuda.current_stream())))
+
+    def comStabilization(self):
+        """Centre of mass of |P| of mode 0, last slice (BaseEngine.py:
+        1724-1745, FP64 moments on the GPU); when it lies more than one pixel
+        off centre, roll probe and object -- and the mPIE/multiPIE buffers and
+        momenta -- back by the rounded offset (:1747-1776)."""
+        import torch
+
+        T, r = self._T, self.reconstruction
+        lib, h = _native.load(), _native._stream_handle(torch.cuda.current_stream())
+        mom = torch.empty(3, dtype=torch.float64, device=T["probe"].device)
+        p0 = T["probe"][0, 0, 0, -1].contiguous()
+        _native.check(lib.pty_field_moments(p0.data_ptr(), int(r.Np), T["Xp"].data_ptr(), T["Yp"].data_ptr(),
+                                            mom.data_ptr(), h))
+        sx, sy, s0 = (float(v) for v in mom.cpu().numpy())
+        demon = s0 * r.dxp
+        xc, yc = int(np.around(sx / demon)), int(np.around(sy / demon))
+        if xc ** 2 + yc ** 2 <= 1:
+            return 0, 0
+        keys = ["probe", "object"]
+        if self._momentum_enabled():
+            keys += ["probeMomentum", "probeBuffer", "objectMomentum", "objectBuffer"]
+        for k in keys:
+            x = T[k]
+            n = x.shape[-1]
+            scratch = torch.empty_like(x)
+            _native.check(lib.pty_roll_fields(x.data_ptr(), scratch.data_ptr(), x.numel() // (n * n), n, n,
+                                              -yc, -xc, h))
+        torch.cuda.current_stream().synchronize()
+        return yc, xc
 
     def orthogonalization(self):
         """'snapShots' orthogonalisation of the probe (npsm > 1) or object
@@ -340,7 +434,7 @@ class BaseEngine:
         n_iter = int(self.numIterations)
         done = 0
         try:
-            orth = self.params.orthogonalizationSwitch
+            orth = self._has_constraints()
             if not orth:
                 # no host work between iterations: one launch per iteration,
                 # no synchronisation until the end

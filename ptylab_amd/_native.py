@@ -34,6 +34,11 @@ EXPORTED = (
     "pty_plan_destroy",
     "pty_propagate",
     "pty_fft2",
+    "pty_field_power",
+    "pty_scale_to_power",
+    "pty_window_fields",
+    "pty_field_moments",
+    "pty_roll_fields",
 )
 
 
@@ -120,6 +125,12 @@ def load(path: os.PathLike | None = None):
     lib.pty_plan_destroy.argtypes = [vp]
     lib.pty_propagate.argtypes = [vp, i32, i32, i32, i32, vp, vp]
     lib.pty_fft2.argtypes = [vp, i32, i32, i32, vp]
+    sz, f32 = ctypes.c_size_t, ctypes.c_float
+    lib.pty_field_power.argtypes = [vp, sz, vp, vp]
+    lib.pty_scale_to_power.argtypes = [vp, sz, vp, f32, vp]
+    lib.pty_window_fields.argtypes = [vp, i32, i32, vp, f32, vp]
+    lib.pty_field_moments.argtypes = [vp, i32, vp, vp, vp, vp]
+    lib.pty_roll_fields.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp]
     for name in EXPORTED[2:]:
         getattr(lib, name).restype = ctypes.c_int
     if lib.pty_abi_version() != PTY_ABI_VERSION:

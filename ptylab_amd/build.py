@@ -6,9 +6,9 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 REPO = HERE.parent
-SRC = HERE / "csrc" / "pty_hip.hip"
+SRC = [HERE / "csrc" / "pty_hip.hip", HERE / "csrc" / "pty_constraints.hip"]
 OUT = HERE / "libptyhip.so"
-NET_SRC = HERE / "csrc" / "pty_net.hip"
+NET_SRC = [HERE / "csrc" / "pty_net.hip"]
 NET_OUT = HERE / "libptynet.so"
 
 HIPCC_FLAGS = [
@@ -26,13 +26,13 @@ HIPCC_FLAGS = [
 ]
 
 
-def _compile(src, out, verbose):
+def _compile(srcs, out, verbose):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = out.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, f"-I{REPO / 'include'}", "-o", str(tmp), str(src)]
+    cmd = [hipcc, *HIPCC_FLAGS, f"-I{REPO / 'include'}", "-o", str(tmp), *[str(s) for s in srcs]]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src.name} ({res.returncode}):\n{res.stderr[-4000:]}")
+        raise RuntimeError(f"hipcc failed on {out.name} ({res.returncode}):\n{res.stderr[-4000:]}")
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")

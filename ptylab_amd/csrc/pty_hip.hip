@@ -1605,6 +1605,9 @@ static int launch_fft2(float2* data, int nf, int sign, hipStream_t st) {
 
 extern "C" {
 
+int pty_internal_set_error(int code, const char* msg) __attribute__((visibility("hidden")));
+int pty_internal_set_error(int code, const char* msg) { return set_err(code, "%s", msg); }
+
 int pty_abi_version(void) { return PTY_ABI_VERSION; }
 
 const char* pty_last_error(void) { return g_err.c_str(); }

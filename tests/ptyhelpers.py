@@ -28,14 +28,17 @@ def make_data(d):
 
 
 def build_package(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=None,
-                  refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None):
-    """Engine set up as the reference's scripts do, not yet reconstructed."""
+                  refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None, switches=None):
+    """Engine set up as the reference's scripts do, not yet reconstructed.
+    ``switches``: further Params attributes (constraint switches)."""
     nl, no, npm, ns = modes
     data = make_data(d)
     params = P.Params()
     params.propagatorType = propagator
     params.positionOrder = order
     params.orthogonalizationSwitch = orth
+    for k, v in (switches or {}).items():
+        setattr(params, k, v)
     rec = P.Reconstruction(data, params)
     if No is not None:
         rec.No = int(No)  # user override after construction (exampleReconstructionCPM.py:79)
@@ -61,16 +64,17 @@ def build_package(d, engine, propagator, modes, iters, seed, order="sequential",
 
 
 def run_package(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=None,
-                refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None):
+                refrIndex=1.0, init=None, workgroups=0, orth=False, coeffs=None, switches=None):
     eng = build_package(d, engine, propagator, modes, iters, seed, order=order, No=No, dz=dz,
-                        refrIndex=refrIndex, init=init, workgroups=workgroups, orth=orth, coeffs=coeffs)
+                        refrIndex=refrIndex, init=init, workgroups=workgroups, orth=orth, coeffs=coeffs,
+                        switches=switches)
     np.random.seed(seed)
     eng.reconstruct()
     return eng.reconstruction
 
 
 def run_oracle(d, engine, propagator, modes, iters, seed, order="sequential", No=None, dz=0.0,
-               refrIndex=1.0, init=None, precision="c64", orth=False, coeffs=None):
+               refrIndex=1.0, init=None, precision="c64", orth=False, coeffs=None, switches=None):
     nl, no, npm, ns = modes
     sd = float(d["wavelength"]) * np.linspace(0.98, 1.02, nl) if nl > 1 else None
     geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"],
@@ -83,6 +87,7 @@ def run_oracle(d, engine, propagator, modes, iters, seed, order="sequential", No
     else:
         obj, pr = init
     kw = dict(coeffs or {})
+    kw.update(switches or {})
     run = O.OracleRun(geo, obj.copy(), pr.copy(), engine=engine, propagator=propagator, positionOrder=order,
                       numIterations=iters, dz=dz or 0.0, refrIndex=refrIndex, precision=precision,
                       orthogonalizationSwitch=orth, **kw)

@@ -170,3 +170,36 @@ def test_qnewton_on_photon_scaled_inputs(c0):
         ref = run_oracle(d, "qNewton", "Fraunhofer", (1, 1, 1, 1), iters, 3, order="random", No=c0.No)
         eo, ep, ee = _report(f"C0 qNewton {iters} it", gpu, ref)
         assert eo <= bar and ep <= bar and ee <= 1e-3
+
+
+@pytest.mark.parametrize("engine", ["ePIE", "mPIE"])
+def test_constraints_against_oracle(c0, engine):
+    """Probe power correction (FP64 reduction on the GPU), COM stabilisation
+    every 2 iterations from an off-centre probe (so the probe, object and
+    mPIE buffers are rolled), absorbing probe boundary -- against the oracle
+    with the float32 window of the GPU path (c64pure)."""
+    import oracle as O
+
+    d = c0.as_dict()
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
+                     No=c0.No)
+    np.random.seed(3)
+    obj = O.initial_guess(geo.shape_O, "ones", geo)
+    pr = np.roll(O.initial_guess(geo.shape_P, "circ", geo), (6, -5), axis=(-2, -1))
+    sw = dict(probePowerCorrectionSwitch=True, comStabilizationSwitch=2, absorbingProbeBoundary=True)
+    gpu = run_package(d, engine, "Fraunhofer", (1, 1, 1, 1), 6, 3, order="random", No=c0.No, init=(obj, pr),
+                      switches=sw)
+    ref = run_oracle(d, engine, "Fraunhofer", (1, 1, 1, 1), 6, 3, order="random", No=c0.No, init=(obj, pr),
+                     precision="c64pure", switches=sw)
+    eo, ep, ee = _report(f"C0 {engine} constraints", gpu, ref)
+    assert eo <= 1e-4 and ep <= 1e-4 and ee <= 1e-3
+
+
+def test_hard_probe_boundary_against_oracle(c0):
+    d = c0.as_dict()
+    sw = dict(probeBoundary=True)
+    gpu = run_package(d, "ePIE", "Fraunhofer", (1, 1, 1, 1), 3, 3, order="random", No=c0.No, switches=sw)
+    ref = run_oracle(d, "ePIE", "Fraunhofer", (1, 1, 1, 1), 3, 3, order="random", No=c0.No, precision="c64pure",
+                     switches=sw)
+    eo, ep, ee = _report("C0 ePIE probe boundary", gpu, ref)
+    assert eo <= 1e-5 and ep <= 1e-5

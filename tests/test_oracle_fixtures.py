@@ -114,3 +114,47 @@ def test_nrmse_closed_form():
     assert O.nrmse(t * (0.3 - 2j), t) < 1e-12
     noise = 1e-3 * (rng.standard_normal((20, 20)))
     assert 1e-5 < O.nrmse(t + noise, t) < 1e-2
+
+
+def _c0_run(**kw):
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=64, No=256, F=30, seed=0)
+    geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+    np.random.seed(1)
+    obj = O.initial_guess(geo.shape_O, "ones", geo)
+    pr = O.initial_guess(geo.shape_P, "circ", geo)
+    return O.OracleRun(geo, obj, pr, engine="ePIE", numIterations=1, **kw), geo
+
+
+def test_probe_power_correction_reaches_the_measured_power():
+    run, geo = _c0_run(probePowerCorrectionSwitch=True)
+    run._initial_probe_power_correction()
+    p = np.sum(np.abs(run.probe.astype(np.complex128)) ** 2)
+    assert p == pytest.approx(float(run.max_probe_power) ** 2, rel=1e-5)
+
+
+def test_com_stabilization_finds_and_undoes_a_known_shift():
+    run, geo = _c0_run(comStabilizationSwitch=1)
+    g = np.exp(-((geo.Xp / (8 * geo.dxp)) ** 2 + (geo.Yp / (8 * geo.dxp)) ** 2)).astype(np.complex64)
+    shifted = np.roll(g, (5, -7), axis=(0, 1))
+    run.probe[0, 0, 0, 0] = shifted
+    obj0 = run.object.copy()
+    yc, xc = run._com_stabilization()
+    assert (yc, xc) == (5, -7)
+    np.testing.assert_array_equal(run.probe[0, 0, 0, 0], g)
+    np.testing.assert_array_equal(run.object, np.roll(obj0, (-5, 7), axis=(-2, -1)))
+
+
+def test_probe_windows():
+    run, geo = _c0_run(probeBoundary=True)
+    run._probe_window()
+    d = geo.entrancePupilDiameter * 1.2
+    np.testing.assert_array_equal(run.probeWindow, geo.Xp ** 2 + geo.Yp ** 2 < (d / 2) ** 2)
+    run, geo = _c0_run(absorbingProbeBoundary=True)
+    run._probe_window()
+    w = run.probeWindow
+    assert w[geo.Np // 2, geo.Np // 2] == pytest.approx(1.0) and w[0, 0] < 1e-6
+    # super-Gaussian of order 10: flat to 1e-3 inside half its width
+    r = np.sqrt(geo.Xp ** 2 + geo.Yp ** 2) / (3 / 4 * geo.Np * geo.dxp / 2.355)
+    assert np.all(w[r < 0.5] > 1 - 1e-3)
