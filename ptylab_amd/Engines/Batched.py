@@ -33,6 +33,7 @@ class Batched:
         self.graph = graph
         self.tuner = tuner
         self.last_workgroups = None
+        self.last_plan_workgroups = None  # active workgroups of the last plan
 
     def reconstruct(self):
         import torch
@@ -54,6 +55,7 @@ class Batched:
         self.last_workgroups = G
         plan = _native.Plan(insts, [s[1] for s in staged], torch.cuda.current_stream())
         plan.use_graph(self.graph)
+        self.last_plan_workgroups = plan.workgroups()
         n_iter = [int(e.numIterations) for e in self.engines]
         done = 0
         try:

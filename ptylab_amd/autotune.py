@@ -4,8 +4,11 @@ A batched plan runs many independent reconstructions in one persistent
 launch per iteration; instance i runs on a cluster of G_i workgroups that
 splits every line and column pass of a position G_i ways.  Larger clusters
 cut an instance's time per position but cost more workgroups and more
-barrier traffic, and the whole launch must stay co-resident
-(sum G_i <= pty_coresident_workgroups).  An iteration of the batch lasts as
+barrier traffic, and the whole launch must stay co-resident: each cluster
+runs whole on one of the N_XCD XCDs, so the clusters must pack into N_XCD
+bins of pty_coresident_workgroups / N_XCD workgroups each, in the placement
+the plan makes (largest first, each to the least-loaded XCD).  An iteration
+of the batch lasts as
 long as its slowest instance, so the tuner
 
 1. measures the time per position t(G) of the batched kernel for every
@@ -33,6 +36,7 @@ import numpy as np
 from . import _native
 
 CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
+N_XCD = 8  # XCDs of the device; a batched cluster runs whole on one of them
 BATCH_MAX_NP = 256  # largest probe size of the batched kernel
 _DEFAULT_CACHE = Path(os.environ.get("PTYLAB_AMD_TUNE_CACHE", Path.home() / ".cache" / "ptylab_amd" / "tune.json"))
 
@@ -73,19 +77,33 @@ def candidate_sizes(shape: Shape, capacity: int):
     return [g for g in CANDIDATES if g <= capacity and shape.Np % g == 0]
 
 
-def assign(jobs, cost, capacity):
+def packs(G, bins, per_bin):
+    """Whether clusters of sizes G fit the plan's placement: stable
+    largest-first order, each cluster to the least-loaded bin (lowest index
+    on ties), no bin above per_bin (pty_plan_create's XCD packing)."""
+    used = [0] * bins
+    for i in sorted(range(len(G)), key=lambda i: -G[i]):
+        x = min(range(bins), key=lambda y: used[y])
+        if used[x] + G[i] > per_bin:
+            return False
+        used[x] += G[i]
+    return True
+
+
+def assign(jobs, cost, capacity, bins=1):
     """Cluster sizes minimising the batch makespan.
 
     ``cost(shape, G)`` is the time per position (any unit); ``capacity`` the
-    workgroups that fit at once.  Returns a list of cluster sizes, one per
-    job, with sum <= capacity (ValueError if even all-ones does not fit).
+    workgroups that fit at once, split into ``bins`` equal bins that each
+    cluster must fit whole.  Returns a list of cluster sizes, one per job,
+    that pack into the bins (ValueError if even all-ones does not).
     """
     n = len(jobs)
-    if n > capacity:
+    per_bin = capacity // bins
+    if not packs([1] * n, bins, per_bin):
         raise ValueError(f"{n} instances need at least {n} workgroups; {capacity} fit on the device")
     G = [1] * n
     T = [j.n_iter * j.F * cost(j.shape, 1) for j in jobs]
-    used = n
     blocked = [False] * n
     while True:
         order = sorted((i for i in range(n) if not blocked[i]), key=lambda i: -T[i])
@@ -95,14 +113,13 @@ def assign(jobs, cost, capacity):
         if T[i] < max(T):
             break  # the longest instance cannot be shortened further
         g2 = 2 * G[i]
-        if g2 not in candidate_sizes(jobs[i].shape, capacity) or used + G[i] > capacity:
+        if g2 not in candidate_sizes(jobs[i].shape, per_bin) or not packs(G[:i] + [g2] + G[i + 1:], bins, per_bin):
             blocked[i] = True
             continue
         t2 = jobs[i].n_iter * jobs[i].F * cost(jobs[i].shape, g2)
         if t2 >= T[i]:
             blocked[i] = True
             continue
-        used += G[i]
         G[i], T[i] = g2, t2
     return G
 
@@ -236,7 +253,7 @@ def tune(instances, table: CostTable | None = None):
         a = instances[idx[0]]
         cap = _native.coresident_workgroups(np_ or BATCH_MAX_NP, a.engine, a.propagator, np_ == 0)
         jobs = [Job(shape_of(instances[i]), int(instances[i].F), int(instances[i].n_iter)) for i in idx]
-        G = assign(jobs, table, cap)
+        G = assign(jobs, table, cap, N_XCD if np_ == 0 else 1)
         for i, g in zip(idx, G):
             out[i] = g
     return out

@@ -34,6 +34,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -50,10 +51,13 @@ using namespace pty;
 // control block of an instance (32-bit words; each counter on its own 128-byte line)
 #define CTRL_TOP 0
 #define CTRL_FAIL 1
+#define CTRL_PLACE 2       // set when a workgroup of an XCD-local cluster ran on another XCD
 #define CTRL_SLOTS 32      // 4 ring slots x 32 words of per-position maxima
 #define CTRL_GROUP 160     // 8 XCD-group arrival counters, 32 words apart
 #define CTRL_RELEASE 416   // published barrier generation
 #define CTRL_WORDS 448
+
+#define N_XCD 8  // XCDs of an MI355X (gfx950), each with its own L2
 
 // pass families of the persistent kernel
 #define FAM_PIE_FRAUNHOFER 0
@@ -146,6 +150,8 @@ struct Run {
   float2* xa;  // LDS exchange buffers, one line-field slot per item
   float2* xb;
   int* fail;
+  bool loc;  // every workgroup of the cluster runs on one XCD: exchange through its L2
+  int grp;   // arrival-counter group of this workgroup in the cluster barrier
 };
 
 // ---------------------------------------------------------------------------
@@ -211,7 +217,7 @@ __device__ __forceinline__ bool cluster_sync(Run& R) {
   if (threadIdx.x == 0) {
     const DevInst& D = *R.D;
     unsigned* ctrl = D.ctrl;
-    const int grp = blockIdx.x & 7;
+    const int grp = R.grp;
     unsigned* gctr = ctrl + CTRL_GROUP + 32 * grp;
     unsigned* top = ctrl + CTRL_TOP;
     unsigned* rel = ctrl + CTRL_RELEASE;
@@ -257,6 +263,56 @@ __device__ __forceinline__ void cstoreu(unsigned* p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// XCD-local accessors.  When every workgroup of a cluster runs on one XCD,
+// cross-workgroup data need only be coherent in that XCD's L2: stores are
+// plain (the CU's L1 is write-through) and loads are buffer loads with the
+// sc0 bit, which miss in L1 and are served by L2.  The agent-scope forms
+// above instead miss in L2 as well.  The buffer resource is built from the
+// first active lane's address minus 2^31, so the 32-bit offsets of one
+// wave's accesses reach 2 GiB either side of it (a wave's accesses of a pass
+// lie within one field).
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+struct LocalRef {
+  __amdgpu_buffer_rsrc_t r;
+  uint32_t off;  // this lane's byte offset from the resource base
+};
+__device__ __forceinline__ LocalRef local_ref(const void* p) {
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  uint64_t base = ((uint64_t)hi << 32) | lo;
+  base = base >= 0x80000000ull ? base - 0x80000000ull : 0ull;
+  return {__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, -1, 0x00020000), (uint32_t)(a - base)};
+}
+__device__ __forceinline__ float2 lload(const LocalRef& L, uint32_t byte_off) {
+  const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off + byte_off, 0, 1);
+  return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+}
+__device__ __forceinline__ float2 cload(const Run& R, const float2* p) {
+  if (R.loc) return lload(local_ref(p), 0u);
+  return cload(p);
+}
+__device__ __forceinline__ void cstore(const Run& R, float2* p, float2 v) {
+  if (R.loc)
+    *(__attribute__((address_space(1))) uint64_t*)p = __builtin_bit_cast(uint64_t, v);
+  else
+    cstore(p, v);
+}
+__device__ __forceinline__ double cloadd(const Run& R, const double* p) {
+  if (R.loc) {
+    const LocalRef L = local_ref(p);
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off, 0, 1);
+    return __builtin_bit_cast(double, w);
+  }
+  return cloadd(p);
+}
+__device__ __forceinline__ void cstored(const Run& R, double* p, double v) {
+  if (R.loc)
+    *(__attribute__((address_space(1))) double*)p = v;
+  else
+    cstored(p, v);
+}
+
 // ---------------------------------------------------------------------------
 // line helpers
 // ---------------------------------------------------------------------------
@@ -265,6 +321,23 @@ __device__ __forceinline__ void cstoreu(unsigned* p, unsigned v) {
 // memory instructions.
 #define GAS __attribute__((address_space(1)))
 
+template <int NP>
+__device__ __forceinline__ void load_row(const Run& R, float2 (&v)[Line<NP>::L], const float2* row, int t) {
+  if (R.loc) {
+    const LocalRef L = local_ref(row + t);
+#pragma unroll
+    for (int l = 0; l < Line<NP>::L; ++l) v[l] = lload(L, (uint32_t)(l * Line<NP>::LPF * 8));
+    return;
+  }
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = cload(row + l * Line<NP>::LPF + t);
+}
+template <int NP>
+__device__ __forceinline__ void store_row(const Run& R, float2* row, const float2 (&v)[Line<NP>::L], int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) cstore(R, row + l * Line<NP>::LPF + t, v[l]);
+}
+// device-coherent forms for the standalone propagation kernels
 template <int NP>
 __device__ __forceinline__ void load_row(float2 (&v)[Line<NP>::L], const float2* row, int t) {
 #pragma unroll
@@ -401,8 +474,8 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const FieldIdx x = field_idx(f, no, npm, ns);
     float2 of[L], pf[L], q[L];
-    load_row<NP>(of, O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
-    load_row<NP>(pf, P + x.fp * Np2 + (size_t)r * NP, R.t);
+    load_row<NP>(R, of, O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
+    load_row<NP>(R, pf, P + x.fp * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       // qNewton.py:114: Pmax = max(sum |P|); the PIE engines take sum |P|^2
@@ -422,7 +495,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       v[l] = e;
     }
     fft_fwd<NP>(v, R.tw, R.t);
-    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
 #pragma unroll
@@ -483,14 +556,20 @@ template <int NP, int CWR>
 using ColVals = float2[Col<NP, CWR>::CL];
 
 template <int NP, int CWR>
-__device__ __forceinline__ void load_colg(ColVals<NP, CWR>& v, const float2* f, int p, int rho) {
+__device__ __forceinline__ void load_colg(const Run& R, ColVals<NP, CWR>& v, const float2* f, int p, int rho) {
+  if (R.loc) {
+    const LocalRef L = local_ref(f + (size_t)rho * NP + p);
+#pragma unroll
+    for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = lload(L, (uint32_t)(l * Col<NP, CWR>::LPC * NP * 8));
+    return;
+  }
 #pragma unroll
   for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = cload(f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p);
 }
 template <int NP, int CWR>
-__device__ __forceinline__ void store_colg(float2* f, const ColVals<NP, CWR>& v, int p, int rho) {
+__device__ __forceinline__ void store_colg(const Run& R, float2* f, const ColVals<NP, CWR>& v, int p, int rho) {
 #pragma unroll
-  for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p, v[l]);
+  for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(R, f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p, v[l]);
 }
 template <int NP, int CWR>
 __device__ __forceinline__ void fft_fwd_colg(ColVals<NP, CWR>& v, const float2* tw, int rho) {
@@ -519,7 +598,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
   double err = 0.0;
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     float2 v[CL];
-    load_colg<NP, CWR>(v, W + f * Np2, p, rho);
+    load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
     fft_fwd_colg<NP, CWR>(v, R.tw, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
@@ -536,11 +615,11 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
       v[l] = cscale(v[l], sqrtf(im / (iest + 1e-10f)));
     }
     fft_inv_colg<NP, CWR>(v, R.tw, rho);
-    if (act) store_colg<NP, CWR>(W + f * Np2, v, p, rho);
+    if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
     __syncthreads();
   CITEMS_END
   const double s = block_sum(R, err);
-  if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
+  if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
 
 // Column pass of an angular-spectrum propagation: forward column transform,
@@ -560,7 +639,7 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
     // the transfer function of field f's wavelength
     const float2* __restrict__ HT = D.HT + (size_t)(f / D.fpl) * Np2;
     float2 v[CL];
-    load_colg<NP, CWR>(v, W + f * Np2, p, rho);
+    load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
     fft_fwd_colg<NP, CWR>(v, R.tw, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
@@ -569,7 +648,7 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
       v[l] = cmul(v[l], h);
     }
     fft_inv_colg<NP, CWR>(v, R.tw, rho);
-    if (act) store_colg<NP, CWR>(W + f * Np2, v, p, rho);
+    if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
   CITEMS_END
 }
 
@@ -588,7 +667,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     float2 v[L];
     float im[L];
-    load_row<NP>(v, W + f * Np2 + (size_t)r * NP, R.t);
+    load_row<NP>(R, v, W + f * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
     fft_inv<NP>(v, R.tw, R.t);
@@ -605,11 +684,11 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
       v[l] = cscale(v[l], sqrtf(im[l] / (iest + 1e-10f)));
     }
     fft_fwd<NP>(v, R.tw, R.t);
-    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
   ITEMS_END
   const double s = block_sum(R, err);
-  if (threadIdx.x == 0) cstored(ep + (size_t)j * R.G + R.g, s);
+  if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
 
 // Last row pass: inverse row transform gives the updated exit wave of the
@@ -650,9 +729,9 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], of[L], pf[L];
-    load_row<NP>(d, W + f * Np2 + prow_off, R.t);
-    load_row<NP>(of, O + x.fo * No2 + orow_off, R.t);
-    load_row<NP>(pf, P + x.fp * Np2 + prow_off, R.t);
+    load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+    load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+    load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
     fft_inv<NP>(d, R.tw, R.t);
     float2 tO[L], tP[L];
 #pragma unroll
@@ -693,7 +772,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     constexpr bool qn = QN;
     // qNewton keeps the difference for its probe step; only this item
     // touches row r of field f in this pass
-    if (qn && act) store_row<NP>(W + f * Np2 + prow_off, d, R.t);
+    if (qn && act) store_row<NP>(R, W + f * Np2 + prow_off, d, R.t);
     __syncthreads();
     if (act && x.ip == 0) {
       // object field x.fo: ePIE and qNewton sum over (lambda, psm, slice),
@@ -709,8 +788,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
         }
         nv[l] = cadd(of[l], cscale(acc, bO));
       }
-      if (qn) store_row<NP>(D.OP + x.fo * Np2 + prow_off, of, R.t);
-      store_row<NP>(O + x.fo * No2 + orow_off, nv, R.t);
+      if (qn) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
+      store_row<NP>(R, O + x.fo * No2 + orow_off, nv, R.t);
     }
     if (act && x.io == 0 && !qn) {
       // probe field x.fp: ePIE sums over (lambda, osm, slice), mPIE over osm,
@@ -727,7 +806,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
         }
         nv[l] = cadd(pf[l], cscale(acc, engine == PTY_EPIE ? D.a.betaProbe : stepP));
       }
-      store_row<NP>(P + x.fp * Np2 + prow_off, nv, R.t);
+      store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
     }
     __syncthreads();
   ITEMS_END
@@ -747,7 +826,7 @@ __device__ __forceinline__ void qn_omax_pass(Run& R, unsigned* slot) {
   float m = 0.f;
   for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
     float s = 0.f;
-    for (int fo = 0; fo < D.nO; ++fo) s += sqrtf(cabs2(cload(O + fo * n + i)));
+    for (int fo = 0; fo < D.nO; ++fo) s += sqrtf(cabs2(cload(R, O + fo * n + i)));
     m = fmaxf(m, s);
   }
   m = wave_max(m);
@@ -772,8 +851,8 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
     const FieldIdx x = field_idx(f, no, npm, ns);
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], op[L], t[L];
-    load_row<NP>(d, D.W + f * Np2 + prow_off, R.t);
-    load_row<NP>(op, D.OP + x.fo * Np2 + prow_off, R.t);
+    load_row<NP>(R, d, D.W + f * Np2 + prow_off, R.t);
+    load_row<NP>(R, op, D.OP + x.fo * Np2 + prow_off, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const float a = sqrtf(cabs2(op[l])) / Omax;
@@ -785,7 +864,7 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
     __syncthreads();
     if (act && x.io == 0) {
       float2 pf[L], nv[L];
-      load_row<NP>(pf, P + x.fp * Np2 + prow_off, R.t);
+      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 acc = make_float2(0.f, 0.f);
@@ -795,7 +874,7 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
         }
         nv[l] = cadd(pf[l], cscale(acc, bP));
       }
-      store_row<NP>(P + x.fp * Np2 + prow_off, nv, R.t);
+      store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
     }
     __syncthreads();
   ITEMS_END
@@ -817,12 +896,12 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
     const size_t b = chunk * R.g;
     const size_t e = b + chunk < n ? b + chunk : n;
     for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
-      const float2 x = cload(X + i), bb = cload(B + i), m = cload(M + i);
+      const float2 x = cload(R, X + i), bb = cload(R, B + i), m = cload(R, M + i);
       const float2 mn = make_float2((bb.x - x.x) + m.x * eta, (bb.y - x.y) + m.y * eta);
       const float2 xn = make_float2(x.x - mn.x * nu, x.y - mn.y * nu);
-      cstore(M + i, mn);
-      cstore(X + i, xn);
-      cstore(B + i, xn);
+      cstore(R, M + i, mn);
+      cstore(R, X + i, xn);
+      cstore(R, B + i, xn);
     }
   }
 }
@@ -853,20 +932,20 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     float2 of[L], pv[L], q[L];
-    load_row<NP>(of, O + (size_t)((f / npm) * ns + 0) * No2 + orow_off, R.t);
-    load_row<NP>(pv, P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP, R.t);
+    load_row<NP>(R, of, O + (size_t)((f / npm) * ns + 0) * No2 + orow_off, R.t);
+    load_row<NP>(R, pv, P + (size_t)(f * ns + 0) * Np2 + (size_t)r * NP, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) q[l] = make_float2(act ? cabs2(pv[l]) : 0.f, 0.f);
     lds_put<NP>(R.xa, islot, q, R.t);
     float2 v[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) v[l] = cmul(of[l], pv[l]);
-    if (act) store_row<NP>(E + (size_t)f * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, E + (size_t)f * Np2 + (size_t)r * NP, v, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l)
       if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
     fft_fwd<NP>(v, R.tw, R.t);
-    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
 #pragma unroll
@@ -882,7 +961,7 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
           for (int l = 0; l < L; ++l) {
             float so = 0.f;
             for (int il = 0; il < nl; ++il)
-              so += cabs2(cload(O + (size_t)(il * ns + s) * No2 + orow_off + l * LPF + R.t));
+              so += cabs2(cload(R, O + (size_t)(il * ns + s) * No2 + orow_off + l * LPF + R.t));
             omx[s] = fmaxf(omx[s], so);
           }
         }
@@ -922,22 +1001,22 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     float2 v[L], os[L], q[L];
-    load_row<NP>(v, W + f * Np2 + (size_t)r * NP, R.t);
-    load_row<NP>(os, O + (size_t)((f / npm) * ns + s) * No2 + orow_off, R.t);
+    load_row<NP>(R, v, W + f * Np2 + (size_t)r * NP, R.t);
+    load_row<NP>(R, os, O + (size_t)((f / npm) * ns + s) * No2 + orow_off, R.t);
     fft_inv<NP>(v, R.tw, R.t);
-    if (act) store_row<NP>(P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       q[l] = make_float2(act ? cabs2(v[l]) : 0.f, 0.f);
       v[l] = cmul(v[l], os[l]);
     }
     lds_put<NP>(R.xa, islot, q, R.t);
-    if (act) store_row<NP>(E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, E + (size_t)(s * nF + f) * Np2 + (size_t)r * NP, v, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l)
       if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
     fft_fwd<NP>(v, R.tw, R.t);
-    if (act) store_row<NP>(W + f * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
 #pragma unroll
@@ -982,10 +1061,10 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], pv[L], ov[L], ev[L];
-    load_row<NP>(d, W + f * Np2 + prow_off, R.t);
-    load_row<NP>(pv, P + (size_t)(f * ns + su) * Np2 + prow_off, R.t);
-    load_row<NP>(ev, E + (size_t)(esl * nF + f) * Np2 + prow_off, R.t);
-    load_row<NP>(ov, O + (size_t)((f / npm) * ns + su) * No2 + orow_off, R.t);
+    load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+    load_row<NP>(R, pv, P + (size_t)(f * ns + su) * Np2 + prow_off, R.t);
+    load_row<NP>(R, ev, E + (size_t)(esl * nF + f) * Np2 + prow_off, R.t);
+    load_row<NP>(R, ov, O + (size_t)((f / npm) * ns + su) * No2 + orow_off, R.t);
     fft_inv<NP>(d, R.tw, R.t);
     float2 tO[L], tP[L];
 #pragma unroll
@@ -1011,7 +1090,7 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
         for (int g = 0; g < nF; ++g) acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
         nv[l] = cadd(ov[l], cscale(acc, bO));
       }
-      store_row<NP>(O + (size_t)((f / npm) * ns + su) * No2 + orow_off, nv, R.t);
+      store_row<NP>(R, O + (size_t)((f / npm) * ns + su) * No2 + orow_off, nv, R.t);
     }
     // probe slice su of this field: sum over lambda (axes (0, 1))
     float2 nv[L];
@@ -1021,10 +1100,10 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
       for (int g = f % npm; g < nF; g += npm) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
       nv[l] = cadd(pv[l], cscale(acc, bP));
     }
-    if (act) store_row<NP>(P + (size_t)(f * ns + su) * Np2 + prow_off, nv, R.t);
+    if (act) store_row<NP>(R, P + (size_t)(f * ns + su) * Np2 + prow_off, nv, R.t);
     if (su >= 1) {
       fft_fwd<NP>(nv, R.tw, R.t);
-      if (act) store_row<NP>(W + f * Np2 + prow_off, nv, R.t);
+      if (act) store_row<NP>(R, W + f * Np2 + prow_off, nv, R.t);
     }
     __syncthreads();
   ITEMS_END
@@ -1125,7 +1204,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       for (int jj = threadIdx.x; jj < F; jj += R.nt) {
         const int kk = order[jj];
         double s = 0.0;
-        for (int g = 0; g < R.G; ++g) s += cloadd(ep + (size_t)jj * R.G + g);
+        for (int g = 0; g < R.G; ++g) s += cloadd(R, ep + (size_t)jj * R.G + g);
         const float en = D.a.energy[kk] + 1e-20f;
         const double e = s / (double)en;
         D.a.err_at_pos[(size_t)it * F + kk] = e;
@@ -1160,6 +1239,8 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   R.xa = s_x;
   R.xb = s_x + R.ipr * NP;
   R.fail = s_fail;
+  R.loc = false;
+  R.grp = blockIdx.x & 7;
   for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
 }
@@ -1170,7 +1251,7 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
 template <int NP, int FAM>
 __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst* __restrict__ insts,
                                                                   const int2* __restrict__ bmap, int it0, int it1,
-                                                                  int jb, int je) {
+                                                                  int jb, int je, unsigned* claim, int nslots) {
   __shared__ float2 s_tw[256];
   __shared__ double s_red[Cfg<NP>::NW];
   __shared__ float2 s_x[Cfg<NP>::XBUF];
@@ -1205,44 +1286,82 @@ constexpr int BATCH_XBUF =
          BatchCfg<256>::XBUF);
 
 template <int FAM>
-__global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) void pty_batch_kernel(const DevInst* __restrict__ insts,
-                                                             const int2* __restrict__ bmap, int it0, int it1, int jb,
-                                                             int je) {
+__global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) void pty_batch_kernel(
+    const DevInst* __restrict__ insts, const int2* __restrict__ bmap, int it0, int it1, int jb, int je,
+    unsigned* claim, int nslots) {
   __shared__ float2 s_tw[256];
   __shared__ double s_red[BATCH_NT / 64];
   __shared__ float2 s_x[BATCH_XBUF];
   __shared__ int s_fail;
-  const int2 bm = bmap[blockIdx.x];
+  __shared__ int s_blk;
+  // The plan packs every cluster onto one XCD: bmap entry N_XCD * s + x is
+  // slot s of XCD x.  A workgroup reads the XCD it runs on (XCC_ID hardware
+  // register) and claims the next slot of that XCD, so the placement holds
+  // whatever XCD the dispatcher's round robin starts from.
+  if (threadIdx.x == 0) {
+    const unsigned xcc = (unsigned)__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u;
+    const unsigned slot = __hip_atomic_fetch_add(claim + 32 * xcc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_blk = slot < (unsigned)nslots ? (int)(N_XCD * slot + xcc) : -1;
+  }
+  __syncthreads();
+  const int blk = s_blk;
+  if (blk < 0) {
+    // more workgroups reached this XCD than it has slots, so some cluster
+    // misses a member: stop every cluster of the launch
+    for (int b = threadIdx.x; b < N_XCD * nslots; b += BATCH_NT) {
+      const int2 e = bmap[b];
+      if (e.x >= 0 && e.y == 0) {
+        __hip_atomic_store(insts[e.x].ctrl + CTRL_PLACE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(insts[e.x].ctrl + CTRL_FAIL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    return;
+  }
+  const int2 bm = bmap[blk];
   if (bm.x < 0) return;
   const DevInst* D = insts + bm.x;
   Run R;
   switch (D->a.Np) {
     case 16:
       init_run<16, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      R.loc = true;
+      R.grp = blk & 7;
       run_iterations<16, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 32:
       init_run<32, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      R.loc = true;
+      R.grp = blk & 7;
       run_iterations<32, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 64:
       init_run<64, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      R.loc = true;
+      R.grp = blk & 7;
       run_iterations<64, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 128:
       init_run<128, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      R.loc = true;
+      R.grp = blk & 7;
       run_iterations<128, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 256:
       init_run<256, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      R.loc = true;
+      R.grp = blk & 7;
       run_iterations<256, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
   }
 }
 
-__global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts) {
+// clears the control block of every instance (one block each) and the
+// XCD slot counters of the batched launches
+__global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts, unsigned* claim, int nclaim) {
   unsigned* c = insts[blockIdx.x].ctrl;
   for (int i = threadIdx.x; i < CTRL_WORDS; i += blockDim.x) c[i] = 0u;
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < nclaim; i += blockDim.x) claim[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1404,6 +1523,8 @@ struct LaunchClass {
   int nthreads = 0;
   bool cooperative = false;  // some instance of the class has G > 1
   int2* d_bmap = nullptr;
+  unsigned* d_claim = nullptr;  // batched class: N_XCD slot counters, 32 words apart
+  int nslots = 0;               // batched class: slots per XCD (nblocks / N_XCD)
   hipGraphNode_t node = nullptr;
 };
 
@@ -1413,6 +1534,8 @@ struct pty_plan {
   int nblocks = 0;
   int max_iter = 0;
   DevInst* d_insts = nullptr;
+  unsigned* d_claims = nullptr;  // slot counters of every class, cleared with the control blocks
+  int nclaim = 0;
   std::vector<DevInst> h_insts;
   std::vector<LaunchClass> classes;
   std::vector<void*> allocs;
@@ -1502,6 +1625,17 @@ void free_plan(pty_plan* p) {
 }
 
 // whether an instance runs in the batched class of its family
+// Co-resident workgroups of the batched kernel of family fam on one XCD
+// (gfx950: N_XCD XCDs of equal CU count).
+int xcd_capacity(int fam, int* cap) {
+  int dev = 0, cus = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(0, fam), BATCH_NT, 0));
+  *cap = per_cu * cus / N_XCD;
+  return PTY_OK;
+}
+
 bool joins_batch(const pty_instance& a, int ninst) {
   return a.Np <= BATCH_MAX_NP && (ninst > 1 || (a.flags & PTY_FLAG_BATCHED));
 }
@@ -1637,6 +1771,17 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     memset(&d, 0, sizeof d);
     d.a = a;
     int G = a.workgroups > 0 ? a.workgroups : auto_workgroups(a, ninst, nt);
+    if (batched) {
+      // a batched cluster runs on one XCD
+      int xcap = 0;
+      if ((rc = xcd_capacity(family_of(a), &xcap))) goto fail;
+      if (a.workgroups > xcap) {
+        rc = set_err(PTY_EINVAL, "instance %d: %d workgroups exceed the %d of one XCD that a batched cluster may use", i,
+                     a.workgroups, xcap);
+        goto fail;
+      }
+      if (G > xcap) G = xcap;
+    }
     if (G > a.Np) G = a.Np;
     if (G < 1) G = 1;
     while (a.Np % G) --G;
@@ -1735,10 +1880,55 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     }
     for (int x = 0; x < 8; ++x) d.ngroups += d.gsize[x] > 0;
   }
+  // Batched classes: each cluster goes whole onto one XCD, the largest
+  // clusters first, each to the least-loaded XCD; cluster member g of an
+  // instance on XCD x in slot s takes block 8 s + x.  Unused blocks get
+  // instance -1 and exit at once.
+  for (size_t c = 0; c < p->classes.size(); ++c) {
+    if (p->classes[c].np != 0) continue;
+    std::vector<int> ids;
+    for (const int2& e : cmaps[c])
+      if (e.y == 0) ids.push_back(e.x);
+    std::stable_sort(ids.begin(), ids.end(), [&](int u, int v) { return p->h_insts[u].G > p->h_insts[v].G; });
+    int xcap = 0;
+    if ((rc = xcd_capacity(p->classes[c].fam, &xcap))) goto fail;
+    int used[N_XCD] = {0};
+    std::vector<int2> m;
+    for (const int i : ids) {
+      DevInst& d = p->h_insts[i];
+      int x = 0;
+      for (int y = 1; y < N_XCD; ++y)
+        if (used[y] < used[x]) x = y;
+      if (used[x] + d.G > xcap) {
+        rc = set_err(PTY_EINVAL, "batched launch of family %d: the clusters do not fit on %d XCDs of %d workgroups",
+                     p->classes[c].fam, N_XCD, xcap);
+        goto fail;
+      }
+      for (int g = 0; g < d.G; ++g) {
+        const size_t b = (size_t)N_XCD * (used[x] + g) + x;
+        if (m.size() <= b) m.resize(b + 1, make_int2(-1, 0));
+        m[b] = make_int2(i, g);
+      }
+      used[x] += d.G;
+      for (int y = 0; y < 8; ++y) d.gsize[y] = 0;
+      d.gsize[x] = d.G;
+      d.ngroups = 1;
+    }
+    int nslots = 0;
+    for (int y = 0; y < N_XCD; ++y) nslots = used[y] > nslots ? used[y] : nslots;
+    m.resize((size_t)N_XCD * nslots, make_int2(-1, 0));  // every XCD receives nslots blocks
+    p->classes[c].nslots = nslots;
+    cmaps[c].swap(m);
+  }
   {
     void* mem = nullptr;
     if ((rc = dalloc(p, &mem, sizeof(DevInst) * ninst))) goto fail;
     p->d_insts = (DevInst*)mem;
+    p->nclaim = (int)p->classes.size() * N_XCD * 32;
+    if ((rc = dalloc(p, &mem, sizeof(unsigned) * p->nclaim))) goto fail;
+    p->d_claims = (unsigned*)mem;
+    HIP_TRY(hipMemsetAsync(p->d_claims, 0, sizeof(unsigned) * p->nclaim, st));
+    for (size_t c = 0; c < p->classes.size(); ++c) p->classes[c].d_claim = p->d_claims + c * N_XCD * 32;
     HIP_TRY(hipMemcpyAsync(p->d_insts, p->h_insts.data(), sizeof(DevInst) * ninst, hipMemcpyHostToDevice, st));
     int cus = 0, coop = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
@@ -1747,7 +1937,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     for (size_t c = 0; c < p->classes.size(); ++c) {
       LaunchClass& lc = p->classes[c];
       lc.nblocks = (int)cmaps[c].size();
-      p->nblocks += lc.nblocks;
+      for (const int2& e : cmaps[c]) p->nblocks += e.x >= 0;
       if ((rc = dalloc(p, &mem, sizeof(int2) * cmaps[c].size()))) goto fail;
       lc.d_bmap = (int2*)mem;
       HIP_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
@@ -1799,7 +1989,9 @@ static int graph_build(pty_plan* p) {
   const DevInst* di = p->d_insts;
   int zero = 0;
   hipKernelNodeParams kp = {};
-  void* rargs[] = {(void*)&di};
+  unsigned* claims = p->d_claims;
+  int nclaim = p->nclaim;
+  void* rargs[] = {(void*)&di, (void*)&claims, (void*)&nclaim};
   kp.func = (void*)pty_reset_ctrl_kernel;
   kp.gridDim = dim3(p->ninst);
   kp.blockDim = dim3(64);
@@ -1810,7 +2002,9 @@ static int graph_build(pty_plan* p) {
   HIP_TRY(hipGraphAddKernelNode(&prev, p->graph, nullptr, 0, &kp));
   for (LaunchClass& lc : p->classes) {
     const int2* bm = lc.d_bmap;
-    void* args[] = {(void*)&di, (void*)&bm, (void*)&zero, (void*)&zero, (void*)&zero, (void*)&zero};
+    unsigned* cl = lc.d_claim;
+    void* args[] = {(void*)&di, (void*)&bm, (void*)&zero, (void*)&zero, (void*)&zero, (void*)&zero, (void*)&cl,
+                    (void*)&lc.nslots};
     kp.func = (void*)iterate_kernel_ptr(lc.np, lc.fam);
     kp.gridDim = dim3(lc.nblocks);
     kp.blockDim = dim3(lc.nthreads);
@@ -1830,7 +2024,8 @@ static int graph_replay(pty_plan* p, int it0, int it1, int jb, int je, hipStream
   const DevInst* di = p->d_insts;
   for (LaunchClass& lc : p->classes) {
     const int2* bm = lc.d_bmap;
-    void* args[] = {(void*)&di, (void*)&bm, (void*)&it0, (void*)&it1, (void*)&jb, (void*)&je};
+    unsigned* cl = lc.d_claim;
+    void* args[] = {(void*)&di, (void*)&bm, (void*)&it0, (void*)&it1, (void*)&jb, (void*)&je, (void*)&cl, (void*)&lc.nslots};
     hipKernelNodeParams kp = {};
     kp.func = (void*)iterate_kernel_ptr(lc.np, lc.fam);
     kp.gridDim = dim3(lc.nblocks);
@@ -1858,13 +2053,14 @@ static int plan_launch(pty_plan* p, int32_t it_begin, int32_t it_end, int32_t jb
     }
     return PTY_OK;
   }
-  pty_reset_ctrl_kernel<<<p->ninst, 64, 0, st>>>(p->d_insts);
+  pty_reset_ctrl_kernel<<<p->ninst, 64, 0, st>>>(p->d_insts, p->d_claims, p->nclaim);
   HIP_TRY(hipGetLastError());
   const DevInst* di = p->d_insts;
   int a0 = it_begin, a1 = it_end, b0 = jb, b1 = je;
   for (LaunchClass& lc : p->classes) {
     const int2* bm = lc.d_bmap;
-    void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1};
+    unsigned* cl = lc.d_claim;
+    void* args[] = {(void*)&di, (void*)&bm, (void*)&a0, (void*)&a1, (void*)&b0, (void*)&b1, (void*)&cl, (void*)&lc.nslots};
     if (lc.cooperative)
       HIP_TRY(hipLaunchCooperativeKernel(iterate_kernel_ptr(lc.np, lc.fam), dim3(lc.nblocks), dim3(lc.nthreads), args, 0, st));
     else
@@ -1904,9 +2100,11 @@ int pty_plan_check(pty_plan* p, void* hip_stream) {
   hipStream_t st = (hipStream_t)hip_stream;
   HIP_TRY(hipStreamSynchronize(st));
   for (size_t i = 0; i < p->h_insts.size(); ++i) {
-    unsigned flag = 0;
-    HIP_TRY(hipMemcpy(&flag, p->h_insts[i].ctrl + CTRL_FAIL, sizeof flag, hipMemcpyDeviceToHost));
-    if (flag) return set_err(PTY_ETIMEDOUT, "instance %zu: a cluster barrier timed out", i);
+    unsigned w[3] = {0u, 0u, 0u};
+    HIP_TRY(hipMemcpy(w, p->h_insts[i].ctrl, sizeof w, hipMemcpyDeviceToHost));
+    if (w[CTRL_PLACE])
+      return set_err(PTY_EDEVICE, "instance %zu: a workgroup of its XCD-local cluster ran on another XCD", i);
+    if (w[CTRL_FAIL]) return set_err(PTY_ETIMEDOUT, "instance %zu: a cluster barrier timed out", i);
   }
   return PTY_OK;
 }

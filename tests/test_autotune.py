@@ -81,3 +81,26 @@ def test_cost_table_round_trips_json(tmp_path):
     t.table[s.key(2)] = 12.5
     t.save()
     assert A.CostTable(tmp_path / "t.json")(s, 2) == 12.5
+
+
+def test_packing_follows_the_plan_placement():
+    # largest first, each to the least-loaded bin
+    assert A.packs([4, 2, 2], 2, 4)
+    assert not A.packs([4, 4, 1], 2, 4)
+    assert A.packs([1] * 8, 8, 1) and not A.packs([1] * 9, 8, 1)
+    # powers of two up to the bin size pack whenever the total fits
+    assert A.packs([2, 1, 1, 2, 2], 2, 4)
+
+
+@pytest.mark.parametrize("bins,per_bin", [(2, 4), (4, 4), (8, 2), (3, 8)])
+def test_greedy_with_bins_matches_brute_force(bins, per_bin):
+    jobs = _jobs([(64, 1, 300, 10), (128, 1, 120, 10), (32, 2, 700, 10), (64, 1, 500, 10)])
+    cap = bins * per_bin
+    G = A.assign(jobs, _cost, cap, bins)
+    assert A.packs(G, bins, per_bin) and max(G) <= per_bin
+    best = None
+    for c in itertools.product(*[A.candidate_sizes(j.shape, per_bin) for j in jobs]):
+        if A.packs(list(c), bins, per_bin):
+            m = A.predicted_makespan(jobs, _cost, c)
+            best = m if best is None else min(best, m)
+    assert A.predicted_makespan(jobs, _cost, G) == pytest.approx(best, rel=1e-12)

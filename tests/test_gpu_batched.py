@@ -121,3 +121,22 @@ def test_too_many_fields_for_a_workgroup_raise(datasets):
     eng = build_package(w.as_dict(), "ePIE", "Fraunhofer", (1, 1, 16, 1), 1, 1, No=w.No)
     with pytest.raises(_native.NativeError, match="exceed the 8 waves"):
         P.Engines.Batched([eng], workgroups=[1]).reconstruct()
+
+
+def test_cluster_larger_than_one_xcd_raises(datasets):
+    """A batched cluster runs whole on one XCD (64 co-resident workgroups of
+    the batched kernel on MI355X); plan creation rejects a larger one."""
+    import ptylab_amd as P
+    from ptylab_amd import _native
+
+    engs = _engines(datasets)
+    G = [1, 1, 1, 128, 1, 1]
+    with pytest.raises(_native.NativeError, match="of one XCD"):
+        P.Engines.Batched(engs, workgroups=G).reconstruct()
+
+
+def test_plan_reports_active_workgroups(datasets):
+    """Blocks left idle by the XCD packing are not counted as workgroups."""
+    _, b = _batched(datasets, workgroups=[1, 1, 2, 4, 2, 1], graph=True)
+    assert b.last_workgroups == [1, 1, 2, 4, 2, 1]
+    assert b.last_plan_workgroups == sum(b.last_workgroups)
