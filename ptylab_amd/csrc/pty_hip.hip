@@ -52,10 +52,10 @@ using namespace pty;
 #define CTRL_TOP 0
 #define CTRL_FAIL 1
 #define CTRL_PLACE 2       // set when a workgroup of an XCD-local cluster ran on another XCD
-#define CTRL_SLOTS 32      // 4 ring slots x 32 words of per-position maxima
 #define CTRL_GROUP 160     // 8 XCD-group arrival counters, 32 words apart
 #define CTRL_RELEASE 416   // published barrier generation
 #define CTRL_WORDS 448
+#define SLOT_WORDS 32  // words of per-position maxima in each of the 4 ring slots
 
 #define N_XCD 8  // XCDs of an MI355X (gfx950), each with its own L2
 
@@ -110,6 +110,7 @@ struct DevInst {
   int fpl;           // W fields per transfer function (fields of one wavelength)
   float2* OP;        // qNewton: the object patch before the position's update (nO, Np, Np)
   unsigned* ctrl;    // control block, CTRL_* layout
+  unsigned* mx;      // per-position maxima: 4 ring slots x SLOT_WORDS words x G workgroup partials
   int gsize[8];      // workgroups of this instance in each XCD group
   int ngroups;       // non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
@@ -181,11 +182,26 @@ __device__ __forceinline__ double block_sum(Run& R, double v) {
   return s;
 }
 
-__device__ __forceinline__ void atomic_fmax_slot(unsigned* p, float v) {
-  __hip_atomic_fetch_max(p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Per-position maxima (of non-negative floats, ordered as their bit
+// patterns).  Each workgroup keeps its own partial of every word of a ring
+// slot, at slot + w * G + g, which its waves combine with an atomic max; a
+// reader takes the maximum over the G partials.  Atomics on G separate
+// addresses proceed in parallel where one shared word per maximum would
+// serialise every wave of the cluster.
+__device__ __forceinline__ void atomic_fmax_slot(const Run& R, unsigned* slot, int w, float v) {
+  __hip_atomic_fetch_max(slot + w * R.G + R.g, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ float load_slot(const unsigned* p) {
-  return __uint_as_float(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+// called by every lane of a wave; returns the maximum to all of them
+__device__ __forceinline__ float load_slot(const Run& R, const unsigned* slot, int w) {
+  const unsigned* p = slot + w * R.G;
+  float m = 0.f;
+  for (int g = R.lane; g < R.G; g += 64)
+    m = fmaxf(m, __uint_as_float(__hip_atomic_load(p + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  return wave_max(m);
+}
+// clears this workgroup's partials of a ring slot
+__device__ __forceinline__ void clear_slot(const Run& R, unsigned* slot) {
+  if (threadIdx.x < SLOT_WORDS) __hip_atomic_store(slot + threadIdx.x * R.G + R.g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Cross-workgroup data of the persistent kernel (exit waves, object, probe,
@@ -469,7 +485,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   const bool cb = prop_checkerboard(D.a.propagator);
   const float2* __restrict__ Qp = D.Q;
   constexpr bool qn = QN;
-  if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
+  clear_slot(R, zslot);
   float pmax = 0.f, omax = 0.f;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const FieldIdx x = field_idx(f, no, npm, ns);
@@ -515,8 +531,8 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   pmax = wave_max(pmax);
   omax = wave_max(omax);
   if (R.lane == 0) {
-    atomic_fmax_slot(slot + 0, pmax);
-    atomic_fmax_slot(slot + 1, omax);
+    atomic_fmax_slot(R, slot, 0, pmax);
+    atomic_fmax_slot(R, slot, 1, omax);
   }
 }
 
@@ -714,7 +730,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   const float2* __restrict__ Qp = D.Q;
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
-  const float Pmax = load_slot(slot + 0), Omax = load_slot(slot + 1);
+  const float Pmax = load_slot(R, slot, 0), Omax = load_slot(R, slot, 1);
   const float bO = D.a.betaObject;
   const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
   // ePIE.py:128-160 weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
@@ -830,7 +846,7 @@ __device__ __forceinline__ void qn_omax_pass(Run& R, unsigned* slot) {
     m = fmaxf(m, s);
   }
   m = wave_max(m);
-  if (R.lane == 0) atomic_fmax_slot(slot + 2, m);
+  if (R.lane == 0) atomic_fmax_slot(R, slot, 2, m);
 }
 
 // qNewton probe step (qNewton.py:129-140): probe += betaProbe sum over
@@ -844,7 +860,7 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
   const int nF = D.nF;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ P = (float2*)D.a.probe;
-  const float Omax = load_slot(slot + 2);
+  const float Omax = load_slot(R, slot, 2);
   const float reg = D.a.alphaProbe, bP = D.a.betaProbe;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     (void)frw;
@@ -924,7 +940,7 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
   float2* __restrict__ W = D.W;
   float2* __restrict__ E = D.ESW;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER && ns == 1;
-  if (R.g == 0 && threadIdx.x < 32) cstoreu(zslot + threadIdx.x, 0u);
+  clear_slot(R, zslot);
   float omx[MAX_SLICES];
 #pragma unroll
   for (int s = 0; s < MAX_SLICES; ++s) omx[s] = 0.f;
@@ -970,12 +986,12 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
     __syncthreads();
   ITEMS_END
   pmax = wave_max(pmax);
-  if (R.lane == 0) atomic_fmax_slot(slot + 0, pmax);
+  if (R.lane == 0) atomic_fmax_slot(R, slot, 0, pmax);
 #pragma unroll
   for (int s = 0; s < MAX_SLICES; ++s) {
     if (s < ns) {
       const float m = wave_max(omx[s]);
-      if (R.lane == 0) atomic_fmax_slot(slot + 2 * s + 1, m);
+      if (R.lane == 0) atomic_fmax_slot(R, slot, 2 * s + 1, m);
     }
   }
 }
@@ -1029,7 +1045,7 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
     __syncthreads();
   ITEMS_END
   pmax = wave_max(pmax);
-  if (R.lane == 0) atomic_fmax_slot(slot + 2 * s, pmax);
+  if (R.lane == 0) atomic_fmax_slot(R, slot, 2 * s, pmax);
 }
 
 // Update of slice su from the difference wave: `fromDetector` selects the
@@ -1051,8 +1067,8 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
   const float2* __restrict__ E = D.ESW;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER;
   const float invN = 1.0f / (float)NP;
-  const float invP = 1.0f / load_slot(slot + 2 * su);
-  const float invO = 1.0f / load_slot(slot + 2 * su + 1);
+  const float invP = 1.0f / load_slot(R, slot, 2 * su);
+  const float invO = 1.0f / load_slot(R, slot, 2 * su + 1);
   const float bO = D.a.betaObject;
   const float bP = su >= 1 ? D.a.sliceBeta : D.a.betaProbe;
   const int esl = fromDetector ? ns - 1 : su;
@@ -1183,8 +1199,8 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
     const int j1 = it == it1 - 1 ? (je < F ? je : F) : F;
     for (int j = j0; j < j1; ++j, ++q) {
       const int k = order[j];
-      unsigned* slot = D.ctrl + CTRL_SLOTS + (q & 3u) * 32;
-      unsigned* zslot = D.ctrl + CTRL_SLOTS + ((q + 2u) & 3u) * 32;
+      unsigned* slot = D.mx + (size_t)(q & 3u) * SLOT_WORDS * R.G;
+      unsigned* zslot = D.mx + (size_t)((q + 2u) & 3u) * SLOT_WORDS * R.G;
       bool ok;
       if constexpr (FAM == FAM_E3PIE)
         ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
@@ -1360,6 +1376,9 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
 __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts, unsigned* claim, int nclaim) {
   unsigned* c = insts[blockIdx.x].ctrl;
   for (int i = threadIdx.x; i < CTRL_WORDS; i += blockDim.x) c[i] = 0u;
+  unsigned* m = insts[blockIdx.x].mx;
+  const int nm = 4 * SLOT_WORDS * insts[blockIdx.x].G;
+  for (int i = threadIdx.x; i < nm; i += blockDim.x) m[i] = 0u;
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < nclaim; i += blockDim.x) claim[i] = 0u;
 }
@@ -1835,6 +1854,8 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     }
     if ((rc = dalloc(p, &mem, CTRL_WORDS * sizeof(unsigned)))) goto fail;
     d.ctrl = (unsigned*)mem;
+    if ((rc = dalloc(p, &mem, (size_t)4 * SLOT_WORDS * G * sizeof(unsigned)))) goto fail;
+    d.mx = (unsigned*)mem;
     if ((rc = dalloc(p, &mem, (size_t)2 * a.F * G * sizeof(double)))) goto fail;
     d.errpart = (double*)mem;
     if (prop_asp(a.propagator)) {
