@@ -48,13 +48,13 @@ using namespace pty;
 #define MAX_SLICES 15
 #define SPIN_LIMIT (1u << 22)
 
-// control block of an instance (32-bit words; each counter on its own 128-byte line)
-#define CTRL_TOP 0
-#define CTRL_FAIL 1
-#define CTRL_PLACE 2       // set when a workgroup of an XCD-local cluster ran on another XCD
-#define CTRL_GROUP 160     // 8 XCD-group arrival counters, 32 words apart
-#define CTRL_RELEASE 416   // published barrier generation
-#define CTRL_WORDS 448
+// control block of an instance (32-bit words)
+#define CTRL_TOP 0       // counter barrier: arrivals of complete groups
+#define CTRL_FAIL 1      // set when a cluster barrier timed out (every workgroup then leaves)
+#define CTRL_PLACE 2     // set when a workgroup of an XCD-local cluster found no slot on its XCD
+#define CTRL_GROUP 32    // counter barrier: 8 group arrival counters, 32 words apart
+#define CTRL_RELEASE 288 // counter barrier: published generation
+#define CTRL_WORDS 320
 #define SLOT_WORDS 32  // words of per-position maxima in each of the 4 ring slots
 
 #define N_XCD 8  // XCDs of an MI355X (gfx950), each with its own L2
@@ -111,8 +111,9 @@ struct DevInst {
   float2* OP;        // qNewton: the object patch before the position's update (nO, Np, Np)
   unsigned* ctrl;    // control block, CTRL_* layout
   unsigned* mx;      // per-position maxima: 4 ring slots x SLOT_WORDS words x G workgroup partials
-  int gsize[8];      // workgroups of this instance in each XCD group
-  int ngroups;       // non-empty groups
+  unsigned* bar;     // flag barrier: generation reached by each workgroup (G words)
+  int gsize[8];      // counter barrier: workgroups of this instance in each block group b % 8
+  int ngroups;       // counter barrier: non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
 };
 
@@ -152,7 +153,6 @@ struct Run {
   float2* xb;
   int* fail;
   bool loc;  // every workgroup of the cluster runs on one XCD: exchange through its L2
-  int grp;   // arrival-counter group of this workgroup in the cluster barrier
 };
 
 // ---------------------------------------------------------------------------
@@ -197,7 +197,8 @@ __device__ __forceinline__ float load_slot(const Run& R, const unsigned* slot, i
   float m = 0.f;
   for (int g = R.lane; g < R.G; g += 64)
     m = fmaxf(m, __uint_as_float(__hip_atomic_load(p + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-  return wave_max(m);
+  // uniform: kept in a scalar register
+  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(m))));
 }
 // clears this workgroup's partials of a ring slot
 __device__ __forceinline__ void clear_slot(const Run& R, unsigned* slot) {
@@ -206,17 +207,48 @@ __device__ __forceinline__ void clear_slot(const Run& R, unsigned* slot) {
 
 // Cross-workgroup data of the persistent kernel (exit waves, object, probe,
 // momenta, maxima, error partials) is only ever read and written with
-// agent-scope coherent accesses (relaxed atomics: global_load/store ... sc1),
-// so no cache holds a stale copy of it.  A barrier then needs no cache
-// maintenance: every wave drains its outstanding stores (s_waitcnt
-// vmcnt(0): the stores have been performed at the coherence point), the
-// workgroup barrier orders that before thread 0's counter increment, and
-// once the counter reaches the generation target the workgroup proceeds.
-// This avoids the L2 write-back/invalidate of agent-scope fences, whose cost
-// grows with the number of workgroups per XCD.  The spin is bounded: on
-// time-out the failure flag is raised and every workgroup of the instance
-// leaves.
+// coherent accesses: agent-scope relaxed atomics (global_load/store ... sc1)
+// in general, or, for a cluster on one XCD, plain stores and L1-missing sc0
+// loads served by that XCD's L2; no cache the readers use holds a stale
+// copy.  A barrier then needs no cache maintenance: every wave drains its
+// outstanding stores (s_waitcnt vmcnt(0): the stores have been performed at
+// the coherence point), the workgroup barrier orders that before the
+// workgroup's arrival, and once every workgroup has arrived the cluster
+// proceeds.  This avoids the L2 write-back/invalidate of agent-scope fences,
+// whose cost grows with the number of workgroups per XCD.  The spin is
+// bounded: on time-out the failure flag is raised and every workgroup of the
+// instance leaves.
 __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */ }
+
+// XCD-local accessors.  When every workgroup of a cluster runs on one XCD,
+// cross-workgroup data need only be coherent in that XCD's L2: stores are
+// plain (the CU's L1 is write-through) and loads are buffer loads with the
+// sc0 bit, which miss in L1 and are served by L2.  The agent-scope forms
+// above instead miss in L2 as well.  The buffer resource is built from the
+// first active lane's address minus 2^31, so the 32-bit offsets of one
+// wave's accesses reach 2 GiB either side of it (a wave's accesses of a pass
+// lie within one field).
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+struct LocalRef {
+  __amdgpu_buffer_rsrc_t r;
+  uint32_t off;  // this lane's byte offset from the resource base
+};
+__device__ __forceinline__ LocalRef local_ref(const void* p) {
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  uint64_t base = ((uint64_t)hi << 32) | lo;
+  base = base >= 0x80000000ull ? base - 0x80000000ull : 0ull;
+  return {__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, -1, 0x00020000), (uint32_t)(a - base)};
+}
+__device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const unsigned o = (unsigned)__shfl_xor((int)v, m);
+    v = o < v ? o : v;
+  }
+  return v;
+}
 
 // Arrival is two-level: a workgroup increments the counter of its XCD group
 // (blocks of one launch are dispatched round-robin, block b on XCD b % 8;
@@ -225,15 +257,11 @@ __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0x0F
 // counter, and the last arriver overall publishes the generation on a
 // release line that the waiting workgroups poll.  Arrivals and polls thus
 // never contend for one address.
-__device__ __forceinline__ bool cluster_sync(Run& R) {
-  drain_stores();
-  __syncthreads();
-  if (R.G == 1) return true;
-  R.gen += 1u;
+__device__ __forceinline__ void counter_sync(Run& R) {
   if (threadIdx.x == 0) {
     const DevInst& D = *R.D;
     unsigned* ctrl = D.ctrl;
-    const int grp = R.grp;
+    const int grp = blockIdx.x & 7;
     unsigned* gctr = ctrl + CTRL_GROUP + 32 * grp;
     unsigned* top = ctrl + CTRL_TOP;
     unsigned* rel = ctrl + CTRL_RELEASE;
@@ -257,8 +285,46 @@ __device__ __forceinline__ bool cluster_sync(Run& R) {
     }
     *R.fail = fail;
   }
-  __syncthreads();
-  return *R.fail == 0;
+}
+
+// Flag barrier: workgroup g publishes the generation it has reached in its
+// own word D.bar[g] (a store, no atomic); wave 0 of every workgroup polls
+// the G words until all of them reach the generation.  Used by clusters on
+// one XCD: stores are plain and the polls sc0 buffer loads served by the
+// XCD's L2 (the volatile bit of the load keeps the compiler from hoisting it
+// out of the loop).  Measured against the counter barrier above it is the
+// faster one within an XCD and the slower one across XCDs, where every poll
+// of G words goes past the L2s.
+__device__ __forceinline__ void flag_sync(Run& R) {
+  if (R.wave == 0) {
+    const DevInst& D = *R.D;
+    unsigned* bar = D.bar;
+    unsigned* ctrl = D.ctrl;
+    const unsigned gen = R.gen;
+    if (R.lane == 0) {
+      *(__attribute__((address_space(1))) unsigned*)(bar + R.g) = gen;
+    }
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(bar, (short)0, R.G * 4, 0x00020000);
+    unsigned spins = 0;
+    int fail = 0;
+    for (;;) {
+      unsigned mn = 0xffffffffu;
+      for (int g = R.lane; g < R.G; g += 64) {
+        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(rb, (uint32_t)g * 4u, 0, 1 | (int)(1u << 31));
+        mn = v < mn ? v : mn;
+      }
+      if (wave_min_u32(mn) >= gen) break;
+      ++spins;
+      if ((spins & 255u) == 0u &&
+          (spins > SPIN_LIMIT || __hip_atomic_load(ctrl + CTRL_FAIL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        if (R.lane == 0) __hip_atomic_store(ctrl + CTRL_FAIL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (R.lane == 0) *R.fail = fail;
+  }
 }
 
 // coherent (agent-scope) accessors for cross-workgroup data
@@ -279,27 +345,22 @@ __device__ __forceinline__ void cstoreu(unsigned* p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// XCD-local accessors.  When every workgroup of a cluster runs on one XCD,
-// cross-workgroup data need only be coherent in that XCD's L2: stores are
-// plain (the CU's L1 is write-through) and loads are buffer loads with the
-// sc0 bit, which miss in L1 and are served by L2.  The agent-scope forms
-// above instead miss in L2 as well.  The buffer resource is built from the
-// first active lane's address minus 2^31, so the 32-bit offsets of one
-// wave's accesses reach 2 GiB either side of it (a wave's accesses of a pass
-// lie within one field).
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-struct LocalRef {
-  __amdgpu_buffer_rsrc_t r;
-  uint32_t off;  // this lane's byte offset from the resource base
-};
-__device__ __forceinline__ LocalRef local_ref(const void* p) {
-  const uint64_t a = (uint64_t)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-  uint64_t base = ((uint64_t)hi << 32) | lo;
-  base = base >= 0x80000000ull ? base - 0x80000000ull : 0ull;
-  return {__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, -1, 0x00020000), (uint32_t)(a - base)};
+// The cluster barrier: the flag form for a cluster on one XCD, the
+// counter form otherwise.
+__device__ __forceinline__ bool cluster_sync(Run& R) {
+  drain_stores();
+  __syncthreads();
+  if (R.G == 1) return true;
+  R.gen += 1u;
+  if (R.loc)
+    flag_sync(R);
+  else
+    counter_sync(R);
+  __syncthreads();
+  return *R.fail == 0;
 }
+
+// run-state accessors: XCD-local or device-coherent (local_ref above)
 __device__ __forceinline__ float2 lload(const LocalRef& L, uint32_t byte_off) {
   const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off + byte_off, 0, 1);
   return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
@@ -1256,7 +1317,6 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   R.xb = s_x + R.ipr * NP;
   R.fail = s_fail;
   R.loc = false;
-  R.grp = blockIdx.x & 7;
   for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
 }
@@ -1341,31 +1401,26 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
     case 16:
       init_run<16, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      R.grp = blk & 7;
       run_iterations<16, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 32:
       init_run<32, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      R.grp = blk & 7;
       run_iterations<32, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 64:
       init_run<64, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      R.grp = blk & 7;
       run_iterations<64, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 128:
       init_run<128, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      R.grp = blk & 7;
       run_iterations<128, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
     case 256:
       init_run<256, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      R.grp = blk & 7;
       run_iterations<256, FAM, BATCH_CW>(R, it0, it1, jb, je);
       break;
   }
@@ -1379,6 +1434,8 @@ __global__ void pty_reset_ctrl_kernel(const DevInst* __restrict__ insts, unsigne
   unsigned* m = insts[blockIdx.x].mx;
   const int nm = 4 * SLOT_WORDS * insts[blockIdx.x].G;
   for (int i = threadIdx.x; i < nm; i += blockDim.x) m[i] = 0u;
+  unsigned* b = insts[blockIdx.x].bar;
+  for (int i = threadIdx.x; i < insts[blockIdx.x].G; i += blockDim.x) b[i] = 0u;
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < nclaim; i += blockDim.x) claim[i] = 0u;
 }
@@ -1856,6 +1913,8 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     d.ctrl = (unsigned*)mem;
     if ((rc = dalloc(p, &mem, (size_t)4 * SLOT_WORDS * G * sizeof(unsigned)))) goto fail;
     d.mx = (unsigned*)mem;
+    if ((rc = dalloc(p, &mem, (size_t)G * sizeof(unsigned)))) goto fail;
+    d.bar = (unsigned*)mem;
     if ((rc = dalloc(p, &mem, (size_t)2 * a.F * G * sizeof(double)))) goto fail;
     d.errpart = (double*)mem;
     if (prop_asp(a.propagator)) {
@@ -1931,9 +1990,6 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
         m[b] = make_int2(i, g);
       }
       used[x] += d.G;
-      for (int y = 0; y < 8; ++y) d.gsize[y] = 0;
-      d.gsize[x] = d.G;
-      d.ngroups = 1;
     }
     int nslots = 0;
     for (int y = 0; y < N_XCD; ++y) nslots = used[y] > nslots ? used[y] : nslots;
