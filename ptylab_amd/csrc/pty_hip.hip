@@ -112,6 +112,8 @@ struct DevInst {
   unsigned* ctrl;    // control block, CTRL_* layout
   unsigned* mx;      // per-position maxima: 4 ring slots x SLOT_WORDS words x G workgroup partials
   unsigned* bar;     // flag barrier: generation reached by each workgroup (G words)
+  unsigned osel[32]; // fields whose object terms the owner of field f's object sums (bit g)
+  unsigned psel[32]; // fields whose probe terms the owner of field f's probe sums (bit g)
   int gsize[8];      // counter barrier: workgroups of this instance in each block group b % 8
   int ngroups;       // counter barrier: non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
@@ -508,7 +510,7 @@ struct Cfg {
 struct FieldIdx {
   int il, io, ip, is, fo, fp;
 };
-__device__ __forceinline__ FieldIdx field_idx(int f, int no, int npm, int ns) {
+__host__ __device__ __forceinline__ FieldIdx field_idx(int f, int no, int npm, int ns) {
   FieldIdx x;
   x.is = f % ns;
   x.ip = (f / ns) % npm;
@@ -803,6 +805,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     (void)frw;
     const FieldIdx x = field_idx(f, no, npm, ns);
+    const unsigned osel = D.osel[f], psel = D.psel[f];
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], of[L], pf[L];
@@ -858,11 +861,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g) {
-          const FieldIdx y = field_idx(g, no, npm, ns);
-          if (engine == PTY_EPIE || qn ? y.io == x.io : y.fo == x.fo)
-            acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
-        }
+        for (int g = 0; g < nF; ++g)
+          if ((osel >> g) & 1u) acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
         nv[l] = cadd(of[l], cscale(acc, bO));
       }
       if (qn) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
@@ -875,12 +875,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g) {
-          const FieldIdx y = field_idx(g, no, npm, ns);
-          const bool sel = engine == PTY_EPIE ? y.ip == x.ip
-                                              : (y.ip == x.ip && y.is == x.is && (engine == PTY_MULTIPIE || y.il == x.il));
-          if (sel) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
-        }
+        for (int g = 0; g < nF; ++g)
+          if ((psel >> g) & 1u) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
         nv[l] = cadd(pf[l], cscale(acc, engine == PTY_EPIE ? D.a.betaProbe : stepP));
       }
       store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
@@ -926,6 +922,7 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     (void)frw;
     const FieldIdx x = field_idx(f, no, npm, ns);
+    const unsigned psel = D.psel[f];
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], op[L], t[L];
     load_row<NP>(R, d, D.W + f * Np2 + prow_off, R.t);
@@ -945,10 +942,8 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g) {
-          const FieldIdx y = field_idx(g, no, npm, ns);
-          if (y.ip == x.ip) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
-        }
+        for (int g = 0; g < nF; ++g)
+          if ((psel >> g) & 1u) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
         nv[l] = cadd(pf[l], cscale(acc, bP));
       }
       store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
@@ -1773,6 +1768,7 @@ int validate(const pty_instance& a, int idx, int nt) {
     const int nF = a.nlambda * a.nosm * a.npsm * a.nslice;
     if (nF > slots)
       return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed %d for Np=%d", idx, nF, slots, a.Np);
+    if (nF > 32) return set_err(PTY_EINVAL, "instance %d: %d exit-wave fields exceed 32", idx, nF);
     if ((a.engine == PTY_MPIE || a.engine == PTY_MULTIPIE) && a.momentum_mask &&
         (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
       return set_err(PTY_EINVAL, "instance %d: momentum needs buffer and momentum arrays", idx);
@@ -1873,6 +1869,20 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     }
     d.nFp = 1;
     while (d.nFp < d.nF) d.nFp *= 2;
+    if (a.engine != PTY_E3PIE) {
+      // the field sums of the object and probe updates (ePIE.py:128-160,
+      // mPIE.py:226-270, multiPIE.py:153-205, qNewton.py:116-140)
+      const bool full = a.engine == PTY_EPIE || a.engine == PTY_QNEWTON;
+      for (int f = 0; f < d.nF; ++f) {
+        const FieldIdx x = field_idx(f, a.nosm, a.npsm, a.nslice);
+        for (int g = 0; g < d.nF; ++g) {
+          const FieldIdx y = field_idx(g, a.nosm, a.npsm, a.nslice);
+          if (full ? y.io == x.io : y.fo == x.fo) d.osel[f] |= 1u << g;
+          if (full ? y.ip == x.ip : (y.ip == x.ip && y.is == x.is && (a.engine == PTY_MULTIPIE || y.il == x.il)))
+            d.psel[f] |= 1u << g;
+        }
+      }
+    }
     d.nO = a.nlambda * a.nosm * a.nslice;
     d.nPr = a.nlambda * a.npsm * a.nslice;
     d.omAlphaO = (float)(1.0 - (double)a.alphaObject);
