@@ -33,6 +33,12 @@ __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 
+// Complex product with fused multiply-adds (two roundings per component
+// instead of three); used inside the transforms only.
+__device__ __forceinline__ float2 cmul_f(float2 a, float2 b) {
+  return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
+}
+
 __device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
   return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
@@ -62,19 +68,25 @@ __device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __
         const float2 w = tw[i * (L / (2 * sr))];
         const float2 a = v[l], b = v[l + sr];
         v[l] = cadd(a, b);
-        v[l + sr] = cmul(csub(a, b), w);
+        v[l + sr] = cmul_f(csub(a, b), w);
       }
     }
   }
+  // Cross-lane stages without divergence: the lower lane of a pair keeps
+  // x + p, the upper one (p - x) w; both are (p + sgn x) w' with sgn = -1,
+  // w' = w in the upper lane and sgn = 1, w' = 1 in the lower one.
 #pragma unroll
   for (int s = LPF / 2; s >= 1; s >>= 1) {
     const bool up = (t & s) != 0;
-    const float2 w = tw[(t & (s - 1)) * (N / (2 * s))];
+    const float2 w0 = tw[(t & (s - 1)) * (N / (2 * s))];
+    const float2 w = up ? w0 : make_float2(1.f, 0.f);
+    const float sgn = up ? -1.f : 1.f;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const float2 x = v[l];
       const float2 p = shfl_xor2(x, s * LS);
-      v[l] = up ? cmul(csub(p, x), w) : cadd(x, p);
+      const float2 y = make_float2(__builtin_fmaf(sgn, x.x, p.x), __builtin_fmaf(sgn, x.y, p.y));
+      v[l] = cmul_f(y, w);
     }
   }
 }
@@ -87,15 +99,20 @@ __device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __
   unsigned off = 0;
   __asm__ volatile("" : "+s"(off));
   const float2* __restrict__ tw = tw_ + off;
+  // cross-lane stages without divergence: the upper lane of a pair first
+  // multiplies by w (the lower one by 1), then keeps p - x, the lower x + p;
+  // both are p + sgn x
 #pragma unroll
   for (int s = 1; s < LPF; s <<= 1) {
     const bool up = (t & s) != 0;
-    const float2 w = conjf2(tw[(t & (s - 1)) * (N / (2 * s))]);
+    const float2 w0 = conjf2(tw[(t & (s - 1)) * (N / (2 * s))]);
+    const float2 w = up ? w0 : make_float2(1.f, 0.f);
+    const float sgn = up ? -1.f : 1.f;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      const float2 x = up ? cmul(v[l], w) : v[l];
+      const float2 x = cmul_f(v[l], w);
       const float2 p = shfl_xor2(x, s * LS);
-      v[l] = up ? csub(p, x) : cadd(x, p);
+      v[l] = make_float2(__builtin_fmaf(sgn, x.x, p.x), __builtin_fmaf(sgn, x.y, p.y));
     }
   }
 #pragma unroll
@@ -106,7 +123,7 @@ __device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __
         const int i = (l & (sr - 1)) * LPF + t;
         const float2 w = conjf2(tw[i * (L / (2 * sr))]);
         const float2 a = v[l];
-        const float2 b = cmul(v[l + sr], w);
+        const float2 b = cmul_f(v[l + sr], w);
         v[l] = cadd(a, b);
         v[l + sr] = csub(a, b);
       }
