@@ -13,6 +13,7 @@ they are NumPy arrays again, as after the reference's _move_data_to_cpu
 import logging
 
 import numpy as np
+from pathlib import Path
 
 from .utils import circ
 
@@ -174,6 +175,78 @@ class Reconstruction:
 
     def initializeProbeMomentum(self):
         self.probeMomentum = np.zeros_like(self.initialGuessProbe)
+
+    # results on disk (Reconstruction.py:495-576) ------------------------------
+    _SAVE_TYPES = ["all", "object", "probe"]
+
+    def _results(self, type, squeeze):
+        if type not in self._SAVE_TYPES:
+            raise NotImplementedError(f"Only {self._SAVE_TYPES} are allowed keywords for type")
+        sq = np.squeeze if squeeze else (lambda x: x)
+        c64 = lambda x: np.asarray(x, dtype=np.complex64)  # noqa: E731
+        f32 = lambda x: np.asarray(x, dtype=np.float32)  # noqa: E731
+        if type == "probe":
+            return {"probe": c64(sq(np.asarray(self.probe)))}
+        if type == "object":
+            return {"object": c64(sq(np.asarray(self.object)))}
+        out = {
+            "probe": c64(self.probe),
+            "object": c64(self.object),
+            "error": f32(getattr(self, "error", [])),
+            "zo": f32(self._zo),
+            "wavelength": f32(self.wavelength),
+            "dxp": f32(self.dxp),
+            "purityProbe": f32(self.purityProbe),
+            "purityObject": f32(self.purityObject),
+            "I object": f32(np.abs(np.asarray(self.object))),
+            "I probe": f32(np.abs(np.asarray(self.probe))),
+            "encoder_corrected": np.asarray(self.encoder_corrected),
+        }
+        if getattr(self, "theta", None) is not None:
+            out["theta"] = f32(self.theta)
+        return out
+
+    def saveResults(self, fileName="recent", type="all", squeeze=False):
+        """Save the reconstruction with the reference's dataset names and
+        dtypes: an HDF5 file when h5py is importable and the name does not end
+        in .npz, else an .npz archive (".npz" appended when missing).  Returns
+        the path written."""
+        data = self._results(type, squeeze)
+        self._move_data_to_cpu()
+        path = Path(fileName)
+        h5 = path.suffix != ".npz"
+        if h5:
+            try:
+                import h5py
+            except ImportError:
+                h5 = False
+        if h5:
+            with h5py.File(path, "w") as hf:
+                for k, v in data.items():
+                    hf.create_dataset(k, data=v)
+            return path
+        if path.suffix != ".npz":
+            path = path.with_name(path.name + ".npz")
+        np.savez(path, **data)
+        return path
+
+    def load(self, filename):
+        """Load what saveResults wrote (Reconstruction.py:495-507)."""
+        path = Path(filename)
+        if path.suffix == ".npz":
+            with np.load(path, allow_pickle=False) as z:
+                src = {k: z[k] for k in z.files}
+        else:
+            import h5py
+
+            with h5py.File(path, "r") as hf:
+                src = {k: np.array(hf[k]) for k in hf.keys()}
+        for k in ("probe", "object", "error", "wavelength", "dxp", "purityProbe", "purityObject"):
+            setattr(self, k, np.array(src[k]))
+        # the zo setter recomputes dxp from the loaded wavelength, as the reference's does
+        self.zo = np.array(src["zo"])
+        if "theta" in src:
+            self.theta = np.array(src["theta"])
 
     # host/device movement -----------------------------------------------------
     def _move_data_to_cpu(self):

@@ -151,3 +151,41 @@ def test_fresnel_phase_and_polychrome_transfer_functions_match_oracle():
     np.testing.assert_array_equal(tfs, np.fft.ifftshift(ref, axes=(-2, -1)).astype(np.complex64))
     # distinct wavelengths give distinct transfer functions
     assert not np.array_equal(tfs[0], tfs[2])
+
+
+@pytest.mark.parametrize("kind", ["all", "object", "probe"])
+def test_save_results_round_trip(tmp_path, kind):
+    """saveResults writes the reference's dataset names and dtypes
+    (Reconstruction.py:509-566); without h5py the archive is an .npz."""
+    d = regression_dataset()
+    data = make_data(d)
+    rec = P.Reconstruction(data, P.Params())
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    rec.object, rec.probe = rec.initialGuessObject, rec.initialGuessProbe
+    rec.error = [3.0, 2.0]
+    path = rec.saveResults(str(tmp_path / "recent.npz"), type=kind, squeeze=True)
+    with np.load(path, allow_pickle=False) as z:
+        keys = set(z.files)
+        if kind == "all":
+            assert {"probe", "object", "error", "zo", "wavelength", "dxp", "purityProbe", "purityObject",
+                    "I object", "I probe", "encoder_corrected"} <= keys
+            assert z["object"].dtype == np.complex64 and z["error"].dtype == np.float32
+            np.testing.assert_array_equal(z["I probe"], np.abs(rec.probe).astype(np.float32))
+        else:
+            assert keys == {kind}
+            np.testing.assert_array_equal(z[kind], np.squeeze(getattr(rec, kind)).astype(np.complex64))
+    if kind == "all":
+        other = P.Reconstruction(data, P.Params())
+        other.load(path)
+        np.testing.assert_array_equal(other.object, rec.object)
+        np.testing.assert_array_equal(other.probe, rec.probe)
+        np.testing.assert_array_equal(other.error, np.float32([3.0, 2.0]))
+        assert np.isclose(other.dxp, rec.dxp, rtol=1e-6)
+
+
+def test_save_results_rejects_unknown_type(tmp_path):
+    d = regression_dataset()
+    rec = P.Reconstruction(make_data(d), P.Params())
+    with pytest.raises(NotImplementedError):
+        rec.saveResults(str(tmp_path / "x.npz"), type="everything")
