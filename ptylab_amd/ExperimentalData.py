@@ -44,9 +44,20 @@ class ExperimentalData:
 
     def loadData(self, filename):
         """Load the reference's HDF5 layout (root datasets ptychogram, encoder,
-        dxd, zo, wavelength[, entrancePupilDiameter, ...]) or an .npz with the
-        same keys."""
+        dxd, zo, wavelength[, entrancePupilDiameter, ..., orientation]) or an
+        .npz with the same keys; "test:nodata" sets the reference's minimal
+        stub (ExperimentalData.py:101-116)."""
+        if str(filename) == "test:nodata":
+            self.filename = filename
+            self.ptychogram = np.zeros((1, 16, 16), dtype=np.float32)
+            self.encoder = np.zeros((1, 2), dtype=np.float64)
+            self.wavelength, self.dxd, self.zo = 500e-9, 6.5e-6, 0.1
+            for k in self.optionalFieldsCPM:
+                setattr(self, k, None)
+            self._setData()
+            return
         path = Path(filename)
+        self.filename = filename
         if not path.exists():
             raise FileNotFoundError(f"Could not find file {path}.")
         if path.suffix == ".npz":
@@ -67,8 +78,34 @@ class ExperimentalData:
             setattr(self, k, _scalify(src[k]) if k in src else None)
         enc = np.asarray(self.encoder)
         if enc.shape[0] < enc.shape[1]:
+            # io/readHdf5.py:68-75: (2, F) encoders are transposed and centred
             self.encoder = enc.T - enc.T.mean(axis=0, keepdims=True)
         self._setData()
+        # last, as in ExperimentalData.py:147-149: it may change the frames
+        orientation = None
+        if "orientation" in src:
+            orientation = _scalify(np.asarray(src["orientation"]).ravel()[0])
+        self.setOrientation(orientation)
+
+    def setOrientation(self, orientation, force_contiguous=True):
+        """ptypy's orientation codes 0-7 (ExperimentalData.py:192-232): bit 2
+        transposes the frames, then bit 0 flips columns and bit 1 rows."""
+        if orientation is None:
+            return
+        if not isinstance(orientation, int):
+            raise TypeError("Orientation value is not valid.")
+        if orientation == 0:
+            return
+        if not 0 < orientation <= 7:
+            raise ValueError(f"Orientation {orientation} is not implemented")
+        if orientation & 4:
+            self.ptychogram = np.transpose(self.ptychogram, (0, 2, 1))
+        if orientation & 1:
+            self.ptychogram = np.flip(self.ptychogram, axis=-1)
+        if orientation & 2:
+            self.ptychogram = np.flip(self.ptychogram, axis=-2)
+        if force_contiguous:
+            self.ptychogram = np.ascontiguousarray(self.ptychogram)
 
     def _setData(self):
         """Derived fields (ExperimentalData.py:234-250)."""

@@ -189,3 +189,31 @@ def test_save_results_rejects_unknown_type(tmp_path):
     rec = P.Reconstruction(make_data(d), P.Params())
     with pytest.raises(NotImplementedError):
         rec.saveResults(str(tmp_path / "x.npz"), type="everything")
+
+
+def test_load_data_npz_transposed_encoder_and_orientation(tmp_path):
+    """loadData with the reference's root names: a (2, F) encoder is
+    transposed and centred (io/readHdf5.py:68-75) and the orientation code
+    flips / transposes the frames (ExperimentalData.py:192-232)."""
+    rng = np.random.default_rng(0)
+    frames = rng.random((3, 8, 8)).astype(np.float32)
+    enc = rng.random((2, 3))
+    np.savez(tmp_path / "d.npz", ptychogram=frames, encoder=enc, dxd=np.float64(5e-6), zo=np.float64(0.05),
+             wavelength=np.float64(632.8e-9), orientation=np.array([5]))
+    data = P.ExperimentalData()
+    data.loadData(tmp_path / "d.npz")
+    np.testing.assert_allclose(data.encoder, enc.T - enc.T.mean(axis=0, keepdims=True))
+    # orientation 5: transpose, then flip columns
+    np.testing.assert_array_equal(data.ptychogram, np.flip(np.transpose(frames, (0, 2, 1)), axis=-1))
+    assert data.numFrames == 3 and data.Nd == 8
+    for code, want in [(1, frames[:, :, ::-1]), (2, frames[:, ::-1, :]), (3, frames[:, ::-1, ::-1]),
+                       (4, frames.transpose(0, 2, 1)), (6, frames.transpose(0, 2, 1)[:, ::-1, :]),
+                       (7, frames.transpose(0, 2, 1)[:, ::-1, ::-1])]:
+        data.ptychogram = frames
+        data.setOrientation(code)
+        np.testing.assert_array_equal(data.ptychogram, want)
+    with pytest.raises(ValueError):
+        data.setOrientation(8)
+    stub = P.ExperimentalData()
+    stub.loadData("test:nodata")
+    assert stub.ptychogram.shape == (1, 16, 16) and stub.zo == 0.1
