@@ -126,7 +126,11 @@ const char* pty_last_error(void);
  * 613-635) and the engines' initializeReconstructionParams.  Validates every
  * shape, allocates the per-instance workspace, permutes each ptychogram and
  * transfer function into the kernel's bit-reversed layout, uploads the
- * descriptors.  Synchronises the device once (set-up only).
+ * descriptors.  In the batched launch every cluster runs whole on one XCD:
+ * an instance's `workgroups` may not exceed one XCD's co-resident share
+ * (pty_coresident_workgroups / 8), and the clusters must pack into the 8
+ * XCDs (largest first, each to the least-loaded XCD); PTY_EINVAL otherwise.
+ * Synchronises the device once (set-up only).
  */
 int pty_plan_create(pty_plan** plan, const pty_instance* insts, int32_t ninst, void* hip_stream);
 
@@ -147,8 +151,9 @@ int pty_plan_run(pty_plan* plan, int32_t it_begin, int32_t it_end, void* hip_str
 int pty_plan_run_positions(pty_plan* plan, int32_t it, int32_t j_begin, int32_t j_end, void* hip_stream);
 
 /*
- * Read the plan's device-side failure flag; synchronises the stream.
- * Returns PTY_ETIMEDOUT if a cluster barrier gave up.
+ * Read the plan's device-side failure flags; synchronises the stream.
+ * Returns PTY_ETIMEDOUT if a cluster barrier gave up, PTY_EDEVICE if a
+ * workgroup of the batched launch found no slot on its XCD.
  */
 int pty_plan_check(pty_plan* plan, void* hip_stream);
 
@@ -170,7 +175,8 @@ int pty_plan_use_graph(pty_plan* plan, int32_t enable);
  */
 int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count);
 
-/* Number of workgroups the plan launches per iteration (host query). */
+/* Number of workgroups that work per iteration (host query; blocks left
+ * idle by the XCD packing of the batched launch are not counted). */
 int pty_plan_workgroups(const pty_plan* plan, int32_t* total);
 
 /* Release a plan (device-synchronising). */
