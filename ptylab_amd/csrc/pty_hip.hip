@@ -107,6 +107,7 @@ struct DevInst {
   const float* Ik;   // ptychogram in kernel layout
   const float2* HT;  // transfer function(s), bit-reversed layout, / Np^2; one per wavelength
   const float2* Q;   // Fresnel quadratic phase (Np, Np), natural order, or null
+  const unsigned char* live;  // ASP: per transfer function, 1 where column p of HT has a nonzero entry; or null
   int fpl;           // W fields per transfer function (fields of one wavelength)
   float2* OP;        // qNewton: the object patch before the position's update (nO, Np, Np)
   unsigned* ctrl;    // control block, CTRL_* layout
@@ -701,6 +702,17 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
 
+// Row elements in columns that the column pass skipped (outside the band
+// limit of field f's transfer function) are zero.
+template <int NP>
+__device__ __forceinline__ void zero_dead_columns(const DevInst& D, int f, float2 (&v)[Line<NP>::L], int t) {
+  if (!D.live) return;
+  const unsigned char* lv = D.live + (size_t)(f / D.fpl) * NP;
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l)
+    if (!lv[l * Line<NP>::LPF + t]) v[l] = make_float2(0.f, 0.f);
+}
+
 // Column pass of an angular-spectrum propagation: forward column transform,
 // multiply by the (bit-reversed, 1/Np^2-scaled) transfer function or its
 // conjugate, inverse column transform.  Fields are independent.
@@ -717,6 +729,9 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
     (void)frw;
     // the transfer function of field f's wavelength
     const float2* __restrict__ HT = D.HT + (size_t)(f / D.fpl) * Np2;
+    // a column outside the transfer function's band limit is zero after the
+    // product; it is neither read nor written (the next row pass zeroes it)
+    if (D.live && __builtin_amdgcn_ballot_w64(D.live[(f / D.fpl) * NP + p] != 0) == 0) continue;
     float2 v[CL];
     load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
     fft_fwd_colg<NP, CWR>(v, R.tw, rho);
@@ -747,6 +762,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     float2 v[L];
     float im[L];
     load_row<NP>(R, v, W + f * Np2 + (size_t)r * NP, R.t);
+    zero_dead_columns<NP>(D, f, v, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
     fft_inv<NP>(v, R.tw, R.t);
@@ -812,6 +828,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
     load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
     load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+    if constexpr (ASPF) zero_dead_columns<NP>(D, f, d, R.t);
     fft_inv<NP>(d, R.tw, R.t);
     float2 tO[L], tP[L];
 #pragma unroll
@@ -1453,6 +1470,22 @@ __global__ void pty_permute_frames_kernel(const float* __restrict__ src, float* 
 }
 
 // ht[n][p] = tf[brev(n)][brev(p)] / np^2 (np^2 is a power of two: exact)
+// live[t][p] = 1 when column p of transfer function t (kernel layout) has a
+// nonzero entry
+__global__ void pty_live_columns_kernel(const float2* __restrict__ ht, unsigned char* __restrict__ live, int np,
+                                        int ntf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntf * np) return;
+  const int t = i / np, p = i - t * np;
+  const float2* h = ht + (size_t)t * np * np;
+  unsigned char any = 0;
+  for (int r = 0; r < np; ++r) {
+    const float2 v = h[(size_t)r * np + p];
+    any |= (v.x != 0.f || v.y != 0.f);
+  }
+  live[i] = any;
+}
+
 __global__ void pty_permute_tf_kernel(const float2* __restrict__ tf, float2* __restrict__ ht, int np, int lg) {
   const int total = np * np;
   const float sc = 1.0f / ((float)np * (float)np);
@@ -1947,6 +1980,12 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
                                                    np, log2_np(np));
       HIP_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
+      if (prop_asp(a.propagator) && a.engine != PTY_E3PIE) {
+        if ((rc = dalloc(p, &mem, (size_t)ntf * np))) goto fail;
+        pty_live_columns_kernel<<<(ntf * np + 255) / 256, 256, 0, st>>>(d.HT, (unsigned char*)mem, np, ntf);
+        HIP_TRY(hipGetLastError());
+        d.live = (const unsigned char*)mem;
+      }
     }
     if (a.n_iter > p->max_iter) p->max_iter = a.n_iter;
     int ci = -1;
