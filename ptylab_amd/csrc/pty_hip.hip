@@ -458,7 +458,10 @@ __device__ __forceinline__ float2 ldg2(const float2* p) {
 template <int NP, int CWR = 0>
 struct Col {
 #ifndef PTY_COLUMN_WIDTH
-  static constexpr int CWD = NP >= 256 ? 2 : 1;
+  // measured on Np = 256 clusters of 128 workgroups: CW = 1 41.2 ms (ASP
+  // with band-limited columns skipped) and 32.1 ms (Fraunhofer) per
+  // 1000-position iteration, CW = 2 46.3 and 32.3, CW = 4 51.6 (ASP)
+  static constexpr int CWD = NP >= 512 ? 2 : 1;
 #else
   static constexpr int CWD = PTY_COLUMN_WIDTH;
 #endif
