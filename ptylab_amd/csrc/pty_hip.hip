@@ -1205,7 +1205,8 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // per-position drivers
 // ---------------------------------------------------------------------------
 template <int NP, bool ASP, int CWR, bool QN>
-__device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
+__device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
+                                             bool sync_after) {
   const DevInst& D = *R.D;
   pie_rows_first<NP, QN, ASP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
@@ -1221,7 +1222,7 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     if (!cluster_sync(R)) return false;
   }
   pie_rows_update<NP, QN, ASP>(R, k, slot);
-  if (!cluster_sync(R)) return false;
+  if (sync_after && !cluster_sync(R)) return false;
   if constexpr (QN) {
     qn_omax_pass<NP>(R, slot);
     if (!cluster_sync(R)) return false;
@@ -1277,14 +1278,35 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       const int k = order[j];
       unsigned* slot = D.mx + (size_t)(q & 3u) * SLOT_WORDS * R.G;
       unsigned* zslot = D.mx + (size_t)((q + 2u) & 3u) * SLOT_WORDS * R.G;
+      const bool mom = FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j];
+      // The barrier after the update pass orders this position's object
+      // writes before the next position's patch reads by other workgroups;
+      // every other cross-workgroup dependency between the two (probe rows,
+      // W rows, maxima, error partials) stays inside one workgroup or is
+      // ordered by an earlier barrier.  When the next patch in this launch
+      // does not overlap this one and no momentum step follows, the PIE
+      // engines skip it.
+      bool sync_after = true;
+      if constexpr (FAM == FAM_PIE_FRAUNHOFER || FAM == FAM_PIE_ASP) {
+        int kn = -1;
+        if (j + 1 < j1)
+          kn = order[j + 1];
+        else if (j1 == F && it + 1 < it1)
+          kn = D.a.order[(size_t)(it + 1) * F];
+        if (!mom && kn >= 0) {
+          const int dr = D.a.positions[2 * k] - D.a.positions[2 * kn];
+          const int dc = D.a.positions[2 * k + 1] - D.a.positions[2 * kn + 1];
+          sync_after = !(dr >= NP || -dr >= NP || dc >= NP || -dc >= NP);
+        }
+      }
       bool ok;
       if constexpr (FAM == FAM_E3PIE)
         ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
       else
         ok = pie_position<NP, FAM == FAM_PIE_ASP || FAM == FAM_QN_ASP, CWR,
-                          FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP>(R, k, j, slot, zslot, ep);
+                          FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP>(R, k, j, slot, zslot, ep, sync_after);
       if (!ok) return;
-      if (FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j]) {
+      if (mom) {
         momentum_pass<NP>(R);
         if (!cluster_sync(R)) return;
       }
