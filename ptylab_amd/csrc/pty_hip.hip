@@ -581,16 +581,23 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
+      // fields outer, elements inner: the L reads of one field are in
+      // flight together (sums still in increasing field order)
+      float sp[L], so[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) sp[l] = so[l] = 0.f;
+      for (int g = 0; g < nF; ++g) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          const float2 c = R.xa[(lslot + g) * NP + l * LPF + R.t];
+          sp[l] += c.x;
+          so[l] += c.y;
+        }
+      }
 #pragma unroll
       for (int l = 0; l < L; ++l) {
-        float sp = 0.f, so = 0.f;
-        for (int g = 0; g < nF; ++g) {
-          const float2 c = R.xa[(lslot + g) * NP + l * LPF + R.t];
-          sp += c.x;
-          so += c.y;
-        }
-        pmax = fmaxf(pmax, sp);
-        omax = fmaxf(omax, so);
+        pmax = fmaxf(pmax, sp[l]);
+        omax = fmaxf(omax, so[l]);
       }
     }
     __syncthreads();
@@ -689,13 +696,18 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
       xs[(islot * CL + l) * 64 + R.lane] = act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f;
     }
     __syncthreads();
+    float iest[CL];
+#pragma unroll
+    for (int l = 0; l < CL; ++l) iest[l] = 0.f;
+    for (int g = 0; g < nF; ++g) {
+#pragma unroll
+      for (int l = 0; l < CL; ++l) iest[l] += xs[((lslot + g) * CL + l) * 64 + R.lane];
+    }
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
       const float im = ldg(Ik + (size_t)(l * LPC + rho) * NP + p);
-      float iest = 0.f;
-      for (int g = 0; g < nF; ++g) iest += xs[((lslot + g) * CL + l) * 64 + R.lane];
-      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im - iest);
-      v[l] = cscale(v[l], sqrtf(im / (iest + 1e-10f)));
+      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im - iest[l]);
+      v[l] = cscale(v[l], sqrtf(im / (iest[l] + 1e-10f)));
     }
     fft_inv_colg<NP, CWR>(v, R.tw, rho);
     if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
@@ -706,14 +718,23 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
 }
 
 // Row elements in columns that the column pass skipped (outside the band
-// limit of field f's transfer function) are zero.
+// limit of field f's transfer function) are zero.  live_bits reads the
+// column flags of this lane's elements (bit l for element l * LPF + t); it is
+// issued before the row loads so that its latency hides under theirs.
 template <int NP>
-__device__ __forceinline__ void zero_dead_columns(const DevInst& D, int f, float2 (&v)[Line<NP>::L], int t) {
-  if (!D.live) return;
+__device__ __forceinline__ unsigned live_bits(const DevInst& D, int f, int t) {
+  if (!D.live) return ~0u;
   const unsigned char* lv = D.live + (size_t)(f / D.fpl) * NP;
+  unsigned b = 0u;
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) b |= (lv[l * Line<NP>::LPF + t] ? 1u : 0u) << l;
+  return b;
+}
+template <int NP>
+__device__ __forceinline__ void zero_dead_columns(unsigned bits, float2 (&v)[Line<NP>::L]) {
 #pragma unroll
   for (int l = 0; l < Line<NP>::L; ++l)
-    if (!lv[l * Line<NP>::LPF + t]) v[l] = make_float2(0.f, 0.f);
+    if (!((bits >> l) & 1u)) v[l] = make_float2(0.f, 0.f);
 }
 
 // Column pass of an angular-spectrum propagation: forward column transform,
@@ -764,8 +785,9 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     float2 v[L];
     float im[L];
+    const unsigned lb = live_bits<NP>(D, f, R.t);
     load_row<NP>(R, v, W + f * Np2 + (size_t)r * NP, R.t);
-    zero_dead_columns<NP>(D, f, v, R.t);
+    zero_dead_columns<NP>(lb, v);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
     fft_inv<NP>(v, R.tw, R.t);
@@ -774,12 +796,17 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     for (int l = 0; l < L; ++l) q[l] = make_float2(act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f, 0.f);
     lds_put<NP>(R.xa, islot, q, R.t);
     __syncthreads();
+    float iest[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) iest[l] = 0.f;
+    for (int g = 0; g < nF; ++g) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) iest[l] += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
+    }
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      float iest = 0.f;
-      for (int g = 0; g < nF; ++g) iest += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
-      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest);
-      v[l] = cscale(v[l], sqrtf(im[l] / (iest + 1e-10f)));
+      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest[l]);
+      v[l] = cscale(v[l], sqrtf(im[l] / (iest[l] + 1e-10f)));
     }
     fft_fwd<NP>(v, R.tw, R.t);
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
@@ -828,10 +855,12 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], of[L], pf[L];
+    unsigned lb = ~0u;
+    if constexpr (ASPF) lb = live_bits<NP>(D, f, R.t);
     load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
     load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
     load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
-    if constexpr (ASPF) zero_dead_columns<NP>(D, f, d, R.t);
+    if constexpr (ASPF) zero_dead_columns<NP>(lb, d);
     fft_inv<NP>(d, R.tw, R.t);
     float2 tO[L], tP[L];
 #pragma unroll
@@ -877,28 +906,34 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     if (act && x.ip == 0) {
       // object field x.fo: ePIE and qNewton sum over (lambda, psm, slice),
       // the m/multi engines over psm
-      float2 nv[L];
+      float2 nv[L], acc[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g)
-          if ((osel >> g) & 1u) acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
-        nv[l] = cadd(of[l], cscale(acc, bO));
+      for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+      for (int g = 0; g < nF; ++g) {
+        if ((osel >> g) & 1u) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) acc[l] = cadd(acc[l], R.xa[(lslot + g) * NP + l * LPF + R.t]);
+        }
       }
+#pragma unroll
+      for (int l = 0; l < L; ++l) nv[l] = cadd(of[l], cscale(acc[l], bO));
       if (qn) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
       store_row<NP>(R, O + x.fo * No2 + orow_off, nv, R.t);
     }
     if (act && x.io == 0 && !qn) {
       // probe field x.fp: ePIE sums over (lambda, osm, slice), mPIE over osm,
       // multiPIE over (lambda, osm)
-      float2 nv[L];
+      float2 nv[L], acc[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g)
-          if ((psel >> g) & 1u) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
-        nv[l] = cadd(pf[l], cscale(acc, engine == PTY_EPIE ? D.a.betaProbe : stepP));
+      for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+      for (int g = 0; g < nF; ++g) {
+        if ((psel >> g) & 1u) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) acc[l] = cadd(acc[l], R.xb[(lslot + g) * NP + l * LPF + R.t]);
+        }
       }
+#pragma unroll
+      for (int l = 0; l < L; ++l) nv[l] = cadd(pf[l], cscale(acc[l], engine == PTY_EPIE ? D.a.betaProbe : stepP));
       store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
     }
     __syncthreads();
@@ -959,13 +994,17 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
     if (act && x.io == 0) {
       float2 pf[L], nv[L];
       load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      float2 acc[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g)
-          if ((psel >> g) & 1u) acc = cadd(acc, R.xb[(lslot + g) * NP + l * LPF + R.t]);
-        nv[l] = cadd(pf[l], cscale(acc, bP));
+      for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+      for (int g = 0; g < nF; ++g) {
+        if ((psel >> g) & 1u) {
+#pragma unroll
+          for (int l = 0; l < L; ++l) acc[l] = cadd(acc[l], R.xb[(lslot + g) * NP + l * LPF + R.t]);
+        }
       }
+#pragma unroll
+      for (int l = 0; l < L; ++l) nv[l] = cadd(pf[l], cscale(acc[l], bP));
       store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
     }
     __syncthreads();
@@ -1040,12 +1079,15 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
+      float sp[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float sp = 0.f;
-        for (int g = 0; g < nF; ++g) sp += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
-        pmax = fmaxf(pmax, sp);
+      for (int l = 0; l < L; ++l) sp[l] = 0.f;
+      for (int g = 0; g < nF; ++g) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) sp[l] += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
       }
+#pragma unroll
+      for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
 #pragma unroll
       for (int s = 0; s < MAX_SLICES; ++s) {
         if (s < ns) {
@@ -1111,12 +1153,15 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
+      float sp[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float sp = 0.f;
-        for (int g = 0; g < nF; ++g) sp += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
-        pmax = fmaxf(pmax, sp);
+      for (int l = 0; l < L; ++l) sp[l] = 0.f;
+      for (int g = 0; g < nF; ++g) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) sp[l] += R.xa[(lslot + g) * NP + l * LPF + R.t].x;
       }
+#pragma unroll
+      for (int l = 0; l < L; ++l) pmax = fmaxf(pmax, sp[l]);
     }
     __syncthreads();
   ITEMS_END
@@ -1175,13 +1220,15 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
     __syncthreads();
     if (act && f % npm == 0) {
       // object slice su of this lambda: sum over (lambda, psm) (axes (0, 2))
-      float2 nv[L];
+      float2 nv[L], acc[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float2 acc = make_float2(0.f, 0.f);
-        for (int g = 0; g < nF; ++g) acc = cadd(acc, R.xa[(lslot + g) * NP + l * LPF + R.t]);
-        nv[l] = cadd(ov[l], cscale(acc, bO));
+      for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+      for (int g = 0; g < nF; ++g) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) acc[l] = cadd(acc[l], R.xa[(lslot + g) * NP + l * LPF + R.t]);
       }
+#pragma unroll
+      for (int l = 0; l < L; ++l) nv[l] = cadd(ov[l], cscale(acc[l], bO));
       store_row<NP>(R, O + (size_t)((f / npm) * ns + su) * No2 + orow_off, nv, R.t);
     }
     // probe slice su of this field: sum over lambda (axes (0, 1))
