@@ -43,7 +43,7 @@ import numpy as np
 
 __all__ = [
     "fft2c", "ifft2c", "circ", "aspw_transfer", "asp_transfer_stack",
-    "quad_phase", "polychrome_transfer_stack", "propagate", "pixel_positions", "Geometry", "initial_guess",
+    "quad_phase", "polychrome_transfer_stack", "scaled_asp_phases", "propagate", "pixel_positions", "Geometry", "initial_guess",
     "energy_at_pos", "position_order", "orthogonalize_modes", "OracleRun",
     "nrmse", "GEMMA",
 ]
@@ -124,6 +124,30 @@ def quad_phase(z, wavelength, Np, dxp):
     return np.exp(1.0j * np.pi / (wavelength * z) * (X ** 2 + Y ** 2))
 
 
+def scaled_asp_phases(z, wavelength, N, dx, dq):
+    """Q1 (source plane) and Q2 (centred frequency grid) of the scaled
+    angular spectrum method with band limits, float64/complex128
+    (Operators.py:706-760, scaledASP with bandlimit=True, exactSolution=False).
+    dx: source sampling (dxo), dq: destination sampling (dxd)."""
+    k = 2 * np.pi / wavelength
+    N = int(N)
+    x1 = np.arange(-N // 2, N // 2) * dx
+    X1, Y1 = np.meshgrid(x1, x1)
+    r1sq = X1 ** 2 + Y1 ** 2
+    f = np.arange(-N // 2, N // 2) / (N * dx)
+    FX, FY = np.meshgrid(f, f)
+    fsq = FX ** 2 + FY ** 2
+    m = dq / dx
+    Q1 = np.exp(1.0j * k / 2 * (1 - m) / z * r1sq)
+    Q2 = np.exp(-1.0j * np.pi ** 2 * 2 * z / m / k * fsq)
+    if m != 1:
+        r1sq_max = wavelength * z / (2 * dx * (1 - m))
+        Q1 = Q1 * np.array(circ(X1, Y1, 2 * r1sq_max))
+    fsq_max = m / (2 * z * wavelength * (1 / (N * dx)))
+    Q2 = Q2 * np.array(circ(FX, FY, 2 * fsq_max))
+    return Q1, Q2
+
+
 def polychrome_transfer_stack(nosm, npsm, Np, z, spectral_density, Lp):
     """The (nlambda, nosm, npsm, 1, Np, Np) stack of per-wavelength ASP
     transfer functions, centred, complex128 as built (Operators.py:910-952)."""
@@ -137,7 +161,7 @@ def polychrome_transfer_stack(nosm, npsm, Np, z, spectral_density, Lp):
 def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pure=False):
     """object2detector / detector2object (Operators.py:561-584, dispatch
     :1082-1103) for Fraunhofer (:27-74), Fresnel (:77-152), ASP (:155-230) and
-    polychromeASP (:458-558).  pure=True keeps complex64 fields complex64
+    polychromeASP (:458-558) and scaledASP (:313-370).  pure=True keeps complex64 fields complex64
     (the quadratic phase cast to complex64, as a complex64 GPU path does);
     the default follows NumPy's promotion with the reference's complex128
     quadratic phase."""
@@ -181,6 +205,18 @@ def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pu
         if inverse:
             tf = tf.conj()
         return ifft2c(fft2c(fields) * tf)
+    if kind == "scaledasp":
+        if shifted:
+            raise ValueError("scaledASP propagatorType works only with fftshiftSwitch = False!")
+        if geo.nlambda > 1:
+            raise ValueError("For multi-wavelength, scaledPolychromeASP needs to be used instead of scaledASP")
+        # stored as complex64 (Operators.py:955-987), so the chain stays in
+        # the fields' precision
+        Q1, Q2 = scaled_asp_phases(geo.zo, geo.wavelength, geo.Np, geo.dxo, geo.dxd)
+        Q1, Q2 = Q1.astype(np.complex64), Q2.astype(np.complex64)
+        if inverse:
+            return ifft2c(fft2c(fields) * Q2.conj()) * Q1.conj()
+        return ifft2c(fft2c(fields * Q1) * Q2)
     raise NotImplementedError(f"propagator {kind!r} is outside the oracle's scope")
 
 

@@ -2,9 +2,10 @@
 
 ``object2detector`` / ``detector2object`` keep the reference's signature
 ``(fields, params, reconstruction) -> (esw, propagated)`` (Operators.py:561-584)
-and run the Fraunhofer, Fresnel, ASP and polychromeASP propagators on the GPU
-through ``pty_propagate`` (Fresnel multiplies by the quadratic phase around a
-Fraunhofer transform, polychromeASP runs one ASP per wavelength).  Transfer
+and run the Fraunhofer, Fresnel, ASP, polychromeASP and scaledASP propagators
+on the GPU through ``pty_propagate`` (Fresnel multiplies by the quadratic phase
+around a Fraunhofer transform, polychromeASP runs one ASP per wavelength,
+scaledASP is an ASP between two checkerboarded phase multiplies).  Transfer
 functions and the quadratic phase are built on the host in float64 exactly as
 the reference builds them (Operators.py:631-684, 861-893, 910-952,
 _propagation_kernels.py:16-36) and cast to complex64 once; they are cached per
@@ -17,7 +18,7 @@ import numpy as np
 from . import _native
 from .utils import circ
 
-PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp"}
+PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp"}
 
 
 @lru_cache(maxsize=8)
@@ -78,6 +79,50 @@ def polychrome_tf_fftorder(z, spectral_density, N, L):
     return np.ascontiguousarray(np.stack([asp_tf_fftorder(z, float(w), N, L) for w in np.atleast_1d(spectral_density)]))
 
 
+@lru_cache(maxsize=8)
+def _scaled_asp_cached(z, wavelength, N, dx, dq):
+    k = 2 * np.pi / wavelength
+    x1 = np.arange(-N // 2, N // 2) * dx
+    X1, Y1 = np.meshgrid(x1, x1)
+    r1sq = X1 ** 2 + Y1 ** 2
+    f = np.arange(-N // 2, N // 2) / (N * dx)
+    FX, FY = np.meshgrid(f, f)
+    fsq = FX ** 2 + FY ** 2
+    m = dq / dx
+    Q1 = np.exp(1.0j * k / 2 * (1 - m) / z * r1sq)
+    Q2 = np.exp(-1.0j * np.pi ** 2 * 2 * z / m / k * fsq)
+    if m != 1:
+        r1sq_max = wavelength * z / (2 * dx * (1 - m))
+        Q1 = Q1 * np.array(circ(X1, Y1, 2 * r1sq_max))
+    fsq_max = m / (2 * z * wavelength * (1 / (N * dx)))
+    Q2 = Q2 * np.array(circ(FX, FY, 2 * fsq_max))
+    Q1, Q2 = Q1.astype(np.complex64), Q2.astype(np.complex64)
+    Q1.setflags(write=False)
+    Q2.setflags(write=False)
+    return Q1, Q2
+
+
+def scaled_asp_phases(z, wavelength, N, dxo, dxd):
+    """Q1 (source plane) and Q2 (centred frequency grid) of the band-limited
+    scaled angular spectrum method, built in float64 and stored complex64 as
+    the reference stores them (Operators.py:706-760, 955-987).  Forward:
+    ifft2c(fft2c(psi Q1) Q2); inverse: ifft2c(fft2c(Psi) conj Q2) conj Q1."""
+    return _scaled_asp_cached(float(z), float(wavelength), int(N), float(dxo), float(dxd))
+
+
+def scaled_asp_kernel_arrays(z, wavelength, N, dxo, dxd):
+    """The (2, N, N) complex64 pair the kernels take for scaledASP: [0] = Q2 as
+    the transfer function of unshifted transforms, [1] = Q1 times the
+    checkerboard (-1)^(y+x).  For even N, fft2c(x) = c F(c x) and ifft2c(X) =
+    c F^-1(c X) with c = (-1)^(y+x), so ifft2c(fft2c(psi Q1) Q2) =
+    c F^-1(F(c Q1 psi) Q2): the centred grid of Q2 needs no shift, and the
+    detector's checkerboard cancels in the intensity constraint."""
+    Q1, Q2 = scaled_asp_phases(z, wavelength, N, dxo, dxd)
+    idx = np.arange(int(N))
+    c = np.where((idx[:, None] + idx[None, :]) % 2 == 0, 1, -1).astype(np.float32)
+    return np.ascontiguousarray(np.stack([Q2, (Q1 * c).astype(np.complex64)]))
+
+
 def _kind(params):
     kind = params.propagatorType.lower()
     if kind not in PROPAGATORS:
@@ -86,7 +131,9 @@ def _kind(params):
 
 
 def _native_kind(params, kind):
-    if kind in ("asp", "polychromeasp"):
+    if kind == "scaledasp" and params.fftshiftSwitch:
+        raise ValueError("scaledASP propagatorType works only with fftshiftSwitch = False!")
+    if kind in ("asp", "polychromeasp", "scaledasp"):
         if params.fftshiftSwitch:
             raise ValueError("ASP propagator only works with fftshiftswitch == False")
         return _native.PTY_ASP
@@ -120,6 +167,21 @@ def _propagate(fields, params, reconstruction, inverse):
             plane = out[il].contiguous()
             _native.propagate(plane, nk, inverse, tfs[il].contiguous(), stream)
             out[il] = plane
+    elif kind == "scaledasp":
+        if r.nlambda > 1:
+            raise ValueError("For multi-wavelength, scaledPolychromeASP needs to be used instead of scaledASP")
+        pair = torch.as_tensor(scaled_asp_kernel_arrays(r.zo, r.wavelength, r.Np, r.dxo, r.dxd), device=dev)
+        q2, q1c = pair[0].contiguous(), pair[1]
+        idx = torch.arange(int(r.Np), device=dev)
+        cb = (1 - 2 * ((idx[:, None] + idx[None, :]) % 2)).to(torch.complex64)  # (-1)^(y+x)
+        if inverse:
+            out = (out * cb).contiguous()
+            _native.propagate(out, nk, True, q2, stream)
+            out = (out * q1c.conj()).contiguous()
+        else:
+            out = (out * q1c).contiguous()
+            _native.propagate(out, nk, False, q2, stream)
+            out = (out * cb).contiguous()
     elif kind == "fresnel":
         Q = torch.as_tensor(quadratic_phase(r.zo, r.wavelength, r.Np, r.dxp), device=dev)
         if inverse:

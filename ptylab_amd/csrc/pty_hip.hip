@@ -65,7 +65,8 @@ using namespace pty;
 #define FAM_E3PIE 2
 #define FAM_QN_FRAUNHOFER 3  // qNewton: its own instantiations keep the PIE kernels' registers
 #define FAM_QN_ASP 4
-#define N_FAMILIES 5
+#define FAM_PIE_SASP 5  // PIE with scaled ASP: the ASP passes plus a phase in the row passes
+#define N_FAMILIES 6
 
 namespace {
 
@@ -125,7 +126,9 @@ __device__ float2 g_twiddle[6][256];  // exp(-2 pi i j / N), j < N/2, N = 16..51
 // propagators whose detector transform folds fft2c's shifts into a checkerboard
 __host__ __device__ static inline bool prop_checkerboard(int prop) { return prop == PTY_FRAUNHOFER || prop == PTY_FRESNEL; }
 // propagators of the angular-spectrum family (detector = ifft(fft(.) H))
-__host__ __device__ static inline bool prop_asp(int prop) { return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP; }
+__host__ __device__ static inline bool prop_asp(int prop) {
+  return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP || prop == PTY_SCALED_ASP;
+}
 
 __host__ __device__ static int log2_np(int np) {
   switch (np) {
@@ -538,7 +541,7 @@ __device__ __forceinline__ void lds_put(float2* buf, int islot, const float2 (&v
 // probe row, (checkerboard for the centred Fraunhofer transform), forward row
 // transform into W; per pixel sums of |P|^2 over the probe fields and of |O|^2
 // over the object fields give the maxima the updates divide by.
-template <int NP, bool QN, bool ASPF>
+template <int NP, bool QN, bool ASPF, bool QPH>
 __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -570,8 +573,9 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float2 e = cmul(of[l], pf[l]);
-      // Fresnel: fft2c(esw * Q) (Operators.py:110); never in ASP-family kernels
-      if constexpr (!ASPF) {
+      // Fresnel: fft2c(esw * Q) (Operators.py:110); scaled ASP: c Q1 esw
+      // (Operators.py:345); no phase in the other ASP-family kernels
+      if constexpr (!ASPF || QPH) {
         if (Qp) e = cmul(e, ldg2(Qp + (size_t)r * NP + l * LPF + R.t));
       }
       if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
@@ -823,7 +827,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
 // and each object / probe field is written by one owner slot (the field with
 // psm index 0, resp. osm index 0).  Each object pixel of the patch has one
 // owner.
-template <int NP, bool QN, bool ASPF>
+template <int NP, bool QN, bool ASPF, bool QPH>
 __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -871,6 +875,9 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
         if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
         // Fresnel: ifft2c(.) * conj(Q) (Operators.py:150)
         if (Qp) u = cmul(u, conjf2(ldg2(Qp + prow_off + l * LPF + R.t)));
+      } else if constexpr (QPH) {
+        // scaled ASP: ... conj(c Q1) (Operators.py:343-344)
+        u = cmul(u, conjf2(ldg2(Qp + prow_off + l * LPF + R.t)));
       }
       const float2 dl = csub(u, cmul(of[l], pf[l]));
       float wo, wp;
@@ -1251,11 +1258,11 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 // per-position drivers
 // ---------------------------------------------------------------------------
-template <int NP, bool ASP, int CWR, bool QN>
+template <int NP, bool ASP, int CWR, bool QN, bool QPH>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
                                              bool sync_after) {
   const DevInst& D = *R.D;
-  pie_rows_first<NP, QN, ASP>(R, k, slot, zslot);
+  pie_rows_first<NP, QN, ASP, QPH>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
     tf_cols<NP, CWR>(R, false);
@@ -1268,7 +1275,7 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs);
     if (!cluster_sync(R)) return false;
   }
-  pie_rows_update<NP, QN, ASP>(R, k, slot);
+  pie_rows_update<NP, QN, ASP, QPH>(R, k, slot);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (QN) {
     qn_omax_pass<NP>(R, slot);
@@ -1334,7 +1341,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       // does not overlap this one and no momentum step follows, the PIE
       // engines skip it.
       bool sync_after = true;
-      if constexpr (FAM == FAM_PIE_FRAUNHOFER || FAM == FAM_PIE_ASP) {
+      if constexpr (FAM == FAM_PIE_FRAUNHOFER || FAM == FAM_PIE_ASP || FAM == FAM_PIE_SASP) {
         int kn = -1;
         if (j + 1 < j1)
           kn = order[j + 1];
@@ -1350,8 +1357,9 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       if constexpr (FAM == FAM_E3PIE)
         ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
       else
-        ok = pie_position<NP, FAM == FAM_PIE_ASP || FAM == FAM_QN_ASP, CWR,
-                          FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP>(R, k, j, slot, zslot, ep, sync_after);
+        ok = pie_position<NP, FAM == FAM_PIE_ASP || FAM == FAM_QN_ASP || FAM == FAM_PIE_SASP, CWR,
+                          FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP, FAM == FAM_PIE_SASP>(R, k, j, slot, zslot, ep,
+                                                                                         sync_after);
       if (!ok) return;
       if (mom) {
         momentum_pass<NP>(R);
@@ -1726,6 +1734,7 @@ static const void* iterate_kernel_fam(int fam) {
   switch (fam) {
     case FAM_PIE_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_PIE_FRAUNHOFER>;
     case FAM_PIE_ASP: return (const void*)pty_iterate_kernel<NP, FAM_PIE_ASP>;
+    case FAM_PIE_SASP: return (const void*)pty_iterate_kernel<NP, FAM_PIE_SASP>;
     case FAM_E3PIE: return (const void*)pty_iterate_kernel<NP, FAM_E3PIE>;
     case FAM_QN_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_QN_FRAUNHOFER>;
     case FAM_QN_ASP: return (const void*)pty_iterate_kernel<NP, FAM_QN_ASP>;
@@ -1739,6 +1748,7 @@ static const void* iterate_kernel_ptr(int np, int fam) {
       switch (fam) {
         case FAM_PIE_FRAUNHOFER: return (const void*)pty_batch_kernel<FAM_PIE_FRAUNHOFER>;
         case FAM_PIE_ASP: return (const void*)pty_batch_kernel<FAM_PIE_ASP>;
+        case FAM_PIE_SASP: return (const void*)pty_batch_kernel<FAM_PIE_SASP>;
         case FAM_E3PIE: return (const void*)pty_batch_kernel<FAM_E3PIE>;
         case FAM_QN_FRAUNHOFER: return (const void*)pty_batch_kernel<FAM_QN_FRAUNHOFER>;
         case FAM_QN_ASP: return (const void*)pty_batch_kernel<FAM_QN_ASP>;
@@ -1757,6 +1767,7 @@ static const void* iterate_kernel_ptr(int np, int fam) {
 static int family_of(const pty_instance& a) {
   if (a.engine == PTY_E3PIE) return FAM_E3PIE;
   if (a.engine == PTY_QNEWTON) return prop_asp(a.propagator) ? FAM_QN_ASP : FAM_QN_FRAUNHOFER;
+  if (a.propagator == PTY_SCALED_ASP) return FAM_PIE_SASP;
   return prop_asp(a.propagator) ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
 }
 
@@ -1831,7 +1842,7 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 int validate(const pty_instance& a, int idx, int nt) {
   if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
   if (a.engine < PTY_EPIE || a.engine > PTY_QNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
-  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_POLYCHROME_ASP)
+  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_SCALED_ASP)
     return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
   if (log2_np(a.Np) < 0)
     return set_err(PTY_EINVAL, "instance %d: Np=%d unsupported (powers of two 16..512)", idx, a.Np);
@@ -1848,6 +1859,14 @@ int validate(const pty_instance& a, int idx, int nt) {
     return set_err(PTY_EINVAL, "instance %d: polychromeASP needs one transfer function per wavelength", idx);
   if (a.propagator == PTY_FRESNEL && !a.tf)
     return set_err(PTY_EINVAL, "instance %d: Fresnel needs the quadratic phase", idx);
+  if (a.propagator == PTY_SCALED_ASP) {
+    if (!a.tf) return set_err(PTY_EINVAL, "instance %d: scaledASP needs Q2 and the checkerboarded Q1", idx);
+    if (a.nlambda > 1)
+      return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, scaledPolychromeASP is needed instead of scaledASP",
+                     idx);
+    if (a.engine == PTY_QNEWTON || a.engine == PTY_E3PIE)
+      return set_err(PTY_EINVAL, "instance %d: scaledASP runs with the ePIE, mPIE and multiPIE engines", idx);
+  }
   const int slots = slots_rt(a.Np, nt);
   {
     // a column pass gives every field of a column group its own wave in one
@@ -2052,6 +2071,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
                                                    np, log2_np(np));
       HIP_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
+      if (a.propagator == PTY_SCALED_ASP) d.Q = (const float2*)a.tf + np2;  // c Q1, natural order
       if (prop_asp(a.propagator) && a.engine != PTY_E3PIE) {
         if ((rc = dalloc(p, &mem, (size_t)ntf * np))) goto fail;
         pty_live_columns_kernel<<<(ntf * np + 255) / 256, 256, 0, st>>>(d.HT, (unsigned char*)mem, np, ntf);
@@ -2275,7 +2295,7 @@ int pty_plan_use_graph(pty_plan* p, int32_t enable) {
 
 int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count) {
   if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_QNEWTON || propagator < PTY_FRAUNHOFER ||
-      propagator > PTY_POLYCHROME_ASP)
+      propagator > PTY_SCALED_ASP)
     return set_err(PTY_EINVAL, "pty_coresident_workgroups: bad arguments");
   pty_instance a;
   memset(&a, 0, sizeof a);

@@ -99,8 +99,11 @@ def test_unsupported_switches_raise():
     params.propagatorType = "Fresnel"
     with pytest.raises(NotImplementedError):
         eng._prepareReconstruction()
-    params.fftshiftSwitch = False
     params.propagatorType = "scaledASP"
+    with pytest.raises(ValueError):
+        eng._prepareReconstruction()
+    params.fftshiftSwitch = False
+    params.propagatorType = "twoStepPolychrome"
     with pytest.raises(NotImplementedError):
         eng._prepareReconstruction()
     params.propagatorType = "Fraunhofer"
@@ -217,3 +220,31 @@ def test_load_data_npz_transposed_encoder_and_orientation(tmp_path):
     stub = P.ExperimentalData()
     stub.loadData("test:nodata")
     assert stub.ptychogram.shape == (1, 16, 16) and stub.zo == 0.1
+
+
+def test_scaled_asp_phases_and_checkerboard_folding():
+    """The host builds scaledASP's Q1, Q2 as the oracle does (which matches
+    the reference's propagator_scaledasp golden), and the kernels' form
+    c F^-1(F(c Q1 psi) Q2) with c = (-1)^(y+x) and unshifted transforms equals
+    ifft2c(fft2c(psi Q1) Q2) (and the inverse), the identity the GPU path
+    relies on."""
+    from ptylab_amd import Operators as Op
+
+    d = regression_dataset()
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"])
+    Q1, Q2 = Op.scaled_asp_phases(geo.zo, geo.wavelength, geo.Np, geo.dxo, geo.dxd)
+    R1, R2 = O.scaled_asp_phases(geo.zo, geo.wavelength, geo.Np, geo.dxo, geo.dxd)
+    np.testing.assert_array_equal(Q1, R1.astype(np.complex64))
+    np.testing.assert_array_equal(Q2, R2.astype(np.complex64))
+    pair = Op.scaled_asp_kernel_arrays(geo.zo, geo.wavelength, geo.Np, geo.dxo, geo.dxd)
+    n = geo.Np
+    c = (-1.0) ** np.add.outer(np.arange(n), np.arange(n))
+    rng = np.random.default_rng(0)
+    psi = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    q1, q2 = Q1.astype(np.complex128), Q2.astype(np.complex128)
+    want_f = O.ifft2c(O.fft2c(psi * q1) * q2)
+    got_f = c * np.fft.ifft2(np.fft.fft2(pair[1].astype(np.complex128) * psi) * pair[0])
+    assert np.max(np.abs(got_f - want_f)) < 1e-12 * np.max(np.abs(want_f))
+    want_b = O.ifft2c(O.fft2c(psi) * q2.conj()) * q1.conj()
+    got_b = np.fft.ifft2(np.fft.fft2(c * psi) * pair[0].conj()) * pair[1].astype(np.complex128).conj()
+    assert np.max(np.abs(got_b - want_b)) < 1e-12 * np.max(np.abs(want_b))
