@@ -46,6 +46,7 @@ extern "C" {
 #define PTY_FRESNEL 3             /* fft2c(psi * Q), fftshiftSwitch=False (Operators.py:77-152) */
 #define PTY_POLYCHROME_ASP 4      /* ASP with one transfer function per wavelength (Operators.py:458-558) */
 #define PTY_SCALED_ASP 5          /* ifft2c(fft2c(psi Q1) Q2), scaled ASP (Operators.py:313-370) */
+#define PTY_SCALED_POLYCHROME_ASP 6 /* scaled ASP, Q1 and Q2 per wavelength (Operators.py:378-455) */
 
 /*
  * One reconstruction.  Shapes follow the reference's 6D layout
@@ -68,7 +69,9 @@ extern "C" {
  *             (Np, Np) in natural order (_propagation_kernels.py:16-36);
  *             for PTY_SCALED_ASP (2, Np, Np): Q2 as built on its centred
  *             grid (used unshifted) and Q1 (-1)^(y+x) in natural order
- *             (Operators.py:706-760, 955-987; nlambda == 1);
+ *             (Operators.py:706-760, 955-987; nlambda == 1); for
+ *             PTY_SCALED_POLYCHROME_ASP (2 nlambda, Np, Np): the nlambda Q2,
+ *             then the nlambda Q1 (-1)^(y+x) (Operators.py:990-1032);
  *             NULL for Fraunhofer
  *   err_at_pos (n_iter, F) float64 and err_iter (n_iter) float64 outputs
  *             (getErrorMetrics, BaseEngine.py:819-841)

@@ -161,7 +161,8 @@ def polychrome_transfer_stack(nosm, npsm, Np, z, spectral_density, Lp):
 def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pure=False):
     """object2detector / detector2object (Operators.py:561-584, dispatch
     :1082-1103) for Fraunhofer (:27-74), Fresnel (:77-152), ASP (:155-230) and
-    polychromeASP (:458-558) and scaledASP (:313-370).  pure=True keeps complex64 fields complex64
+    polychromeASP (:458-558), scaledASP (:313-370) and scaledPolychromeASP
+    (:378-455).  pure=True keeps complex64 fields complex64
     (the quadratic phase cast to complex64, as a complex64 GPU path does);
     the default follows NumPy's promotion with the reference's complex128
     quadratic phase."""
@@ -214,6 +215,21 @@ def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pu
         # the fields' precision
         Q1, Q2 = scaled_asp_phases(geo.zo, geo.wavelength, geo.Np, geo.dxo, geo.dxd)
         Q1, Q2 = Q1.astype(np.complex64), Q2.astype(np.complex64)
+        if inverse:
+            return ifft2c(fft2c(fields) * Q2.conj()) * Q1.conj()
+        return ifft2c(fft2c(fields * Q1) * Q2)
+    if kind == "scaledpolychromeasp":
+        if shifted:
+            raise ValueError("scaledPolychromeASP propagatorType works only with fftshiftSwitch = False!")
+        # per wavelength of the spectral density, each pair as scaledASP
+        # builds it, stored complex64 (Operators.py:990-1032)
+        sd = np.atleast_1d(geo.spectralDensity)
+        shape = (len(sd), 1, 1, 1, int(geo.Np), int(geo.Np))
+        Q1 = np.empty(shape, np.complex64)
+        Q2 = np.empty(shape, np.complex64)
+        for il, wl in enumerate(sd):
+            q1, q2 = scaled_asp_phases(geo.zo, float(wl), geo.Np, geo.dxo, geo.dxd)
+            Q1[il, 0, 0, 0], Q2[il, 0, 0, 0] = q1, q2
         if inverse:
             return ifft2c(fft2c(fields) * Q2.conj()) * Q1.conj()
         return ifft2c(fft2c(fields * Q1) * Q2)

@@ -25,7 +25,7 @@ import numpy as np
 
 from .. import _native
 from ..Operators import (asp_tf_fftorder, aspw_transfer_function, polychrome_tf_fftorder, quadratic_phase,
-                         scaled_asp_kernel_arrays)
+                         scaled_asp_kernel_arrays, scaled_polychrome_kernel_arrays)
 
 # switches whose non-default value changes the per-position math or the
 # per-iteration constraints; the GPU engines implement the defaults and the
@@ -73,12 +73,12 @@ class BaseEngine:
         if p.positionOrder not in ("sequential", "random", "NA"):
             raise ValueError("position order not properly set")
         kind = p.propagatorType.lower()
-        if kind not in ("fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp"):
+        if kind not in ("fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp"):
             raise NotImplementedError(f"propagatorType {p.propagatorType!r} is not implemented by the MI355X engines")
         if kind in ("asp", "polychromeasp") and p.fftshiftSwitch:
             raise ValueError("ASP propagatorType works only with fftshiftSwitch = False")
-        if kind == "scaledasp" and p.fftshiftSwitch:
-            raise ValueError("scaledASP propagatorType works only with fftshiftSwitch = False!")
+        if kind in ("scaledasp", "scaledpolychromeasp") and p.fftshiftSwitch:
+            raise ValueError(f"{p.propagatorType} propagatorType works only with fftshiftSwitch = False!")
         if kind == "fresnel" and p.fftshiftSwitch:
             raise NotImplementedError("Fresnel with fftshiftSwitch=True is not implemented by the MI355X engines")
         if self.experimentalData.operationMode != "CPM":
@@ -159,6 +159,11 @@ class BaseEngine:
             if r.nlambda > 1:
                 raise ValueError("For multi-wavelength, scaledPolychromeASP needs to be used instead of scaledASP")
             return scaled_asp_kernel_arrays(r.zo, r.wavelength, r.Np, r.dxo, r.dxd)
+        if kind == "scaledpolychromeasp":
+            sd = np.atleast_1d(r.spectralDensity)
+            if len(sd) != r.nlambda:
+                raise ValueError(f"scaledPolychromeASP: spectralDensity has {len(sd)} wavelengths, nlambda is {r.nlambda}")
+            return scaled_polychrome_kernel_arrays(r.zo, sd, r.Np, r.dxo, r.dxd)
         return None
 
     def _propagator_id(self):
@@ -171,6 +176,8 @@ class BaseEngine:
             return _native.PTY_FRESNEL
         if kind == "scaledasp":
             return _native.PTY_SCALED_ASP
+        if kind == "scaledpolychromeasp":
+            return _native.PTY_SCALED_POLYCHROME_ASP
         return _native.PTY_FRAUNHOFER_SHIFTED if self.params.fftshiftSwitch else _native.PTY_FRAUNHOFER
 
     # -- device side ------------------------------------------------------------

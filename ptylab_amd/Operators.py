@@ -18,7 +18,7 @@ import numpy as np
 from . import _native
 from .utils import circ
 
-PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp"}
+PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp"}
 
 
 @lru_cache(maxsize=8)
@@ -123,6 +123,14 @@ def scaled_asp_kernel_arrays(z, wavelength, N, dxo, dxd):
     return np.ascontiguousarray(np.stack([Q2, (Q1 * c).astype(np.complex64)]))
 
 
+def scaled_polychrome_kernel_arrays(z, spectral_density, N, dxo, dxd):
+    """(2 nlambda, N, N) complex64 for scaledPolychromeASP: the Q2 of every
+    wavelength of the spectral density, then its Q1 (-1)^(y+x)
+    (Operators.py:990-1032 builds each pair as scaledASP does)."""
+    pairs = [scaled_asp_kernel_arrays(z, float(w), N, dxo, dxd) for w in np.atleast_1d(spectral_density)]
+    return np.ascontiguousarray(np.concatenate([np.stack([q[0] for q in pairs]), np.stack([q[1] for q in pairs])]))
+
+
 def _kind(params):
     kind = params.propagatorType.lower()
     if kind not in PROPAGATORS:
@@ -131,9 +139,9 @@ def _kind(params):
 
 
 def _native_kind(params, kind):
-    if kind == "scaledasp" and params.fftshiftSwitch:
-        raise ValueError("scaledASP propagatorType works only with fftshiftSwitch = False!")
-    if kind in ("asp", "polychromeasp", "scaledasp"):
+    if kind in ("scaledasp", "scaledpolychromeasp") and params.fftshiftSwitch:
+        raise ValueError(f"{params.propagatorType} propagatorType works only with fftshiftSwitch = False!")
+    if kind in ("asp", "polychromeasp", "scaledasp", "scaledpolychromeasp"):
         if params.fftshiftSwitch:
             raise ValueError("ASP propagator only works with fftshiftswitch == False")
         return _native.PTY_ASP
@@ -167,21 +175,34 @@ def _propagate(fields, params, reconstruction, inverse):
             plane = out[il].contiguous()
             _native.propagate(plane, nk, inverse, tfs[il].contiguous(), stream)
             out[il] = plane
-    elif kind == "scaledasp":
-        if r.nlambda > 1:
-            raise ValueError("For multi-wavelength, scaledPolychromeASP needs to be used instead of scaledASP")
-        pair = torch.as_tensor(scaled_asp_kernel_arrays(r.zo, r.wavelength, r.Np, r.dxo, r.dxd), device=dev)
-        q2, q1c = pair[0].contiguous(), pair[1]
+    elif kind in ("scaledasp", "scaledpolychromeasp"):
+        if kind == "scaledasp":
+            if r.nlambda > 1:
+                raise ValueError("For multi-wavelength, scaledPolychromeASP needs to be used instead of scaledASP")
+            arr = scaled_asp_kernel_arrays(r.zo, r.wavelength, r.Np, r.dxo, r.dxd)
+        else:
+            arr = scaled_polychrome_kernel_arrays(r.zo, r.spectralDensity, r.Np, r.dxo, r.dxd)
+        q = torch.as_tensor(arr, device=dev)
+        nl = q.shape[0] // 2
+        if nl > 1 and out.shape[0] != nl:
+            raise ValueError(f"scaledPolychromeASP: {out.shape[0]} wavelength planes, {nl} wavelengths")
         idx = torch.arange(int(r.Np), device=dev)
         cb = (1 - 2 * ((idx[:, None] + idx[None, :]) % 2)).to(torch.complex64)  # (-1)^(y+x)
-        if inverse:
-            out = (out * cb).contiguous()
-            _native.propagate(out, nk, True, q2, stream)
-            out = (out * q1c.conj()).contiguous()
-        else:
-            out = (out * q1c).contiguous()
-            _native.propagate(out, nk, False, q2, stream)
-            out = (out * cb).contiguous()
+        planes = [out] if nl == 1 else [out[il] for il in range(nl)]
+        for il, plane in enumerate(planes):
+            q2, q1c = q[il].contiguous(), q[nl + il]
+            if inverse:
+                plane = (plane * cb).contiguous()
+                _native.propagate(plane, nk, True, q2, stream)
+                plane = plane * q1c.conj()
+            else:
+                plane = (plane * q1c).contiguous()
+                _native.propagate(plane, nk, False, q2, stream)
+                plane = plane * cb
+            if nl == 1:
+                out = plane.contiguous()
+            else:
+                out[il] = plane
     elif kind == "fresnel":
         Q = torch.as_tensor(quadratic_phase(r.zo, r.wavelength, r.Np, r.dxp), device=dev)
         if inverse:

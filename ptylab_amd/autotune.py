@@ -69,7 +69,7 @@ def family(engine, propagator):
         return 2
     if engine == _native.PTY_QNEWTON:
         return 4 if asp else 3
-    if propagator == _native.PTY_SCALED_ASP:
+    if propagator in (_native.PTY_SCALED_ASP, _native.PTY_SCALED_POLYCHROME_ASP):
         return 5
     return 1 if asp else 0
 

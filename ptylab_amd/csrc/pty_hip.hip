@@ -127,7 +127,7 @@ __device__ float2 g_twiddle[6][256];  // exp(-2 pi i j / N), j < N/2, N = 16..51
 __host__ __device__ static inline bool prop_checkerboard(int prop) { return prop == PTY_FRAUNHOFER || prop == PTY_FRESNEL; }
 // propagators of the angular-spectrum family (detector = ifft(fft(.) H))
 __host__ __device__ static inline bool prop_asp(int prop) {
-  return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP || prop == PTY_SCALED_ASP;
+  return prop == PTY_ASP || prop == PTY_POLYCHROME_ASP || prop == PTY_SCALED_ASP || prop == PTY_SCALED_POLYCHROME_ASP;
 }
 
 __host__ __device__ static int log2_np(int np) {
@@ -576,7 +576,8 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       // Fresnel: fft2c(esw * Q) (Operators.py:110); scaled ASP: c Q1 esw
       // (Operators.py:345); no phase in the other ASP-family kernels
       if constexpr (!ASPF || QPH) {
-        if (Qp) e = cmul(e, ldg2(Qp + (size_t)r * NP + l * LPF + R.t));
+        // one phase per wavelength (scaledPolychromeASP; fpl = nF otherwise)
+        if (Qp) e = cmul(e, ldg2(Qp + (size_t)(f / D.fpl) * Np2 + (size_t)r * NP + l * LPF + R.t));
       }
       if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
       v[l] = e;
@@ -876,8 +877,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
         // Fresnel: ifft2c(.) * conj(Q) (Operators.py:150)
         if (Qp) u = cmul(u, conjf2(ldg2(Qp + prow_off + l * LPF + R.t)));
       } else if constexpr (QPH) {
-        // scaled ASP: ... conj(c Q1) (Operators.py:343-344)
-        u = cmul(u, conjf2(ldg2(Qp + prow_off + l * LPF + R.t)));
+        // scaled ASP: ... conj(c Q1) (Operators.py:343-344), per wavelength
+        u = cmul(u, conjf2(ldg2(Qp + (size_t)(f / D.fpl) * Np2 + prow_off + l * LPF + R.t)));
       }
       const float2 dl = csub(u, cmul(of[l], pf[l]));
       float wo, wp;
@@ -1767,7 +1768,7 @@ static const void* iterate_kernel_ptr(int np, int fam) {
 static int family_of(const pty_instance& a) {
   if (a.engine == PTY_E3PIE) return FAM_E3PIE;
   if (a.engine == PTY_QNEWTON) return prop_asp(a.propagator) ? FAM_QN_ASP : FAM_QN_FRAUNHOFER;
-  if (a.propagator == PTY_SCALED_ASP) return FAM_PIE_SASP;
+  if (a.propagator == PTY_SCALED_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP) return FAM_PIE_SASP;
   return prop_asp(a.propagator) ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
 }
 
@@ -1842,7 +1843,7 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 int validate(const pty_instance& a, int idx, int nt) {
   if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
   if (a.engine < PTY_EPIE || a.engine > PTY_QNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
-  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_SCALED_ASP)
+  if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_SCALED_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
   if (log2_np(a.Np) < 0)
     return set_err(PTY_EINVAL, "instance %d: Np=%d unsupported (powers of two 16..512)", idx, a.Np);
@@ -1859,9 +1860,9 @@ int validate(const pty_instance& a, int idx, int nt) {
     return set_err(PTY_EINVAL, "instance %d: polychromeASP needs one transfer function per wavelength", idx);
   if (a.propagator == PTY_FRESNEL && !a.tf)
     return set_err(PTY_EINVAL, "instance %d: Fresnel needs the quadratic phase", idx);
-  if (a.propagator == PTY_SCALED_ASP) {
+  if (a.propagator == PTY_SCALED_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP) {
     if (!a.tf) return set_err(PTY_EINVAL, "instance %d: scaledASP needs Q2 and the checkerboarded Q1", idx);
-    if (a.nlambda > 1)
+    if (a.propagator == PTY_SCALED_ASP && a.nlambda > 1)
       return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, scaledPolychromeASP is needed instead of scaledASP",
                      idx);
     if (a.engine == PTY_QNEWTON || a.engine == PTY_E3PIE)
@@ -2063,15 +2064,17 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     if (a.propagator == PTY_FRESNEL) {
       d.Q = (const float2*)a.tf;  // natural order, read by the row passes
     } else if (a.tf) {
-      const int ntf = a.propagator == PTY_POLYCHROME_ASP ? a.nlambda : 1;
-      if (a.propagator == PTY_POLYCHROME_ASP) d.fpl = d.nF / a.nlambda;
+      const bool per_lambda = a.propagator == PTY_POLYCHROME_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP;
+      const int ntf = per_lambda ? a.nlambda : 1;
+      if (per_lambda) d.fpl = d.nF / a.nlambda;
       if ((rc = dalloc(p, &mem, (size_t)ntf * np2 * sizeof(float2)))) goto fail;
       for (int t = 0; t < ntf; ++t)
         pty_permute_tf_kernel<<<256, 256, 0, st>>>((const float2*)a.tf + (size_t)t * np2, (float2*)mem + (size_t)t * np2,
                                                    np, log2_np(np));
       HIP_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
-      if (a.propagator == PTY_SCALED_ASP) d.Q = (const float2*)a.tf + np2;  // c Q1, natural order
+      if (a.propagator == PTY_SCALED_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP)
+        d.Q = (const float2*)a.tf + (size_t)ntf * np2;  // c Q1 per wavelength, natural order
       if (prop_asp(a.propagator) && a.engine != PTY_E3PIE) {
         if ((rc = dalloc(p, &mem, (size_t)ntf * np))) goto fail;
         pty_live_columns_kernel<<<(ntf * np + 255) / 256, 256, 0, st>>>(d.HT, (unsigned char*)mem, np, ntf);
@@ -2295,7 +2298,7 @@ int pty_plan_use_graph(pty_plan* p, int32_t enable) {
 
 int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count) {
   if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_QNEWTON || propagator < PTY_FRAUNHOFER ||
-      propagator > PTY_SCALED_ASP)
+      propagator > PTY_SCALED_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "pty_coresident_workgroups: bad arguments");
   pty_instance a;
   memset(&a, 0, sizeof a);

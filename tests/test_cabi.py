@@ -82,3 +82,22 @@ def test_ctypes_signatures_match_header_parameter_counts(header, module):
             assert n == 0, name
         else:
             assert len(at) == n, (name, len(at), n)
+
+
+def test_header_constants_match_the_binding():
+    """Every numeric #define PTY_* of include/pty_hip.h has the same value in
+    ptylab_amd._native (engine, propagator, flag and error codes)."""
+    import re
+    from pathlib import Path
+
+    from ptylab_amd import _native
+
+    text = (Path(__file__).resolve().parents[1] / "include" / "pty_hip.h").read_text()
+    found = 0
+    for name, value in re.findall(r"#define (PTY_[A-Z0-9_]+) \(?(-?\d+)\)?", text):
+        if hasattr(_native, name):
+            assert getattr(_native, name) == int(value), name
+            found += 1
+    for name in ("PTY_SCALED_ASP", "PTY_SCALED_POLYCHROME_ASP", "PTY_QNEWTON", "PTY_FLAG_BATCHED"):
+        assert hasattr(_native, name), name
+    assert found >= 12

@@ -53,21 +53,22 @@ def test_engine_matches_oracle_on_golden_inputs(name):
             assert d_gpu64 < 1e-5, (name, key)
 
 
-def test_scaled_asp_engine_matches_oracle():
+@pytest.mark.parametrize("prop,nl", [("scaledASP", 1), ("scaledPolychromeASP", 3)])
+def test_scaled_asp_engine_matches_oracle(prop, nl):
     """ePIE with the scaled angular spectrum propagator on the reference's
     regression dataset.  The reference holds no engine golden for scaledASP;
     the oracle's scaledASP reproduces its propagator_scaledasp golden, and
     the GPU run is held to the golden-input criterion above (its complex64
     chain keeps Q1, Q2 complex64, as the reference stores them)."""
     d = regression_dataset()
-    modes = (1, 1, 1, 1)
-    print("case epie scaledASP", flush=True)
-    gpu = run_package(d, "ePIE", "scaledASP", modes, 2, SEED)
-    c64 = run_oracle(d, "ePIE", "scaledASP", modes, 2, SEED, precision="c64")
-    c128 = run_oracle(d, "ePIE", "scaledASP", modes, 2, SEED, precision="c128")
+    modes = (nl, 1, 1, 1)
+    print(f"case epie {prop}", flush=True)
+    gpu = run_package(d, "ePIE", prop, modes, 2, SEED)
+    c64 = run_oracle(d, "ePIE", prop, modes, 2, SEED, precision="c64")
+    c128 = run_oracle(d, "ePIE", prop, modes, 2, SEED, precision="c128")
     for key in ("object", "probe", "error"):
         g, a, b = np.asarray(getattr(gpu, key)), np.asarray(getattr(c64, key)), np.asarray(getattr(c128, key))
         d_gpu64, d_gpu128, d_64_128 = rel(g, a), rel(g, b), rel(a, b)
-        print(f"epie_scaledasp {key}: gpu-c64 {d_gpu64:.2e} gpu-c128 {d_gpu128:.2e} c64-c128 {d_64_128:.2e}", flush=True)
+        print(f"epie_{prop} {key}: gpu-c64 {d_gpu64:.2e} gpu-c128 {d_gpu128:.2e} c64-c128 {d_64_128:.2e}", flush=True)
         floor = 1e-5 if key == "error" else 1e-6
         assert d_gpu128 <= 2.0 * d_64_128 + floor, key

@@ -59,7 +59,8 @@ def test_asp_propagator_matches_oracle(n):
     assert rel(t.cpu().numpy(), want) < 2e-6 * np.log2(n)
 
 
-@pytest.mark.parametrize("prop,nl", [("Fresnel", 1), ("polychromeASP", 3), ("scaledASP", 1)])
+@pytest.mark.parametrize("prop,nl", [("Fresnel", 1), ("polychromeASP", 3), ("scaledASP", 1),
+                                     ("scaledPolychromeASP", 3)])
 def test_fresnel_and_polychrome_propagators_match_oracle(prop, nl):
     """Operators.object2detector / detector2object on the GPU against the
     oracle's complex128 propagation of the same complex64 field (the
