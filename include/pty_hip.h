@@ -38,6 +38,9 @@ extern "C" {
 #define PTY_MULTIPIE 2
 #define PTY_E3PIE 3
 #define PTY_QNEWTON 4  /* Engines/qNewton.py; alphaObject/alphaProbe carry regObject/regProbe */
+#define PTY_MQNEWTON 5 /* Engines/mqNewton.py with ADAM momentum: as PTY_QNEWTON, plus momFriction = beta1,
+                          momFeedback = beta2, probeWeight = betaObject_m, sliceBeta = betaProbe_m; needs the
+                          buffer and momentum arrays; second moments start at zero */
 
 /* propagators (Operators.py:1082-1103 dispatch keys) */
 #define PTY_FRAUNHOFER 0          /* fft2c with shifts, fftshiftSwitch=False  */

@@ -376,6 +376,10 @@ class OracleRun:
     betaProbe: Optional[float] = None
     regObject: float = 1.0   # qNewton regularisers (qNewton.py:51-52)
     regProbe: float = 1.0
+    beta1: float = 0.5        # mqNewton ADAM coefficients (mqNewton.py:57-60)
+    beta2: float = 0.5
+    betaObject_m: float = 0.25
+    betaProbe_m: float = 0.25
     alphaObject: float = 0.1
     alphaProbe: float = 0.1
     feedbackM: float = 0.3   # mPIE feedback / multiPIE betaM
@@ -397,10 +401,11 @@ class OracleRun:
         (complex64 everywhere, including e3PIE's slice transfer function, as
         the reference's GPU path casts it), "c128" (double precision)."""
         g = self.geo
+        qn = self.engine in ("qNewton", "mqNewton")
         if self.betaObject is None:
-            self.betaObject = 1 if self.engine == "qNewton" else 0.25
+            self.betaObject = 1 if qn else 0.25
         if self.betaProbe is None:
-            self.betaProbe = 1 if self.engine == "qNewton" else 0.25
+            self.betaProbe = 1 if qn else 0.25
         c = np.complex128 if self.precision == "c128" else np.complex64
         self.cdtype = c
         self.object = np.asarray(self.object).astype(c)
@@ -411,10 +416,14 @@ class OracleRun:
         self.errorAtPos = None
         self._tf = {}
         self.schedule = []  # (order, momentum triggers) per iteration, for inspection
-        if self.engine in ("mPIE", "multiPIE"):
-            # mPIE.py:87-94 / multiPIE.py:61-66
+        if self.engine in ("mPIE", "multiPIE", "mqNewton"):
+            # mPIE.py:87-94 / multiPIE.py:61-66 / mqNewton.py:42-46
             self.objectMomentum = np.zeros_like(self.object)
             self.probeMomentum = np.zeros_like(self.probe)
+        if self.engine == "mqNewton":
+            # buffers at construction (mqNewton.py:45-47)
+            self.objectBuffer = self.object.copy()
+            self.probeBuffer = self.probe.copy()
         if self.engine == "e3PIE":
             # e3PIE.py:61-69
             H = aspw_transfer(self.dz, g.wavelength / self.refrIndex, g.Np, g.Lp)
@@ -540,6 +549,10 @@ class OracleRun:
             # mPIE.py:104-105
             self.objectBuffer = self.object.copy()
             self.probeBuffer = self.probe.copy()
+        if self.engine == "mqNewton":
+            # initializeAdaptiveMomentum (mqNewton.py:70-84)
+            self.objectMomentum_v = self.objectMomentum.copy()
+            self.probeMomentum_v = self.probeMomentum.copy()
         F = self.ptychogram.shape[0]
         for loop in range(self.numIterations):
             order = position_order(self.positionOrder, F, len(self.error), g.positions)
@@ -551,6 +564,8 @@ class OracleRun:
                     self._e3pie_step(k)
                 else:
                     self._pie_step(k)
+                if self.engine == "mqNewton":
+                    self._adam_momentum(loop)
                 if self.engine in ("mPIE", "multiPIE"):
                     t = bool(np.random.rand(1) > 0.95)
                     trig.append(t)
@@ -572,7 +587,7 @@ class OracleRun:
         esw_upd = self._intensity_projection(k, esw)
         DELTA = esw_upd - esw
         P = self.probe
-        if self.engine == "qNewton":
+        if self.engine in ("qNewton", "mqNewton"):
             # qNewton.py:106-140: the object step first, then the probe step with
             # Omax taken over the whole, already updated, object
             Pmax = np.max(np.sum(np.abs(P), axis=(0, 1, 2, 3)))
@@ -612,6 +627,31 @@ class OracleRun:
         grad = self.probeBuffer - self.probe
         self.probeMomentum = grad + self.frictionM * self.probeMomentum
         self.probe = self.probe - self.feedbackM * self.probeMomentum
+        self.probeBuffer = self.probe.copy()
+
+    def _adam(self, grad, mt, vt, itr):
+        """mqNewton.ADAM (mqNewton.py:160-174): the second moment takes the
+        squared 2-norm of the whole flattened gradient."""
+        beta1_scale = 1 - self.beta1 ** itr
+        beta2_scale = 1 - self.beta2 ** itr
+        mt = self.beta1 * mt + (1 - self.beta1) * grad
+        vt = self.beta2 * vt + (1 - self.beta2) * np.linalg.norm(grad.flatten().squeeze(), 2) ** 2
+        m_hat = mt / beta1_scale
+        v_hat = vt / beta2_scale
+        return m_hat / (v_hat ** 0.5 + 1e-8), mt, vt
+
+    def _adam_momentum(self, loop):
+        """objectMomentumUpdate / probeMomentumUpdate (mqNewton.py:197-235)
+        after every position, with itr = loop + 1."""
+        grad = self.objectBuffer - self.object
+        upd, self.objectMomentum, self.objectMomentum_v = self._adam(grad, self.objectMomentum,
+                                                                     self.objectMomentum_v, loop + 1)
+        self.object = self.object - self.betaObject_m * upd
+        self.objectBuffer = self.object.copy()
+        grad = self.probeBuffer - self.probe
+        upd, self.probeMomentum, self.probeMomentum_v = self._adam(grad, self.probeMomentum,
+                                                                   self.probeMomentum_v, loop + 1)
+        self.probe = self.probe - self.betaProbe_m * upd
         self.probeBuffer = self.probe.copy()
 
     def _e3pie_step(self, k):
@@ -669,6 +709,7 @@ PINNED = {
     "OracleRun.multiPIE": "identity with mPIE at nlambda=nosm=1 (no golden exists)",
     "OracleRun.e3PIE": "golden e3pie_multislice",
     "OracleRun.qNewton": "golden qnewton_single",
+    "OracleRun.mqNewton": "equals qNewton with betaObject_m = betaProbe_m = 0; ADAM's first step is g / (|g| + 1e-8)",
     "propagate(fresnel/polychromeasp)": "goldens propagator_fresnel / propagator_polychromeasp, epie_fresnel, epie_polychrome",
     "orthogonalize_modes": "orthogonality + descending eigenvalues + span invariants",
     "nrmse": "closed form: invariant to a global complex scale; zero at gamma*true",

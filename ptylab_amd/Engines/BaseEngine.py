@@ -45,6 +45,9 @@ _UNSUPPORTED_TRUE = [
 
 class BaseEngine:
     ENGINE = None
+    # mPIE / multiPIE draw one rand(1) per position for their momentum step
+    # (mPIE.py:170-172); engines with momentum at every position draw none
+    _MOMENTUM_TRIGGERS = True
 
     def __init__(self, reconstruction, experimentalData, params, monitor):
         self.betaObject = 0.25
@@ -209,7 +212,7 @@ class BaseEngine:
         T["ptychogram"] = torch.as_tensor(np.ascontiguousarray(d.ptychogram, dtype=np.float32), device=dev)
         T["positions"] = torch.as_tensor(np.ascontiguousarray(r.positions, dtype=np.int32), device=dev)
         T["energy"] = torch.as_tensor(np.ascontiguousarray(np.asarray(d.energyAtPos, dtype=np.float32)), device=dev)
-        orders, masks = self._schedule(n_iter, momentum)
+        orders, masks = self._schedule(n_iter, momentum and self._MOMENTUM_TRIGGERS)
         self._orders, self._masks = orders, masks
         T["order"] = torch.as_tensor(orders, device=dev)
         T["mask"] = torch.as_tensor(masks, device=dev)

@@ -114,6 +114,8 @@ struct DevInst {
   unsigned* ctrl;    // control block, CTRL_* layout
   unsigned* mx;      // per-position maxima: 4 ring slots x SLOT_WORDS words x G workgroup partials
   unsigned* bar;     // flag barrier: generation reached by each workgroup (G words)
+  double* adampart;  // mqNewton: per-workgroup FP64 partials of |gradient|^2, (2, G)
+  double* adamv;     // mqNewton: ADAM second moments of object and probe, (2 parities, 2)
   unsigned osel[32]; // fields whose object terms the owner of field f's object sums (bit g)
   unsigned psel[32]; // fields whose probe terms the owner of field f's probe sums (bit g)
   int gsize[8];      // counter barrier: workgroups of this instance in each block group b % 8
@@ -1045,6 +1047,69 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
   }
 }
 
+// mqNewton's adaptive momentum after every position (mqNewton.py:112-113,
+// 160-174 ADAM, 195-235): gradient g = buffer - current for the whole object
+// and the whole probe, m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) |g|^2 (the
+// squared 2-norm of the whole gradient, one number), current -= beta_m *
+// (m / (1 - b1^itr)) / (sqrt(v / (1 - b2^itr)) + 1e-8), buffer = current.
+// Two passes: FP64 partials of |g|^2 per workgroup, then (after a barrier)
+// every workgroup sums them in workgroup order and updates its own chunk.
+// The instance fields carry b1 (momFriction), b2 (momFeedback) and the
+// object and probe steps beta_m (probeWeight, sliceBeta).
+__device__ __forceinline__ void adam_norm_pass(Run& R) {
+  const DevInst& D = *R.D;
+  for (int which = 0; which < 2; ++which) {
+    const size_t n = which == 0 ? (size_t)D.nO * D.a.No * D.a.No : (size_t)D.nPr * D.a.Np * D.a.Np;
+    const float2* X = (const float2*)(which == 0 ? D.a.object : D.a.probe);
+    const float2* B = (const float2*)(which == 0 ? D.a.objectBuffer : D.a.probeBuffer);
+    const size_t chunk = (n + R.G - 1) / R.G;
+    const size_t b = chunk * R.g;
+    const size_t e = b + chunk < n ? b + chunk : n;
+    double acc = 0.0;
+    for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
+      const float2 x = cload(R, X + i), bb = cload(R, B + i);
+      const float gx = bb.x - x.x, gy = bb.y - x.y;
+      acc += (double)gx * (double)gx + (double)gy * (double)gy;
+    }
+    const double s = block_sum(R, acc);
+    if (threadIdx.x == 0) cstored(R, D.adampart + (size_t)which * R.G + R.g, s);
+  }
+}
+
+__device__ __forceinline__ void adam_update_pass(Run& R, int itr, unsigned long long p) {
+  const DevInst& D = *R.D;
+  const float b1 = D.a.momFriction, b2 = D.a.momFeedback;
+  const float omb1 = (float)(1.0 - (double)b1), omb2 = (float)(1.0 - (double)b2);
+  const float s1 = (float)(1.0 - pow((double)b1, (double)itr));
+  const float s2 = (float)(1.0 - pow((double)b2, (double)itr));
+  for (int which = 0; which < 2; ++which) {
+    double sq = 0.0;
+    for (int g = 0; g < R.G; ++g) sq += cloadd(R, D.adampart + (size_t)which * R.G + g);
+    const float nrm = sqrtf((float)sq);
+    const float vt = b2 * (float)cloadd(R, D.adamv + (p & 1ull) * 2 + which) + omb2 * (nrm * nrm);
+    if (R.g == 0 && threadIdx.x == 0) cstored(R, D.adamv + ((p + 1ull) & 1ull) * 2 + which, (double)vt);
+    const float den = sqrtf(vt / s2) + 1e-8f;
+    const float bm = which == 0 ? D.a.probeWeight : D.a.sliceBeta;
+    const size_t n = which == 0 ? (size_t)D.nO * D.a.No * D.a.No : (size_t)D.nPr * D.a.Np * D.a.Np;
+    float2* X = (float2*)(which == 0 ? D.a.object : D.a.probe);
+    float2* B = (float2*)(which == 0 ? D.a.objectBuffer : D.a.probeBuffer);
+    float2* M = (float2*)(which == 0 ? D.a.objectMomentum : D.a.probeMomentum);
+    const size_t chunk = (n + R.G - 1) / R.G;
+    const size_t b = chunk * R.g;
+    const size_t e = b + chunk < n ? b + chunk : n;
+    for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
+      const float2 x = cload(R, X + i), bb = cload(R, B + i), m = cload(R, M + i);
+      const float2 gr = make_float2(bb.x - x.x, bb.y - x.y);
+      const float2 mn = make_float2(b1 * m.x + omb1 * gr.x, b1 * m.y + omb1 * gr.y);
+      const float2 u = make_float2((mn.x / s1) / den, (mn.y / s1) / den);
+      const float2 xn = make_float2(x.x - bm * u.x, x.y - bm * u.y);
+      cstore(R, M + i, mn);
+      cstore(R, X + i, xn);
+      cstore(R, B + i, xn);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // e3PIE slice chain (e3PIE.py:89-173, updates :192-225).  W holds the
 // (lambda, psm) fields of the slice being propagated; ESW keeps the exit
@@ -1261,7 +1326,7 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 template <int NP, bool ASP, int CWR, bool QN, bool QPH>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
-                                             bool sync_after) {
+                                             bool sync_after, int it) {
   const DevInst& D = *R.D;
   pie_rows_first<NP, QN, ASP, QPH>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
@@ -1283,6 +1348,12 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     if (!cluster_sync(R)) return false;
     qn_probe_rows<NP>(R, slot);
     if (!cluster_sync(R)) return false;
+    if (D.a.engine == PTY_MQNEWTON) {
+      adam_norm_pass(R);
+      if (!cluster_sync(R)) return false;
+      adam_update_pass(R, it + 1, (unsigned long long)it * D.a.F + j);
+      if (!cluster_sync(R)) return false;
+    }
   }
   return true;
 }
@@ -1360,7 +1431,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       else
         ok = pie_position<NP, FAM == FAM_PIE_ASP || FAM == FAM_QN_ASP || FAM == FAM_PIE_SASP, CWR,
                           FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP, FAM == FAM_PIE_SASP>(R, k, j, slot, zslot, ep,
-                                                                                         sync_after);
+                                                                                         sync_after, it);
       if (!ok) return;
       if (mom) {
         momentum_pass<NP>(R);
@@ -1767,7 +1838,7 @@ static const void* iterate_kernel_ptr(int np, int fam) {
 
 static int family_of(const pty_instance& a) {
   if (a.engine == PTY_E3PIE) return FAM_E3PIE;
-  if (a.engine == PTY_QNEWTON) return prop_asp(a.propagator) ? FAM_QN_ASP : FAM_QN_FRAUNHOFER;
+  if (a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON) return prop_asp(a.propagator) ? FAM_QN_ASP : FAM_QN_FRAUNHOFER;
   if (a.propagator == PTY_SCALED_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP) return FAM_PIE_SASP;
   return prop_asp(a.propagator) ? FAM_PIE_ASP : FAM_PIE_FRAUNHOFER;
 }
@@ -1842,7 +1913,9 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 
 int validate(const pty_instance& a, int idx, int nt) {
   if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
-  if (a.engine < PTY_EPIE || a.engine > PTY_QNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
+  if (a.engine < PTY_EPIE || a.engine > PTY_MQNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
+  if (a.engine == PTY_MQNEWTON && (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
+    return set_err(PTY_EINVAL, "instance %d: mqNewton needs buffer and momentum arrays", idx);
   if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_SCALED_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "instance %d: unknown propagator %d", idx, a.propagator);
   if (log2_np(a.Np) < 0)
@@ -1865,7 +1938,7 @@ int validate(const pty_instance& a, int idx, int nt) {
     if (a.propagator == PTY_SCALED_ASP && a.nlambda > 1)
       return set_err(PTY_EINVAL, "instance %d: For multi-wavelength, scaledPolychromeASP is needed instead of scaledASP",
                      idx);
-    if (a.engine == PTY_QNEWTON || a.engine == PTY_E3PIE)
+    if (a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON || a.engine == PTY_E3PIE)
       return set_err(PTY_EINVAL, "instance %d: scaledASP runs with the ePIE, mPIE and multiPIE engines", idx);
   }
   const int slots = slots_rt(a.Np, nt);
@@ -1997,7 +2070,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     if (a.engine != PTY_E3PIE) {
       // the field sums of the object and probe updates (ePIE.py:128-160,
       // mPIE.py:226-270, multiPIE.py:153-205, qNewton.py:116-140)
-      const bool full = a.engine == PTY_EPIE || a.engine == PTY_QNEWTON;
+      const bool full = a.engine == PTY_EPIE || a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON;
       for (int f = 0; f < d.nF; ++f) {
         const FieldIdx x = field_idx(f, a.nosm, a.npsm, a.nslice);
         for (int g = 0; g < d.nF; ++g) {
@@ -2040,9 +2113,18 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       if ((rc = dalloc(p, &mem, (size_t)a.nslice * d.nF * np2 * sizeof(float2)))) goto fail;
       d.ESW = (float2*)mem;
     }
-    if (a.engine == PTY_QNEWTON) {
+    if (a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON) {
       if ((rc = dalloc(p, &mem, (size_t)d.nO * np2 * sizeof(float2)))) goto fail;
       d.OP = (float2*)mem;
+    }
+    if (a.engine == PTY_MQNEWTON) {
+      if ((rc = dalloc(p, &mem, (size_t)2 * G * sizeof(double)))) goto fail;
+      d.adampart = (double*)mem;
+      // the second moments start at zero (a fresh engine's momentum copy,
+      // mqNewton.py:76-84) and persist across the plan's launches
+      if ((rc = dalloc(p, &mem, 4 * sizeof(double)))) goto fail;
+      HIP_TRY(hipMemsetAsync(mem, 0, 4 * sizeof(double), st));
+      d.adamv = (double*)mem;
     }
     if ((rc = dalloc(p, &mem, CTRL_WORDS * sizeof(unsigned)))) goto fail;
     d.ctrl = (unsigned*)mem;
@@ -2297,7 +2379,7 @@ int pty_plan_use_graph(pty_plan* p, int32_t enable) {
 }
 
 int pty_coresident_workgroups(int32_t Np, int32_t engine, int32_t propagator, int32_t batched, int32_t* count) {
-  if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_QNEWTON || propagator < PTY_FRAUNHOFER ||
+  if (!count || log2_np(Np) < 0 || engine < PTY_EPIE || engine > PTY_MQNEWTON || propagator < PTY_FRAUNHOFER ||
       propagator > PTY_SCALED_POLYCHROME_ASP)
     return set_err(PTY_EINVAL, "pty_coresident_workgroups: bad arguments");
   pty_instance a;

@@ -203,3 +203,16 @@ def test_hard_probe_boundary_against_oracle(c0):
                      switches=sw)
     eo, ep, ee = _report("C0 ePIE probe boundary", gpu, ref)
     assert eo <= 1e-5 and ep <= 1e-5
+
+
+def test_mqnewton_on_photon_scaled_inputs(c0):
+    """mqNewton (mqNewton.py:86-158): qNewton's position step followed by an
+    ADAM step of the whole object and probe after every position -- against
+    the oracle.  The GPU accumulates the gradient norm in FP64 where NumPy's
+    norm accumulates in float32."""
+    d = c0.as_dict()
+    for iters, bar in ((1, 1e-5), (5, 1e-4)):
+        gpu = run_package(d, "mqNewton", "Fraunhofer", (1, 1, 1, 1), iters, 3, order="random", No=c0.No)
+        ref = run_oracle(d, "mqNewton", "Fraunhofer", (1, 1, 1, 1), iters, 3, order="random", No=c0.No)
+        eo, ep, ee = _report(f"C0 mqNewton {iters} it", gpu, ref)
+        assert eo <= bar and ep <= bar and ee <= 1e-3

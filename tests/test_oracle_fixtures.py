@@ -158,3 +158,39 @@ def test_probe_windows():
     # super-Gaussian of order 10: flat to 1e-3 inside half its width
     r = np.sqrt(geo.Xp ** 2 + geo.Yp ** 2) / (3 / 4 * geo.Np * geo.dxp / 2.355)
     assert np.all(w[r < 0.5] > 1 - 1e-3)
+
+
+def test_mqnewton_without_momentum_steps_is_qnewton():
+    """mqNewton with betaObject_m = betaProbe_m = 0 leaves its ADAM steps
+    without effect, so it reproduces qNewton (whose oracle matches the
+    reference's qnewton_single golden) bit for bit."""
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=64, No=256, F=30, seed=0)
+    geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+    runs = []
+    for eng, kw in (("qNewton", {}), ("mqNewton", dict(betaObject_m=0.0, betaProbe_m=0.0))):
+        np.random.seed(1)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        np.random.seed(5)
+        runs.append(O.OracleRun(geo, obj, pr, engine=eng, numIterations=2, **kw).reconstruct())
+    a, b = runs
+    np.testing.assert_array_equal(a.object, b.object)
+    np.testing.assert_array_equal(a.probe, b.probe)
+    np.testing.assert_array_equal(a.error, b.error)
+
+
+def test_adam_closed_forms():
+    """ADAM's first step is g / (|g| + 1e-8); a gradient repeated at the next
+    iteration gives the same step again (the bias corrections undo the
+    moving averages)."""
+    run, geo = _c0_run()
+    rng = np.random.default_rng(0)
+    g = (rng.standard_normal((1, 1, 1, 1, 16, 16)) + 1j * rng.standard_normal((1, 1, 1, 1, 16, 16))).astype(np.complex64)
+    z = np.zeros_like(g)
+    upd, m, v = run._adam(g, z, z.copy(), 1)
+    want = g.astype(np.complex128) / (np.linalg.norm(g.astype(np.complex128).ravel()) + 1e-8)
+    np.testing.assert_allclose(upd, want, rtol=2e-6, atol=1e-9)
+    upd2, _, _ = run._adam(g, m, v, 2)
+    np.testing.assert_allclose(upd2, want, rtol=2e-6, atol=1e-9)
