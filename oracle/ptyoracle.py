@@ -43,7 +43,7 @@ import numpy as np
 
 __all__ = [
     "fft2c", "ifft2c", "circ", "aspw_transfer", "asp_transfer_stack",
-    "quad_phase", "polychrome_transfer_stack", "scaled_asp_phases", "propagate", "pixel_positions", "Geometry", "initial_guess",
+    "quad_phase", "polychrome_transfer_stack", "scaled_asp_phases", "two_step_polychrome", "propagate", "pixel_positions", "Geometry", "initial_guess",
     "energy_at_pos", "position_order", "orthogonalize_modes", "OracleRun",
     "nrmse", "GEMMA",
 ]
@@ -148,6 +148,20 @@ def scaled_asp_phases(z, wavelength, N, dx, dq):
     return Q1, Q2
 
 
+def two_step_polychrome(geo):
+    """Transfer functions and quadratic phase of the two-step polychromatic
+    propagator (Operators.py:1033-1079), complex128 as built: wavelength l
+    first propagates z0 (1 - lambda_0 / lambda_l) by ASP (centred grid), then
+    the Fresnel quadratic phase of lambda_0 and a far-field transform."""
+    sd = np.atleast_1d(geo.spectralDensity)
+    nl = len(sd)
+    T = np.empty((nl, geo.nosm, geo.npsm, 1, int(geo.Np), int(geo.Np)), dtype=np.complex128)
+    for il in range(nl):
+        T[il] = aspw_transfer(geo.zo * (1 - sd[0] / sd[il]), sd[il], geo.Np, geo.Lp)
+    Q = quad_phase(geo.zo, sd[0], geo.Np, geo.dxp)
+    return T, Q
+
+
 def polychrome_transfer_stack(nosm, npsm, Np, z, spectral_density, Lp):
     """The (nlambda, nosm, npsm, 1, Np, Np) stack of per-wavelength ASP
     transfer functions, centred, complex128 as built (Operators.py:910-952)."""
@@ -161,8 +175,8 @@ def polychrome_transfer_stack(nosm, npsm, Np, z, spectral_density, Lp):
 def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pure=False):
     """object2detector / detector2object (Operators.py:561-584, dispatch
     :1082-1103) for Fraunhofer (:27-74), Fresnel (:77-152), ASP (:155-230) and
-    polychromeASP (:458-558), scaledASP (:313-370) and scaledPolychromeASP
-    (:378-455).  pure=True keeps complex64 fields complex64
+    polychromeASP (:458-558), scaledASP (:313-370), scaledPolychromeASP (:378-455) and
+    twoStepPolychrome (:233-310).  pure=True keeps complex64 fields complex64
     (the quadratic phase cast to complex64, as a complex64 GPU path does);
     the default follows NumPy's promotion with the reference's complex128
     quadratic phase."""
@@ -233,6 +247,15 @@ def propagate(fields, kind, geo, inverse=False, shifted=False, tf_cache=None, pu
         if inverse:
             return ifft2c(fft2c(fields) * Q2.conj()) * Q1.conj()
         return ifft2c(fft2c(fields * Q1) * Q2)
+    if kind == "twosteppolychrome":
+        if shifted:
+            raise ValueError("twoStepPolychrome propagatorType works only with fftshiftSwitch = False!")
+        T, Q = two_step_polychrome(geo)
+        if pure and fields.dtype == np.complex64:
+            T, Q = T.astype(np.complex64), Q.astype(np.complex64)
+        if inverse:
+            return ifft2c(fft2c(fields * Q.conj()) * T.conj())
+        return fft2c(ifft2c(fft2c(fields) * T) * Q)
     raise NotImplementedError(f"propagator {kind!r} is outside the oracle's scope")
 
 

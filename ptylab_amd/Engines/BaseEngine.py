@@ -76,6 +76,12 @@ class BaseEngine:
         if p.positionOrder not in ("sequential", "random", "NA"):
             raise ValueError("position order not properly set")
         kind = p.propagatorType.lower()
+        if kind == "twosteppolychrome":
+            # the reference's detector2object returns the same back-propagation
+            # as esw and eswUpdate for it (Operators.py:285-310), so its
+            # engines never change object or probe; not run here
+            raise NotImplementedError("twoStepPolychrome in the engines is not implemented by the MI355X engines "
+                                      "(Operators.object2detector / detector2object support it)")
         if kind not in ("fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp"):
             raise NotImplementedError(f"propagatorType {p.propagatorType!r} is not implemented by the MI355X engines")
         if kind in ("asp", "polychromeasp") and p.fftshiftSwitch:

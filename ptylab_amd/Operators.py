@@ -18,7 +18,7 @@ import numpy as np
 from . import _native
 from .utils import circ
 
-PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp"}
+PROPAGATORS = {"fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp", "twosteppolychrome"}
 
 
 @lru_cache(maxsize=8)
@@ -131,6 +131,16 @@ def scaled_polychrome_kernel_arrays(z, spectral_density, N, dxo, dxd):
     return np.ascontiguousarray(np.concatenate([np.stack([q[0] for q in pairs]), np.stack([q[1] for q in pairs])]))
 
 
+def two_step_polychrome_arrays(z, spectral_density, N, L, dxp):
+    """The two-step polychromatic propagator's (nlambda, N, N) complex64
+    transfer functions in FFT order -- wavelength l propagates
+    z (1 - lambda_0 / lambda_l) -- and the complex64 quadratic phase of
+    lambda_0 (Operators.py:1033-1079; the reference keeps both complex128)."""
+    sd = np.atleast_1d(spectral_density)
+    T = np.stack([asp_tf_fftorder(z * (1 - sd[0] / sd[il]), float(sd[il]), N, L) for il in range(len(sd))])
+    return np.ascontiguousarray(T), quadratic_phase(z, float(sd[0]), N, dxp)
+
+
 def _kind(params):
     kind = params.propagatorType.lower()
     if kind not in PROPAGATORS:
@@ -141,7 +151,9 @@ def _kind(params):
 def _native_kind(params, kind):
     if kind in ("scaledasp", "scaledpolychromeasp") and params.fftshiftSwitch:
         raise ValueError(f"{params.propagatorType} propagatorType works only with fftshiftSwitch = False!")
-    if kind in ("asp", "polychromeasp", "scaledasp", "scaledpolychromeasp"):
+    if kind == "twosteppolychrome" and params.fftshiftSwitch:
+        raise ValueError("twoStepPolychrome propagatorType works only with fftshiftSwitch = False!")
+    if kind in ("asp", "polychromeasp", "scaledasp", "scaledpolychromeasp", "twosteppolychrome"):
         if params.fftshiftSwitch:
             raise ValueError("ASP propagator only works with fftshiftswitch == False")
         return _native.PTY_ASP
@@ -203,6 +215,23 @@ def _propagate(fields, params, reconstruction, inverse):
                 out = plane.contiguous()
             else:
                 out[il] = plane
+    elif kind == "twosteppolychrome":
+        # forward fft2c(ifft2c(fft2c(x) T) Q), inverse ifft2c(fft2c(x conj Q)
+        # conj T) as the reference writes them (Operators.py:276-282)
+        T, Q = two_step_polychrome_arrays(r.zo, r.spectralDensity, r.Np, r.Lp, r.dxp)
+        T, Q = torch.as_tensor(T, device=dev), torch.as_tensor(Q, device=dev)
+        if out.shape[0] != T.shape[0]:
+            raise ValueError(f"twoStepPolychrome: {out.shape[0]} wavelength planes, {T.shape[0]} wavelengths")
+        for il in range(T.shape[0]):
+            plane = out[il].contiguous()
+            if inverse:
+                plane = (plane * Q.conj()).contiguous()
+                _native.propagate(plane, _native.PTY_ASP, True, T[il].contiguous(), stream)
+            else:
+                _native.propagate(plane, _native.PTY_ASP, False, T[il].contiguous(), stream)
+                plane = (plane * Q).contiguous()
+                _native.propagate(plane, _native.PTY_FRAUNHOFER, False, None, stream)
+            out[il] = plane
     elif kind == "fresnel":
         Q = torch.as_tensor(quadratic_phase(r.zo, r.wavelength, r.Np, r.dxp), device=dev)
         if inverse:
@@ -224,7 +253,13 @@ def object2detector(fields, params, reconstruction):
 
 
 def detector2object(fields, params, reconstruction):
-    """Back propagation; returns (reconstruction.esw, propagated)."""
+    """Back propagation; returns (reconstruction.esw, propagated).  For
+    twoStepPolychrome the first value is the back-propagation of
+    reconstruction.ESW, as the reference's propagate_twoStepPolychrome_inv
+    returns it (Operators.py:285-310)."""
     if fields is None:
         fields = reconstruction.ESW
-    return getattr(reconstruction, "esw", None), _propagate(fields, params, reconstruction, True)
+    back = _propagate(fields, params, reconstruction, True)
+    if _kind(params) == "twosteppolychrome":
+        return _propagate(reconstruction.ESW, params, reconstruction, True), back
+    return getattr(reconstruction, "esw", None), back

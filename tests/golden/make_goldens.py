@@ -20,7 +20,7 @@ sys.path.insert(0, str(HERE.parent))
 import oracle as O  # noqa: E402
 from regression_inputs import ENGINE_CASES, SEED, regression_dataset, spectral_density  # noqa: E402
 
-PROPAGATOR_CASES = {"fraunhofer": 1, "asp": 1, "fresnel": 1, "polychromeasp": 3, "scaledasp": 1, "scaledpolychromeasp": 3}
+PROPAGATOR_CASES = {"fraunhofer": 1, "asp": 1, "fresnel": 1, "polychromeasp": 3, "scaledasp": 1, "scaledpolychromeasp": 3, "twosteppolychrome": 3}
 
 
 def engine_case(name):

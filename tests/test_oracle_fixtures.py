@@ -21,7 +21,8 @@ def test_oracle_engine_matches_committed_fixture(name):
         np.testing.assert_allclose(got[k], ref[k], rtol=1e-6, atol=1e-9, err_msg=f"{name}:{k}")
 
 
-@pytest.mark.parametrize("prop", ["fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp"])
+@pytest.mark.parametrize("prop", ["fraunhofer", "asp", "fresnel", "polychromeasp", "scaledasp", "scaledpolychromeasp",
+                                  "twosteppolychrome"])
 def test_oracle_propagator_matches_committed_fixture(prop):
     got = propagator_case(prop)
     ref = np.load(GOLDEN / f"oracle_propagator_{prop}.npz")

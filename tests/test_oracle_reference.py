@@ -42,7 +42,7 @@ def test_oracle_engine_matches_reference_golden(name):
 
 @needs_ref
 @pytest.mark.parametrize("prop,nl", [("fraunhofer", 1), ("asp", 1), ("fresnel", 1), ("polychromeasp", 3),
-                                     ("scaledasp", 1), ("scaledpolychromeasp", 3)])
+                                     ("scaledasp", 1), ("scaledpolychromeasp", 3), ("twosteppolychrome", 3)])
 def test_oracle_propagator_matches_reference_golden(prop, nl):
     d = regression_dataset()
     geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
