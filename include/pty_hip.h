@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 2
+#define PTY_ABI_VERSION 3
 
 /* error codes */
 #define PTY_OK 0
@@ -108,6 +108,16 @@ typedef struct pty_instance {
   double* err_iter;
   int32_t workgroups;  /* workgroups cooperating on this instance; 0 = auto */
   int32_t flags;       /* PTY_FLAG_* bits, 0 by default */
+  /* Position correction (BaseEngine.positionCorrection, BaseEngine.py:1241-1323;
+   * mPIE.py:163-166, pcPIE.py:107-108): with pc_direction set (F, 2) float64
+   * search directions D, in/out, the PIE engines correlate every position's
+   * pre-update patch, shifted by one pixel in the 9 directions, with the
+   * updated patch and set D[k] = 0.5 clip(grad, 3) + 0.9 D[k], from
+   * iteration pc_start of the plan on.  Needs nlambda = nosm = nslice = 1.
+   * The positions are read at every position, so the caller may rewrite
+   * them between launches (positionCorrectionUpdate, BaseEngine.py:1400-1438). */
+  double* pc_direction;
+  int32_t pc_start;
 } pty_instance;
 
 /* Run the instance in the batched launch (the one kernel that serves every
@@ -163,7 +173,9 @@ int pty_plan_run_positions(pty_plan* plan, int32_t it, int32_t j_begin, int32_t 
 /*
  * Read the plan's device-side failure flags; synchronises the stream.
  * Returns PTY_ETIMEDOUT if a cluster barrier gave up, PTY_EDEVICE if a
- * workgroup of the batched launch found no slot on its XCD.
+ * workgroup of the batched launch found no slot on its XCD, PTY_EINVAL if a
+ * position rewritten between launches put its patch outside the object (that
+ * instance stopped at it).
  */
 int pty_plan_check(pty_plan* plan, void* hip_stream);
 

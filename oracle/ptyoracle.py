@@ -416,6 +416,7 @@ class OracleRun:
     probeBoundary: bool = False
     absorbingProbeBoundary: bool = False
     absorbingProbeBoundaryAleph: float = 5e-2
+    positionCorrectionSwitch: bool = False
     precision: str = "c64"
     error: List[float] = field(default_factory=list)
 
@@ -439,12 +440,17 @@ class OracleRun:
         self.errorAtPos = None
         self._tf = {}
         self.schedule = []  # (order, momentum triggers) per iteration, for inspection
-        if self.engine in ("mPIE", "multiPIE", "mqNewton"):
-            # mPIE.py:87-94 / multiPIE.py:61-66 / mqNewton.py:42-46
+        # Reconstruction.encoder_corrected and the positions derived from it
+        # (Reconstruction.py:675-710); position correction moves them
+        self.encoder_corrected = np.array(g.encoder, copy=True)
+        self.positions = g.positions.copy()
+        self.D = np.zeros((self.ptychogram.shape[0], 2))  # _initializePCParameters (BaseEngine.py:178-197)
+        if self.engine in ("mPIE", "multiPIE", "mqNewton", "pcPIE"):
+            # mPIE.py:87-94 / multiPIE.py:61-66 / mqNewton.py:42-46 / pcPIE.py:44-51
             self.objectMomentum = np.zeros_like(self.object)
             self.probeMomentum = np.zeros_like(self.probe)
-        if self.engine == "mqNewton":
-            # buffers at construction (mqNewton.py:45-47)
+        if self.engine in ("mqNewton", "pcPIE"):
+            # buffers at construction (mqNewton.py:45-47, pcPIE.py:49-51)
             self.objectBuffer = self.object.copy()
             self.probeBuffer = self.probe.copy()
         if self.engine == "e3PIE":
@@ -517,7 +523,7 @@ class OracleRun:
             roll = lambda a: np.roll(a, (-yc, -xc), axis=(-2, -1))  # noqa: E731
             self.probe = roll(self.probe)
             self.object = roll(self.object)
-            if self.engine in ("mPIE", "multiPIE"):
+            if self.engine in ("mPIE", "multiPIE", "pcPIE", "mqNewton"):  # params.momentumAcceleration
                 self.probeMomentum, self.probeBuffer = roll(self.probeMomentum), roll(self.probeBuffer)
                 self.objectMomentum, self.objectBuffer = roll(self.objectMomentum), roll(self.objectBuffer)
         return yc, xc
@@ -538,6 +544,8 @@ class OracleRun:
         if self.absorbingProbeBoundary:
             a = self.absorbingProbeBoundaryAleph
             self.probe = (1 - a) * self.probe + a * self.probe * self.probeWindow
+        if self.positionCorrectionSwitch:
+            self._position_correction_update()
 
     def _apply_orthogonalization(self, loop):
         if self.orthogonalizationSwitch and loop % self.orthogonalizationFrequency == 0:
@@ -547,7 +555,7 @@ class OracleRun:
                     for isl in range(g.nslice):
                         modes, ev, V = orthogonalize_modes(self.probe[il, 0, :, isl])
                         self.probe[il, 0, :, isl] = modes
-                        if self.engine in ("mPIE", "multiPIE"):
+                        if self.engine in ("mPIE", "multiPIE", "pcPIE", "mqNewton"):
                             for buf in (self.probeBuffer, self.probeMomentum):
                                 p = buf[il, 0, :, isl].reshape(g.npsm, g.Np ** 2)
                                 buf[il, 0, :, isl] = (V @ p).reshape(g.npsm, g.Np, g.Np)
@@ -556,7 +564,7 @@ class OracleRun:
                     for isl in range(g.nslice):
                         modes, ev, V = orthogonalize_modes(self.object[il, :, 0, isl])
                         self.object[il, :, 0, isl] = modes
-                        if self.engine in ("mPIE", "multiPIE"):
+                        if self.engine in ("mPIE", "multiPIE", "pcPIE", "mqNewton"):
                             for buf in (self.objectBuffer, self.objectMomentum):
                                 p = buf[il, :, 0, isl].reshape(g.nosm, g.No ** 2)
                                 buf[il, :, 0, isl] = (V @ p).reshape(g.nosm, g.No, g.No)
@@ -578,7 +586,7 @@ class OracleRun:
             self.probeMomentum_v = self.probeMomentum.copy()
         F = self.ptychogram.shape[0]
         for loop in range(self.numIterations):
-            order = position_order(self.positionOrder, F, len(self.error), g.positions)
+            order = position_order(self.positionOrder, F, len(self.error), self.positions)
             trig = []
             for j, k in enumerate(order):
                 if positions_limit is not None and j >= positions_limit:
@@ -586,10 +594,12 @@ class OracleRun:
                 if self.engine == "e3PIE":
                     self._e3pie_step(k)
                 else:
-                    self._pie_step(k)
+                    patch = self._pie_step(k)
+                    if self.positionCorrectionSwitch and self.engine in ("mPIE", "pcPIE"):
+                        self._position_correction(k, patch)
                 if self.engine == "mqNewton":
                     self._adam_momentum(loop)
-                if self.engine in ("mPIE", "multiPIE"):
+                if self.engine in ("mPIE", "multiPIE", "pcPIE"):
                     t = bool(np.random.rand(1) > 0.95)
                     trig.append(t)
                     if t:
@@ -602,7 +612,7 @@ class OracleRun:
     def _pie_step(self, k):
         g = self.geo
         Np = int(g.Np)
-        row, col = g.positions[k]
+        row, col = self.positions[k]
         sy = slice(row, row + Np)
         sx = slice(col, col + Np)
         patch = self.object[..., sy, sx].copy()
@@ -634,11 +644,53 @@ class OracleRun:
             absO2 = np.abs(patch) ** 2
             Omax = np.max(np.sum(absO2, axis=(0, 1, 2, 3)), axis=(-1, -2))
             fracP = patch.conj() / (self.alphaProbe * Omax + (1 - self.alphaProbe) * absO2)
-            if self.engine == "mPIE":
+            if self.engine in ("mPIE", "pcPIE"):
                 weight = 1
                 self.probe = P + weight * self.betaProbe * np.sum(fracP * DELTA, axis=1, keepdims=True)
             else:
                 self.probe = P + self.betaProbe * np.sum(fracP * DELTA, axis=(0, 1), keepdims=True)
+        return patch
+
+    _PC_ROW = np.array([-1, -1, -1, 0, 0, 0, 1, 1, 1])  # BaseEngine.py:191-194
+    _PC_COL = np.array([-1, 0, 1, -1, 0, 1, -1, 0, 1])
+
+    def _position_correction(self, k, patch):
+        """positionCorrection (BaseEngine.py:1241-1323), direct branch
+        (radius < 2), CPM, after the update of position k."""
+        if len(self.error) <= 1:  # startAtIteration = 1
+            return
+        Np = int(self.geo.Np)
+        row, col = self.positions[k]
+        sy, sx = slice(row, row + Np), slice(col, col + Np)
+        O = self.object
+        cc = np.zeros((len(self._PC_ROW), 1))
+        for s in range(len(self._PC_ROW)):
+            t = np.roll(patch, self._PC_ROW[s], axis=-2)
+            sh = np.roll(t, self._PC_COL[s], axis=-1)
+            # a complex sum stored into a float array keeps its real part
+            cc[s] = np.squeeze(np.sum(sh.conj() * O[..., sy, sx], axis=(-2, -1))).real
+        cc = abs(cc)
+        normFactor = np.sum(patch.conj() * patch, axis=(-2, -1)).real
+        grad_x = 1000 * np.sum((cc.T - np.mean(cc)) / normFactor * self._PC_COL)
+        grad_y = 1000 * np.sum((cc.T - np.mean(cc)) / normFactor * self._PC_ROW)
+        r = 3
+        if abs(grad_x) > r:
+            grad_x = r * grad_x / abs(grad_x)
+        if abs(grad_y) > r:
+            grad_y = r * grad_y / abs(grad_y)
+        delta_p = 0.5 * np.array([grad_y, grad_x])
+        self.D[k, :] = delta_p + 0.9 * self.D[k, :]
+
+    def _position_correction_update(self):
+        """positionCorrectionUpdate (BaseEngine.py:1400-1438), CPM."""
+        if len(self.error) <= 1:
+            return
+        g = self.geo
+        new = self.encoder_corrected - 1 * self.D * g.dxo
+        new = new - new.mean(axis=0, keepdims=True)
+        new = new + g.encoder.mean(axis=0, keepdims=True)
+        self.encoder_corrected = new
+        self.positions = pixel_positions(new, g.dxo, g.No, g.Np)
 
     def _momentum(self):
         """objectMomentumUpdate/probeMomentumUpdate (mPIE.py:196-224,
@@ -732,6 +784,7 @@ PINNED = {
     "OracleRun.multiPIE": "identity with mPIE at nlambda=nosm=1 (no golden exists)",
     "OracleRun.e3PIE": "golden e3pie_multislice",
     "OracleRun.qNewton": "golden qnewton_single",
+    "OracleRun.pcPIE": "equals mPIE without position correction; the correction's gradient points back along a known shift",
     "OracleRun.mqNewton": "equals qNewton with betaObject_m = betaProbe_m = 0; ADAM's first step is g / (|g| + 1e-8)",
     "propagate(fresnel/polychromeasp)": "goldens propagator_fresnel / propagator_polychromeasp, epie_fresnel, epie_polychrome",
     "orthogonalize_modes": "orthogonality + descending eigenvalues + span invariants",

@@ -35,7 +35,7 @@ from ..Operators import (asp_tf_fftorder, aspw_transfer_function, polychrome_tf_
 # implement (BaseEngine.py:862, :1491)
 _UNSUPPORTED_TRUE = [
     "FourierMaskSwitch", "CPSCswitch", "backgroundModeSwitch", "adaptiveDenoisingSwitch",
-    "positionCorrectionSwitch", "objectTVregSwitch", "OPRP", "saveMemory", "l2reg",
+    "objectTVregSwitch", "OPRP", "saveMemory", "l2reg",
     "modulusEnforcedProbeSwitch", "probeSpectralPowerCorrectionSwitch", "PSDestimationSwitch",
     "probeSmoothenessSwitch", "objectSmoothenessSwitch", "absObjectSwitch", "absProbeSwitch",
     "objectContrastSwitch", "couplingSwitch", "binaryProbeSwitch", "TV_autofocus",
@@ -48,6 +48,9 @@ class BaseEngine:
     # mPIE / multiPIE draw one rand(1) per position for their momentum step
     # (mPIE.py:170-172); engines with momentum at every position draw none
     _MOMENTUM_TRIGGERS = True
+    # engines whose position loop calls positionCorrection (mPIE.py:163-166,
+    # pcPIE.py:107-108); with the switch on the others only recentre
+    _POSITION_CORRECTION = False
 
     def __init__(self, reconstruction, experimentalData, params, monitor):
         self.betaObject = 0.25
@@ -92,6 +95,13 @@ class BaseEngine:
             raise NotImplementedError("Fresnel with fftshiftSwitch=True is not implemented by the MI355X engines")
         if self.experimentalData.operationMode != "CPM":
             raise NotImplementedError("only CPM is implemented")
+        if p.positionCorrectionSwitch and self._POSITION_CORRECTION:
+            r = self.reconstruction
+            if getattr(p, "positionCorrectionSwitch_radius", 1) >= 2:
+                raise NotImplementedError("positionCorrectionSwitch_radius >= 2 (Fourier cross-correlation) is not "
+                                          "implemented by the MI355X engines")
+            if r.nlambda != 1 or r.nosm != 1 or r.nslice != 1:
+                raise NotImplementedError("position correction needs nlambda = nosm = nslice = 1")
 
     def _checkFFT(self):
         """Shift the data once for fftshiftSwitch (BaseEngine.py:405-455)."""
@@ -219,6 +229,11 @@ class BaseEngine:
         T["positions"] = torch.as_tensor(np.ascontiguousarray(r.positions, dtype=np.int32), device=dev)
         T["energy"] = torch.as_tensor(np.ascontiguousarray(np.asarray(d.energyAtPos, dtype=np.float32)), device=dev)
         orders, masks = self._schedule(n_iter, momentum and self._MOMENTUM_TRIGGERS)
+        self._n_err0 = len(getattr(r, "error", []))
+        if self.params.positionCorrectionSwitch and self._POSITION_CORRECTION:
+            # the search directions D start at zero (_initializePCParameters,
+            # BaseEngine.py:178-197)
+            T["pc_direction"] = torch.zeros((d.numFrames, 2), dtype=torch.float64, device=dev)
         self._orders, self._masks = orders, masks
         T["order"] = torch.as_tensor(orders, device=dev)
         T["mask"] = torch.as_tensor(masks, device=dev)
@@ -262,6 +277,9 @@ class BaseEngine:
         inst.momentum_mask = ptr("mask") if momentum else None
         inst.energy, inst.tf = ptr("energy"), ptr("tf")
         inst.err_at_pos, inst.err_iter = ptr("err_at_pos"), ptr("err_iter")
+        inst.pc_direction = ptr("pc_direction")
+        # positionCorrection runs once len(error) > startAtIteration = 1
+        inst.pc_start = max(0, 2 - int(getattr(self, "_n_err0", 0)))
         inst.workgroups = int(self.workgroups)
         return inst
 
@@ -307,6 +325,8 @@ class BaseEngine:
         if n_done:
             r.errorAtPos = T["err_at_pos"][n_done - 1].cpu().numpy()
         r._move_data_to_cpu()
+        if "pc_direction" in T:
+            self.D = T["pc_direction"].cpu().numpy()  # the search directions, as the reference's self.D
         for k in ("object", "probe", "objectBuffer", "probeBuffer", "objectMomentum", "probeMomentum"):
             if k in T:
                 setattr(r, k, T[k].cpu().numpy())
@@ -324,13 +344,14 @@ class BaseEngine:
     def _has_constraints(self):
         p = self.params
         return bool(p.orthogonalizationSwitch or p.probePowerCorrectionSwitch or p.probeBoundary
-                    or p.absorbingProbeBoundary
+                    or p.absorbingProbeBoundary or p.positionCorrectionSwitch
                     or (p.comStabilizationSwitch is not None and p.comStabilizationSwitch is not False))
 
     def applyConstraints(self, loop):
         """In the reference's order: orthogonalization, probe power
         correction, COM stabilisation, probe boundary, absorbing probe
-        boundary (BaseEngine.py:1456-1506)."""
+        boundary (BaseEngine.py:1456-1506), position correction update
+        (:1590-1591)."""
         p = self.params
         if p.orthogonalizationSwitch and loop % p.orthogonalizationFrequency == 0:
             self.orthogonalization()
@@ -343,6 +364,28 @@ class BaseEngine:
             self._window_probe(1.0)
         if p.absorbingProbeBoundary:
             self._window_probe(float(p.absorbingProbeBoundaryAleph))
+        if p.positionCorrectionSwitch:
+            self._position_correction_update(loop)
+
+    def _position_correction_update(self, loop):
+        """positionCorrectionUpdate (BaseEngine.py:1400-1438, CPM): once two
+        errors exist, encoder_corrected -= adaptStep D dxo (adaptStep = 1),
+        recentred on the measured encoder's mean; the new pixel positions go
+        to the device for the next launch."""
+        import torch
+
+        r, T = self.reconstruction, self._T
+        if self._n_err0 + loop + 1 <= 1:  # len(error) > startAtIteration (= 1)
+            return
+        D = T["pc_direction"].cpu().numpy() if "pc_direction" in T else np.zeros((self.experimentalData.numFrames, 2))
+        new = r.encoder_corrected - 1 * D * r.dxo
+        new = new - new.mean(axis=0, keepdims=True)
+        new = new + self.experimentalData.encoder.mean(axis=0, keepdims=True)
+        r.encoder_corrected = new
+        pos = np.asarray(r.positions)
+        if pos.min() < 0 or pos.max() + r.Np > r.No:
+            raise ValueError("position correction moved a patch outside the object")
+        T["positions"].copy_(torch.as_tensor(np.ascontiguousarray(pos, dtype=np.int32)))
 
     def _probe_window(self):
         """_probeWindow (BaseEngine.py:237-266) in float64, cast to float32:

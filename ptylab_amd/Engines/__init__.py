@@ -6,5 +6,6 @@ from .e3PIE import e3PIE  # noqa: F401
 from .ePIE import ePIE  # noqa: F401
 from .mPIE import mPIE  # noqa: F401
 from .mqNewton import mqNewton  # noqa: F401
+from .pcPIE import pcPIE  # noqa: F401
 from .multiPIE import multiPIE  # noqa: F401
 from .qNewton import qNewton  # noqa: F401

@@ -8,6 +8,7 @@ from .BaseEngine import BaseEngine
 
 class mPIE(BaseEngine):
     ENGINE = "mPIE"
+    _POSITION_CORRECTION = True
 
     def __init__(self, reconstruction, experimentalData, params, monitor):
         super().__init__(reconstruction, experimentalData, params, monitor)

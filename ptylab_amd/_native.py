@@ -13,7 +13,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libptyhip.so"
 
-PTY_ABI_VERSION = 2
+PTY_ABI_VERSION = 3
 PTY_FLAG_BATCHED = 1
 PTY_EPIE, PTY_MPIE, PTY_MULTIPIE, PTY_E3PIE, PTY_QNEWTON, PTY_MQNEWTON = 0, 1, 2, 3, 4, 5
 PTY_FRAUNHOFER, PTY_FRAUNHOFER_SHIFTED, PTY_ASP, PTY_FRESNEL, PTY_POLYCHROME_ASP, PTY_SCALED_ASP = 0, 1, 2, 3, 4, 5
@@ -21,7 +21,7 @@ PTY_SCALED_POLYCHROME_ASP = 6
 PTY_ETIMEDOUT = -110
 
 ENGINE_IDS = {"ePIE": PTY_EPIE, "mPIE": PTY_MPIE, "multiPIE": PTY_MULTIPIE, "e3PIE": PTY_E3PIE, "qNewton": PTY_QNEWTON,
-              "mqNewton": PTY_MQNEWTON}
+              "mqNewton": PTY_MQNEWTON, "pcPIE": PTY_MPIE}
 
 EXPORTED = (
     "pty_abi_version",
@@ -82,6 +82,8 @@ class PtyInstance(ctypes.Structure):
         ("err_iter", ctypes.c_void_p),
         ("workgroups", ctypes.c_int32),
         ("flags", ctypes.c_int32),
+        ("pc_direction", ctypes.c_void_p),
+        ("pc_start", ctypes.c_int32),
     ]
 
 

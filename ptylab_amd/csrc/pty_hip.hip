@@ -43,6 +43,8 @@
 
 using namespace pty;
 
+static_assert(sizeof(pty_instance) == 208, "pty_instance layout (ABI 3) changed: update the bindings");
+
 #define NTHREADS 256
 #define NWAVE (NTHREADS / 64)
 #define MAX_SLICES 15
@@ -52,12 +54,14 @@ using namespace pty;
 #define CTRL_TOP 0       // counter barrier: arrivals of complete groups
 #define CTRL_FAIL 1      // set when a cluster barrier timed out (every workgroup then leaves)
 #define CTRL_PLACE 2     // set when a workgroup of an XCD-local cluster found no slot on its XCD
+#define CTRL_BADPOS 3    // set when a position (rewritten by the caller between launches) left the object
 #define CTRL_GROUP 32    // counter barrier: 8 group arrival counters, 32 words apart
 #define CTRL_RELEASE 288 // counter barrier: published generation
 #define CTRL_WORDS 320
 #define SLOT_WORDS 32  // words of per-position maxima in each of the 4 ring slots
 
 #define N_XCD 8  // XCDs of an MI355X (gfx950), each with its own L2
+#define PC_TERMS 19  // position correction: 9 complex correlations and the patch norm
 
 // pass families of the persistent kernel
 #define FAM_PIE_FRAUNHOFER 0
@@ -116,6 +120,7 @@ struct DevInst {
   unsigned* bar;     // flag barrier: generation reached by each workgroup (G words)
   double* adampart;  // mqNewton: per-workgroup FP64 partials of |gradient|^2, (2, G)
   double* adamv;     // mqNewton: ADAM second moments of object and probe, (2 parities, 2)
+  double* pcpart;    // position correction: per-workgroup FP64 partials, (G, PC_TERMS)
   unsigned osel[32]; // fields whose object terms the owner of field f's object sums (bit g)
   unsigned psel[32]; // fields whose probe terms the owner of field f's probe sums (bit g)
   int gsize[8];      // counter barrier: workgroups of this instance in each block group b % 8
@@ -927,7 +932,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       }
 #pragma unroll
       for (int l = 0; l < L; ++l) nv[l] = cadd(of[l], cscale(acc[l], bO));
-      if (qn) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
+      // the pre-update patch: qNewton's probe step and the position correction read it
+      if (qn || D.pcpart) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
       store_row<NP>(R, O + x.fo * No2 + orow_off, nv, R.t);
     }
     if (act && x.io == 0 && !qn) {
@@ -1108,6 +1114,86 @@ __device__ __forceinline__ void adam_update_pass(Run& R, int itr, unsigned long 
       cstore(R, B + i, xn);
     }
   }
+}
+
+// Position correction (BaseEngine.positionCorrection, BaseEngine.py:1241-1323,
+// the direct branch of positionCorrectionSwitch_radius < 2): cc_s =
+// |Re sum conj(roll(Opatch, (rs, cs))) O[sy, sx]| (the reference stores the
+// complex sum into a real array, :1256 and :1270-1272, which keeps the real
+// part) for the 9 shifts rs, cs in
+// {-1, 0, 1} (rowShifts / colShifts of :191-194), Opatch the patch before
+// this position's update and O[sy, sx] the updated one; normFactor =
+// sum |Opatch|^2.  Each workgroup sums its rows in FP64; the partials of a
+// position are combined by workgroup 0 after a barrier (pc_direction_update).
+template <int NP>
+__device__ __forceinline__ void pc_corr_pass(Run& R, int k) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int No = D.a.No;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  const float2* O = (const float2*)D.a.object;
+  const float2* OP = D.OP;
+  double acc[PC_TERMS];
+#pragma unroll
+  for (int s = 0; s < PC_TERMS; ++s) acc[s] = 0.0;
+  const int per = NP / R.G;
+  const int line0 = R.g * per;
+  for (int li = R.wave * Line<NP>::SUB + R.sub; li < per; li += R.nwave * Line<NP>::SUB) {
+    const int i = line0 + li;
+    float2 on[L];
+    load_row<NP>(R, on, O + (size_t)(row0 + i) * No + col0, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int j = l * LPF + R.t;
+#pragma unroll
+      for (int s = 0; s < 9; ++s) {
+        const int rs = s / 3 - 1, cs = s % 3 - 1;
+        // roll(x, r)[i] = x[i - r] on the periodic patch
+        const float2 v = cload(R, OP + (size_t)((i - rs) & (NP - 1)) * NP + ((j - cs) & (NP - 1)));
+        // conj(v) * on
+        acc[2 * s] += (double)v.x * on[l].x + (double)v.y * on[l].y;
+        acc[2 * s + 1] += (double)v.x * on[l].y - (double)v.y * on[l].x;
+        if (s == 4) acc[18] += (double)v.x * v.x + (double)v.y * v.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < PC_TERMS; ++s) {
+    const double t = block_sum(R, acc[s]);
+    if (threadIdx.x == 0) cstored(R, D.pcpart + (size_t)R.g * PC_TERMS + s, t);
+  }
+}
+
+// D[k] = daleth clip(grad) + beth D[k] with daleth = 0.5, beth = 0.9,
+// grad = betaGrad sum_s (cc_s - mean cc) / normFactor (colShift_s,
+// rowShift_s), betaGrad = 1000, clipped at r = 3 pixels
+// (BaseEngine.py:1305-1323, parameters :179-195); workgroup 0, thread 0.
+__device__ __forceinline__ void pc_direction_update(Run& R, int k) {
+  const DevInst& D = *R.D;
+  if (R.g != 0 || threadIdx.x != 0) return;
+  double t[PC_TERMS];
+  for (int s = 0; s < PC_TERMS; ++s) t[s] = 0.0;
+  for (int g = 0; g < R.G; ++g)
+    for (int s = 0; s < PC_TERMS; ++s) t[s] += cloadd(R, D.pcpart + (size_t)g * PC_TERMS + s);
+  double cc[9], mean = 0.0;
+  for (int s = 0; s < 9; ++s) {
+    cc[s] = fabs(t[2 * s]);
+    mean += cc[s];
+  }
+  mean /= 9.0;
+  double gx = 0.0, gy = 0.0;
+  for (int s = 0; s < 9; ++s) {
+    const double w = (cc[s] - mean) / t[18];
+    gx += w * (double)(s % 3 - 1);
+    gy += w * (double)(s / 3 - 1);
+  }
+  gx *= 1000.0;
+  gy *= 1000.0;
+  if (fabs(gx) > 3.0) gx = 3.0 * gx / fabs(gx);
+  if (fabs(gy) > 3.0) gy = 3.0 * gy / fabs(gy);
+  double* Dk = D.a.pc_direction + 2 * (size_t)k;
+  Dk[0] = 0.5 * gy + 0.9 * Dk[0];
+  Dk[1] = 0.5 * gx + 0.9 * Dk[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -1343,6 +1429,13 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   }
   pie_rows_update<NP, QN, ASP, QPH>(R, k, slot);
   if (sync_after && !cluster_sync(R)) return false;
+  if constexpr (!QN) {
+    if (D.pcpart && it >= D.a.pc_start) {
+      pc_corr_pass<NP>(R, k);
+      if (!cluster_sync(R)) return false;
+      pc_direction_update(R, k);
+    }
+  }
   if constexpr (QN) {
     qn_omax_pass<NP>(R, slot);
     if (!cluster_sync(R)) return false;
@@ -1402,6 +1495,16 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
     const int j1 = it == it1 - 1 ? (je < F ? je : F) : F;
     for (int j = j0; j < j1; ++j, ++q) {
       const int k = order[j];
+      {
+        // positions are checked at plan creation, but a caller may rewrite
+        // them between launches (position correction); every workgroup of
+        // the cluster reads the same value and leaves at the same point
+        const int pr = D.a.positions[2 * k], pc = D.a.positions[2 * k + 1];
+        if (pr < 0 || pc < 0 || pr + NP > D.a.No || pc + NP > D.a.No) {
+          if (threadIdx.x == 0) __hip_atomic_store(D.ctrl + CTRL_BADPOS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          return;
+        }
+      }
       unsigned* slot = D.mx + (size_t)(q & 3u) * SLOT_WORDS * R.G;
       unsigned* zslot = D.mx + (size_t)((q + 2u) & 3u) * SLOT_WORDS * R.G;
       const bool mom = FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j];
@@ -1419,7 +1522,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
           kn = order[j + 1];
         else if (j1 == F && it + 1 < it1)
           kn = D.a.order[(size_t)(it + 1) * F];
-        if (!mom && kn >= 0) {
+        if (!mom && kn >= 0 && !(D.pcpart && it >= D.a.pc_start)) {
           const int dr = D.a.positions[2 * k] - D.a.positions[2 * kn];
           const int dc = D.a.positions[2 * k + 1] - D.a.positions[2 * kn + 1];
           sync_after = !(dr >= NP || -dr >= NP || dc >= NP || -dc >= NP);
@@ -1914,6 +2017,10 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 int validate(const pty_instance& a, int idx, int nt) {
   if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
   if (a.engine < PTY_EPIE || a.engine > PTY_MQNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
+  if (a.pc_direction && (a.engine == PTY_E3PIE || a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON))
+    return set_err(PTY_EINVAL, "instance %d: position correction runs with the ePIE, mPIE and multiPIE engines", idx);
+  if (a.pc_direction && (a.nlambda != 1 || a.nosm != 1 || a.nslice != 1))
+    return set_err(PTY_EINVAL, "instance %d: position correction needs nlambda = nosm = nslice = 1", idx);
   if (a.engine == PTY_MQNEWTON && (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
     return set_err(PTY_EINVAL, "instance %d: mqNewton needs buffer and momentum arrays", idx);
   if (a.propagator < PTY_FRAUNHOFER || a.propagator > PTY_SCALED_POLYCHROME_ASP)
@@ -2113,9 +2220,13 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       if ((rc = dalloc(p, &mem, (size_t)a.nslice * d.nF * np2 * sizeof(float2)))) goto fail;
       d.ESW = (float2*)mem;
     }
-    if (a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON) {
+    if (a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON || a.pc_direction) {
       if ((rc = dalloc(p, &mem, (size_t)d.nO * np2 * sizeof(float2)))) goto fail;
       d.OP = (float2*)mem;
+    }
+    if (a.pc_direction) {
+      if ((rc = dalloc(p, &mem, (size_t)G * PC_TERMS * sizeof(double)))) goto fail;
+      d.pcpart = (double*)mem;
     }
     if (a.engine == PTY_MQNEWTON) {
       if ((rc = dalloc(p, &mem, (size_t)2 * G * sizeof(double)))) goto fail;
@@ -2403,8 +2514,9 @@ int pty_plan_check(pty_plan* p, void* hip_stream) {
   hipStream_t st = (hipStream_t)hip_stream;
   HIP_TRY(hipStreamSynchronize(st));
   for (size_t i = 0; i < p->h_insts.size(); ++i) {
-    unsigned w[3] = {0u, 0u, 0u};
+    unsigned w[4] = {0u, 0u, 0u, 0u};
     HIP_TRY(hipMemcpy(w, p->h_insts[i].ctrl, sizeof w, hipMemcpyDeviceToHost));
+    if (w[CTRL_BADPOS]) return set_err(PTY_EINVAL, "instance %zu: a position puts the patch outside the object", i);
     if (w[CTRL_PLACE])
       return set_err(PTY_EDEVICE, "instance %zu: a workgroup of its XCD-local cluster ran on another XCD", i);
     if (w[CTRL_FAIL]) return set_err(PTY_ETIMEDOUT, "instance %zu: a cluster barrier timed out", i);

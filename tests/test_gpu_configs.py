@@ -216,3 +216,31 @@ def test_mqnewton_on_photon_scaled_inputs(c0):
         ref = run_oracle(d, "mqNewton", "Fraunhofer", (1, 1, 1, 1), iters, 3, order="random", No=c0.No)
         eo, ep, ee = _report(f"C0 mqNewton {iters} it", gpu, ref)
         assert eo <= bar and ep <= bar and ee <= 1e-3
+
+
+@pytest.mark.parametrize("engine", ["mPIE", "pcPIE"])
+def test_position_correction_against_oracle(c0, engine):
+    """Position correction (BaseEngine.py:1241-1323, 1400-1438) on the GPU:
+    per position the 9 shifted correlations of the pre-update with the updated
+    patch (FP64 sums), the search directions D, and between launches the
+    encoder and pixel positions -- against the oracle after 4 iterations (the
+    correction runs from the third)."""
+    from ptyhelpers import build_package
+
+    d = c0.as_dict()
+    sw = dict(positionCorrectionSwitch=True)
+    eng = build_package(d, engine, "Fraunhofer", (1, 1, 1, 1), 4, 3, order="random", No=c0.No, switches=sw)
+    np.random.seed(3)
+    eng.reconstruct()
+    gpu = eng.reconstruction
+    ref = run_oracle(d, engine, "Fraunhofer", (1, 1, 1, 1), 4, 3, order="random", No=c0.No, switches=sw)
+    dD = np.max(np.abs(eng.D - ref.D))
+    moved = int(np.count_nonzero(ref.D))
+    print(f"{engine} position correction: max |D_gpu - D_oracle| {dD:.2e} px, nonzero directions {moved}, "
+          f"positions equal {np.array_equal(gpu.positions, ref.positions)}", flush=True)
+    assert moved > 0
+    assert dD < 5e-2
+    np.testing.assert_allclose(gpu.encoder_corrected, ref.encoder_corrected, rtol=0, atol=5e-2 * float(ref.geo.dxo))
+    assert np.array_equal(gpu.positions, ref.positions)
+    eo, ep, ee = _report(f"C0 {engine} position correction", gpu, ref)
+    assert eo <= 1e-4 and ep <= 1e-4 and ee <= 1e-3
