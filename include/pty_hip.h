@@ -115,7 +115,9 @@ typedef struct pty_instance {
    * updated patch and set D[k] = 0.5 clip(grad, 3) + 0.9 D[k], from
    * iteration pc_start of the plan on.  Needs nlambda = nosm = nslice = 1.
    * The positions are read at every position, so the caller may rewrite
-   * them between launches (positionCorrectionUpdate, BaseEngine.py:1400-1438). */
+   * them between launches (positionCorrectionUpdate, BaseEngine.py:1400-1438);
+   * with pc_direction set they are checked against the object at every
+   * position.  Instances with pc_direction never join the batched launch. */
   double* pc_direction;
   int32_t pc_start;
 } pty_instance;
@@ -174,8 +176,8 @@ int pty_plan_run_positions(pty_plan* plan, int32_t it, int32_t j_begin, int32_t 
  * Read the plan's device-side failure flags; synchronises the stream.
  * Returns PTY_ETIMEDOUT if a cluster barrier gave up, PTY_EDEVICE if a
  * workgroup of the batched launch found no slot on its XCD, PTY_EINVAL if a
- * position rewritten between launches put its patch outside the object (that
- * instance stopped at it).
+ * position of an instance with position correction, rewritten between
+ * launches, put its patch outside the object (that instance stopped at it).
  */
 int pty_plan_check(pty_plan* plan, void* hip_stream);
 

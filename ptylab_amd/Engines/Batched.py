@@ -39,6 +39,9 @@ class Batched:
         import torch
 
         for e in self.engines:
+            if e.params.positionCorrectionSwitch and e._POSITION_CORRECTION:
+                raise NotImplementedError("position correction is not run in batched launches; call the engine's "
+                                          "own reconstruct()")
             e._prepareReconstruction()
         staged = [e._stage() for e in self.engines]
         insts = [s[0] for s in staged]

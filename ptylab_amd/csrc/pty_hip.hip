@@ -70,7 +70,16 @@ static_assert(sizeof(pty_instance) == 208, "pty_instance layout (ABI 3) changed:
 #define FAM_QN_FRAUNHOFER 3  // qNewton: its own instantiations keep the PIE kernels' registers
 #define FAM_QN_ASP 4
 #define FAM_PIE_SASP 5  // PIE with scaled ASP: the ASP passes plus a phase in the row passes
-#define N_FAMILIES 6
+// PIE families with the position correction pass (their own instantiations,
+// so its FP64 accumulators stay out of the other kernels' registers); per
+// probe size only, never in the batched launch
+#define FAM_PC_FRAUNHOFER 6
+#define FAM_PC_ASP 7
+#define FAM_PC_SASP 8
+#define N_FAMILIES 9
+__host__ __device__ constexpr int base_family(int fam) {
+  return fam == FAM_PC_FRAUNHOFER ? FAM_PIE_FRAUNHOFER : fam == FAM_PC_ASP ? FAM_PIE_ASP : fam == FAM_PC_SASP ? FAM_PIE_SASP : fam;
+}
 
 namespace {
 
@@ -1410,7 +1419,7 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 // per-position drivers
 // ---------------------------------------------------------------------------
-template <int NP, bool ASP, int CWR, bool QN, bool QPH>
+template <int NP, bool ASP, int CWR, bool QN, bool QPH, bool PC>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
                                              bool sync_after, int it) {
   const DevInst& D = *R.D;
@@ -1429,8 +1438,8 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   }
   pie_rows_update<NP, QN, ASP, QPH>(R, k, slot);
   if (sync_after && !cluster_sync(R)) return false;
-  if constexpr (!QN) {
-    if (D.pcpart && it >= D.a.pc_start) {
+  if constexpr (PC) {
+    if (it >= D.a.pc_start) {
       pc_corr_pass<NP>(R, k);
       if (!cluster_sync(R)) return false;
       pc_direction_update(R, k);
@@ -1481,6 +1490,8 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
 // error of an iteration is reduced when its last position has run.
 template <int NP, int FAM, int CWR>
 __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
+  constexpr int BF = base_family(FAM);  // pass family without the position correction
+  constexpr bool PC = BF != FAM;
   const DevInst& D = *R.D;
   const int F = D.a.F;
   const int engine = D.a.engine;
@@ -1495,10 +1506,11 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
     const int j1 = it == it1 - 1 ? (je < F ? je : F) : F;
     for (int j = j0; j < j1; ++j, ++q) {
       const int k = order[j];
-      {
-        // positions are checked at plan creation, but a caller may rewrite
-        // them between launches (position correction); every workgroup of
-        // the cluster reads the same value and leaves at the same point
+      if constexpr (PC) {
+        // positions are checked at plan creation; with the position
+        // correction the caller rewrites them between launches, so they are
+        // checked again here; every workgroup of the cluster reads the same
+        // value and leaves at the same point
         const int pr = D.a.positions[2 * k], pc = D.a.positions[2 * k + 1];
         if (pr < 0 || pc < 0 || pr + NP > D.a.No || pc + NP > D.a.No) {
           if (threadIdx.x == 0) __hip_atomic_store(D.ctrl + CTRL_BADPOS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1507,7 +1519,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       }
       unsigned* slot = D.mx + (size_t)(q & 3u) * SLOT_WORDS * R.G;
       unsigned* zslot = D.mx + (size_t)((q + 2u) & 3u) * SLOT_WORDS * R.G;
-      const bool mom = FAM != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j];
+      const bool mom = BF != FAM_E3PIE && momentum && D.a.momentum_mask[(size_t)it * F + j];
       // The barrier after the update pass orders this position's object
       // writes before the next position's patch reads by other workgroups;
       // every other cross-workgroup dependency between the two (probe rows,
@@ -1516,25 +1528,25 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
       // does not overlap this one and no momentum step follows, the PIE
       // engines skip it.
       bool sync_after = true;
-      if constexpr (FAM == FAM_PIE_FRAUNHOFER || FAM == FAM_PIE_ASP || FAM == FAM_PIE_SASP) {
+      if constexpr (BF == FAM_PIE_FRAUNHOFER || BF == FAM_PIE_ASP || BF == FAM_PIE_SASP) {
         int kn = -1;
         if (j + 1 < j1)
           kn = order[j + 1];
         else if (j1 == F && it + 1 < it1)
           kn = D.a.order[(size_t)(it + 1) * F];
-        if (!mom && kn >= 0 && !(D.pcpart && it >= D.a.pc_start)) {
+        if (!mom && kn >= 0 && !(PC && it >= D.a.pc_start)) {
           const int dr = D.a.positions[2 * k] - D.a.positions[2 * kn];
           const int dc = D.a.positions[2 * k + 1] - D.a.positions[2 * kn + 1];
           sync_after = !(dr >= NP || -dr >= NP || dc >= NP || -dc >= NP);
         }
       }
       bool ok;
-      if constexpr (FAM == FAM_E3PIE)
+      if constexpr (BF == FAM_E3PIE)
         ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
       else
-        ok = pie_position<NP, FAM == FAM_PIE_ASP || FAM == FAM_QN_ASP || FAM == FAM_PIE_SASP, CWR,
-                          FAM == FAM_QN_FRAUNHOFER || FAM == FAM_QN_ASP, FAM == FAM_PIE_SASP>(R, k, j, slot, zslot, ep,
-                                                                                         sync_after, it);
+        ok = pie_position<NP, BF == FAM_PIE_ASP || BF == FAM_QN_ASP || BF == FAM_PIE_SASP, CWR,
+                          BF == FAM_QN_FRAUNHOFER || BF == FAM_QN_ASP, BF == FAM_PIE_SASP, PC>(R, k, j, slot, zslot, ep,
+                                                                                              sync_after, it);
       if (!ok) return;
       if (mom) {
         momentum_pass<NP>(R);
@@ -1910,6 +1922,9 @@ static const void* iterate_kernel_fam(int fam) {
     case FAM_PIE_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_PIE_FRAUNHOFER>;
     case FAM_PIE_ASP: return (const void*)pty_iterate_kernel<NP, FAM_PIE_ASP>;
     case FAM_PIE_SASP: return (const void*)pty_iterate_kernel<NP, FAM_PIE_SASP>;
+    case FAM_PC_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_PC_FRAUNHOFER>;
+    case FAM_PC_ASP: return (const void*)pty_iterate_kernel<NP, FAM_PC_ASP>;
+    case FAM_PC_SASP: return (const void*)pty_iterate_kernel<NP, FAM_PC_SASP>;
     case FAM_E3PIE: return (const void*)pty_iterate_kernel<NP, FAM_E3PIE>;
     case FAM_QN_FRAUNHOFER: return (const void*)pty_iterate_kernel<NP, FAM_QN_FRAUNHOFER>;
     case FAM_QN_ASP: return (const void*)pty_iterate_kernel<NP, FAM_QN_ASP>;
@@ -1939,7 +1954,13 @@ static const void* iterate_kernel_ptr(int np, int fam) {
   return nullptr;
 }
 
+static int family_of_base(const pty_instance& a);
 static int family_of(const pty_instance& a) {
+  const int f = family_of_base(a);
+  if (!a.pc_direction) return f;
+  return f == FAM_PIE_FRAUNHOFER ? FAM_PC_FRAUNHOFER : f == FAM_PIE_ASP ? FAM_PC_ASP : f == FAM_PIE_SASP ? FAM_PC_SASP : f;
+}
+static int family_of_base(const pty_instance& a) {
   if (a.engine == PTY_E3PIE) return FAM_E3PIE;
   if (a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON) return prop_asp(a.propagator) ? FAM_QN_ASP : FAM_QN_FRAUNHOFER;
   if (a.propagator == PTY_SCALED_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP) return FAM_PIE_SASP;
@@ -1999,7 +2020,8 @@ int xcd_capacity(int fam, int* cap) {
 }
 
 bool joins_batch(const pty_instance& a, int ninst) {
-  return a.Np <= BATCH_MAX_NP && (ninst > 1 || (a.flags & PTY_FLAG_BATCHED));
+  // the position correction runs in its own per-size kernels only
+  return a.Np <= BATCH_MAX_NP && !a.pc_direction && (ninst > 1 || (a.flags & PTY_FLAG_BATCHED));
 }
 
 int auto_workgroups(const pty_instance& a, int ninst, int nt) {
