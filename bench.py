@@ -52,6 +52,9 @@ def parse_args():
     ap.add_argument("--engine", choices=("mPIE", "ePIE"), default="mPIE", help="engine of the batched workload")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true", help="batched: direct launches instead of graph replays")
+    ap.add_argument("--batched-inputs", choices=("asp_modes", "fraunhofer"), default="asp_modes",
+                    help="batched: instances generated as the single mPIE ASP workload (ASP, 4 probe modes; "
+                         "default) or as the plain-ePIE family (Fraunhofer, one mode)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy before the timed region (profiler counter scale)")
     return ap.parse_args()
@@ -205,19 +208,21 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline_batched(workloads, engine, budget_s=15.0):
+def cpu_baseline_batched(workloads, engine, propagator, npsm, budget_s=15.0):
     """The NumPy oracle on one host core over the batch's instances in order,
-    one iteration each, until about budget_s seconds of work."""
+    one iteration each (at most the positions left in the budget's estimate),
+    until about budget_s seconds of work."""
     import oracle as O
 
     done, t_all = 0, 0.0
     names = []
     for w in workloads:
-        geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+        geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No,
+                         npsm=npsm)
         np.random.seed(1)
         obj = O.initial_guess(geo.shape_O, "ones", geo)
         pr = O.initial_guess(geo.shape_P, "circ", geo)
-        run = O.OracleRun(geo, obj, pr, engine=engine, propagator="Fraunhofer", numIterations=1)
+        run = O.OracleRun(geo, obj, pr, engine=engine, propagator=propagator, numIterations=1)
         t0 = time.perf_counter()
         run.reconstruct()
         t_all += time.perf_counter() - t0
@@ -230,7 +235,8 @@ def cpu_baseline_batched(workloads, engine, budget_s=15.0):
         "unit": "position updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"NumPy oracle, {engine} Fraunhofer, one iteration of the first {len(names)} batch instances "
+        "sample": f"NumPy oracle, {engine} {propagator} npsm={npsm}, one iteration of the first {len(names)} "
+                  f"batch instances "
                   f"({', '.join(names)}; {done} positions, {t_all:.1f} s)",
     }
 
@@ -238,9 +244,11 @@ def cpu_baseline_batched(workloads, engine, budget_s=15.0):
 def main_batched(args):
     """Configuration "batched": independent mPIE (or ePIE) reconstructions,
     Np uniform in {64, 128, 256}, No = 4 Np, F uniform in [100, 800], inputs
-    as in the plain-ePIE family, one launch per iteration replayed as a HIP
-    graph, cluster sizes from the autotuner.  A step is one iteration of
-    every instance; value = K * sum(F) / elapsed."""
+    generated as in the single mPIE ASP workload (ASP, 4 probe modes, 1e5
+    photons, per-instance seeds; --batched-inputs fraunhofer for the
+    plain-ePIE family), one launch per iteration replayed as a HIP graph,
+    cluster sizes from the autotuner.  A step is one iteration of every
+    instance; value = K * sum(F) / elapsed."""
     import torch
 
     import ptylab_amd as P
@@ -250,7 +258,8 @@ def main_batched(args):
     # inputs first: the generator forks worker processes, which must happen
     # before this process initialises the GPU
     t_gen = time.perf_counter()
-    ws = S.batched_instances(n=args.instances, workers=min(16, os.cpu_count() or 1))
+    ws = S.batched_instances(n=args.instances, workers=min(16, os.cpu_count() or 1), inputs=args.batched_inputs)
+    prop, npsm = ("ASP", 4) if args.batched_inputs == "asp_modes" else ("Fraunhofer", 1)
     print(f"[bench] {len(ws)} workloads generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     if not torch.cuda.is_available():
         print("no HIP GPU available", file=sys.stderr)
@@ -263,13 +272,16 @@ def main_batched(args):
             setattr(data, k, v)
         data._setData()
         params = P.Params()
-        params.propagatorType = "Fraunhofer"
+        params.propagatorType = prop
         params.positionOrder = "random"
         rec = P.Reconstruction(data, params)
         rec.No = w.No
+        rec.npsm = npsm
         np.random.seed(1000 + i)
         rec.initializeObjectProbe()
         eng = getattr(P.Engines, args.engine)(rec, data, params, P.DummyMonitor())
+        if args.engine == "mPIE":
+            eng.frictionM, eng.feedbackM = 0.7, 0.3
         eng.numIterations = W + K
         engines.append(eng)
     np.random.seed(3)
@@ -308,7 +320,7 @@ def main_batched(args):
     bytes_iter = 0.0
     for e, w in zip(engines, ws):
         frac = float(e._masks[W:W + K].sum()) / (K * e.experimentalData.numFrames) if momentum else 0.0
-        bytes_iter += e.experimentalData.numFrames * bytes_per_position(w.Np, w.No, 1, 1, 1, 1, frac)
+        bytes_iter += e.experimentalData.numFrames * bytes_per_position(w.Np, w.No, 1, 1, npsm, 1, frac)
     for e in engines:
         e._collect(W + K)
     plan.close()
@@ -329,7 +341,7 @@ def main_batched(args):
         "dtype": "c64",
         "data": "synthetic",
         "config": {
-            "workload": f"{len(ws)} independent {args.engine} reconstructions, Fraunhofer, Np in {nps}, No=4Np, "
+            "workload": f"{len(ws)} independent {args.engine} reconstructions, {prop}, npsm={npsm}, Np in {nps}, No=4Np, "
                         f"F uniform in [100, 800] (sum {int(Fs.sum())}), one launch per iteration"
                         + (" (graph replay)" if not args.no_graph else " (direct launch)"),
             "positions_per_step": int(Fs.sum()),
@@ -351,7 +363,7 @@ def main_batched(args):
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline_batched(ws, args.engine)
+        out["cpu_baseline"] = cpu_baseline_batched(ws, args.engine, prop, npsm)
     print(json.dumps(out))
 
 

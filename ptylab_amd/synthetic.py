@@ -202,24 +202,31 @@ def cpm_multislice(Np=128, No=512, F=400, nslice=4, dz=5e-6, seed=20, peak=1e4, 
                     slices[None, None, None], probe0[None, None, None, None], corners, No, Np)
 
 
-def batched_instance(i, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800):
+def batched_instance(i, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, inputs="asp_modes"):
     """Instance i of the batched configuration: Np uniform in `nps`,
-    No = 4 Np, F uniform in [fmin, fmax], inputs as in the plain-ePIE
-    family, all drawn from seed seed0 + i."""
+    No = 4 Np, F uniform in [fmin, fmax], all drawn from seed seed0 + i.
+    inputs="asp_modes": generated as the single mPIE ASP workload
+    (cpm_asp_modes: 4 Hermite-Gaussian x aperture modes of radius Np/4,
+    near-field ASP at 50 mm, 1e5 photons); inputs="fraunhofer": the plain-ePIE
+    family (cpm_fraunhofer, one mode)."""
     rng = np.random.default_rng(seed0 + i)
     Np = int(rng.choice(nps))
     F = int(rng.integers(fmin, fmax + 1))
     No = 4 * Np
     lim = (No - Np) // 2 - 1
     step = min(0.25 * Np, 0.9 * lim * np.sqrt(np.pi / F))
-    return cpm_fraunhofer(Np=Np, No=No, F=F, seed=seed0 + i, step_frac=step / Np)
+    if inputs == "asp_modes":
+        return cpm_asp_modes(Np=Np, No=No, F=F, npsm=4, seed=seed0 + i, probe_radius_px=Np // 4, step_px=step)
+    if inputs == "fraunhofer":
+        return cpm_fraunhofer(Np=Np, No=No, F=F, seed=seed0 + i, step_frac=step / Np)
+    raise ValueError(f"unknown batched inputs {inputs!r}")
 
 
-def batched_instances(n=64, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, workers=1):
+def batched_instances(n=64, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, workers=1, inputs="asp_modes"):
     """Config "batched": n instances (batched_instance(i), i < n).  With
     workers > 1 they are generated by a pool of forked processes; call it
     before the process initialises the GPU."""
-    args = [(i, seed0, nps, fmin, fmax) for i in range(n)]
+    args = [(i, seed0, nps, fmin, fmax, inputs) for i in range(n)]
     if workers <= 1:
         return [batched_instance(*a) for a in args]
     import multiprocessing as mp

@@ -140,3 +140,31 @@ def test_plan_reports_active_workgroups(datasets):
     _, b = _batched(datasets, workgroups=[1, 1, 2, 4, 2, 1], graph=True)
     assert b.last_workgroups == [1, 1, 2, 4, 2, 1]
     assert b.last_plan_workgroups == sum(b.last_workgroups)
+
+
+def test_batched_asp_four_modes_equals_separate_runs():
+    """The benchmark's batched inputs: mPIE with ASP and 4 probe modes at
+    Np = 128 and 256 (XCD-local clusters of 4 and 16 workgroups) leave
+    object and probe bit-identical to separate runs."""
+    import ptylab_amd as P
+
+    cases = [(256, 1024, 24, 2, 7, 16), (128, 512, 30, 3, 8, 4)]
+    ws = [S.cpm_asp_modes(Np=Np, No=No, F=F, npsm=4, seed=seed, probe_radius_px=Np // 4)
+          for Np, No, F, iters, seed, _ in cases]
+
+    def engines():
+        return [build_package(w.as_dict(), "mPIE", "ASP", (1, 1, 4, 1), c[3], c[4], order="random", No=w.No)
+                for w, c in zip(ws, cases)]
+
+    sep = engines()
+    np.random.seed(78)
+    for e in sep:
+        e.reconstruct()
+    bat = engines()
+    np.random.seed(78)
+    P.Engines.Batched(bat, workgroups=[c[5] for c in cases]).reconstruct()
+    for a, b, c in zip(sep, bat, cases):
+        ra, rb = a.reconstruction, b.reconstruction
+        assert np.array_equal(np.asarray(ra.object), np.asarray(rb.object)), c
+        assert np.array_equal(np.asarray(ra.probe), np.asarray(rb.probe)), c
+        np.testing.assert_allclose(np.asarray(ra.error), np.asarray(rb.error), rtol=1e-12)
