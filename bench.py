@@ -292,7 +292,7 @@ def main_batched(args):
     for inst in insts:
         inst.flags = P._native.PTY_FLAG_BATCHED
     t_tune = time.perf_counter()
-    G = autotune.tune(insts)
+    G = autotune.tune(insts, tensors=[s_[1] for s_ in staged])
     print(f"[bench] autotuned cluster sizes in {time.perf_counter() - t_tune:.1f} s: total {sum(G)} workgroups",
           file=sys.stderr, flush=True)
     for inst, g in zip(insts, G):

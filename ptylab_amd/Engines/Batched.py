@@ -52,7 +52,7 @@ class Batched:
             if len(G) != len(insts):
                 raise ValueError("one workgroup count per engine")
         else:
-            G = autotune.tune(insts, self.tuner)
+            G = autotune.tune(insts, self.tuner, [s[1] for s in staged])
         for inst, g in zip(insts, G):
             inst.workgroups = g
         self.last_workgroups = G

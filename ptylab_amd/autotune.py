@@ -50,9 +50,11 @@ class Shape:
     Np: int
     No: int
     nF: int  # exit-wave fields (nlambda * nosm * npsm * nslice)
+    band: int = 0  # ASP: transfer-function columns inside the band limit (0: every column)
 
     def key(self, G):
-        return f"e{self.engine}p{self.propagator}/Np{self.Np}/No{self.No}/nF{self.nF}/G{G}"
+        b = f"/B{self.band}" if self.band else ""
+        return f"e{self.engine}p{self.propagator}/Np{self.Np}/No{self.No}/nF{self.nF}{b}/G{G}"
 
 
 @dataclass
@@ -191,7 +193,16 @@ def measure(shape: Shape, G: int, positions: int = 48, seed: int = 0) -> float:
     T["err_iter"] = torch.zeros((n_iter,), dtype=torch.float64, device=dev)
     if shape.propagator == _native.PTY_ASP or e3:
         ph = torch.rand(Np, Np, generator=gen) * 6.283
-        T["tf"] = torch.polar(torch.ones(Np, Np), ph).to(torch.complex64).to(dev)
+        mag = torch.ones(Np, Np)
+        if shape.band and not e3:
+            # the column passes skip the columns outside the band limit:
+            # keep `band` columns around zero frequency (FFT order)
+            keep = torch.zeros(Np, dtype=torch.bool)
+            keep[:(shape.band + 1) // 2] = True
+            if shape.band // 2:
+                keep[Np - shape.band // 2:] = True
+            mag[:, ~keep] = 0.0
+        T["tf"] = torch.polar(mag, ph).to(torch.complex64).to(dev)
     if momentum:
         for k in ("object", "probe"):
             T[k + "Buffer"] = T[k].clone()
@@ -229,10 +240,27 @@ def measure(shape: Shape, G: int, positions: int = 48, seed: int = 0) -> float:
         plan.close()
 
 
-def shape_of(inst) -> Shape:
+def shape_of(inst, band: int = 0) -> Shape:
     e3 = inst.engine == _native.PTY_E3PIE
     nF = inst.nlambda * inst.npsm if e3 else inst.nlambda * inst.nosm * inst.npsm * inst.nslice
-    return Shape(int(inst.engine), int(inst.propagator), int(inst.Np), int(inst.No), int(nF))
+    return Shape(int(inst.engine), int(inst.propagator), int(inst.Np), int(inst.No), int(nF), int(band))
+
+
+def band_of(inst, tensors) -> int:
+    """Columns of the instance's ASP / polychromeASP transfer function (the
+    widest over wavelengths) that hold a nonzero value, or 0 when every
+    column does or the propagator is another: the column passes skip the
+    others, so the per-position cost depends on it."""
+    if inst.propagator not in (_native.PTY_ASP, _native.PTY_POLYCHROME_ASP) or inst.engine == _native.PTY_E3PIE:
+        return 0
+    tf = tensors.get("tf") if tensors is not None else None
+    if tf is None:
+        return 0
+    Np = int(inst.Np)
+    ntf = int(inst.nlambda) if inst.propagator == _native.PTY_POLYCHROME_ASP else 1
+    h = tf.reshape(-1)[: ntf * Np * Np].reshape(ntf, Np, Np)
+    n = int((h != 0).any(dim=1).sum(dim=-1).max().item())
+    return n if n < Np else 0
 
 
 def launch_key(inst):
@@ -242,11 +270,15 @@ def launch_key(inst):
     return (fam, 0) if inst.Np <= BATCH_MAX_NP else (fam, int(inst.Np))
 
 
-def tune(instances, table: CostTable | None = None):
+def tune(instances, table: CostTable | None = None, tensors=None):
     """Cluster sizes for a list of pty_instance (one batched plan).  The
     launches of a plan run one after another, each alone on the device, so
-    every launch is assigned against its own co-resident capacity."""
+    every launch is assigned against its own co-resident capacity.
+    ``tensors``: the instances' device arrays (dicts as passed to the plan),
+    from which the ASP band limits are read; without them every column of a
+    transfer function counts as live."""
     table = table if table is not None else CostTable()
+    bands = [band_of(inst, tensors[i] if tensors is not None else None) for i, inst in enumerate(instances)]
     out = [1] * len(instances)
     launches = {}
     for i, inst in enumerate(instances):
@@ -254,7 +286,7 @@ def tune(instances, table: CostTable | None = None):
     for (fam, np_), idx in launches.items():
         a = instances[idx[0]]
         cap = _native.coresident_workgroups(np_ or BATCH_MAX_NP, a.engine, a.propagator, np_ == 0)
-        jobs = [Job(shape_of(instances[i]), int(instances[i].F), int(instances[i].n_iter)) for i in idx]
+        jobs = [Job(shape_of(instances[i], bands[i]), int(instances[i].F), int(instances[i].n_iter)) for i in idx]
         G = assign(jobs, table, cap, N_XCD if np_ == 0 else 1)
         for i, g in zip(idx, G):
             out[i] = g

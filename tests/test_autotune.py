@@ -104,3 +104,25 @@ def test_greedy_with_bins_matches_brute_force(bins, per_bin):
             m = A.predicted_makespan(jobs, _cost, c)
             best = m if best is None else min(best, m)
     assert A.predicted_makespan(jobs, _cost, G) == pytest.approx(best, rel=1e-12)
+
+
+def test_band_of_counts_live_transfer_function_columns():
+    """The cost key of an ASP instance carries the number of transfer
+    function columns inside the band limit (the column passes skip the rest);
+    the real band-limited ASP transfer function of the benchmark workload
+    (Np = 256, z = 50 mm, dxo = 4 um) keeps 33 columns."""
+    import numpy as np
+    import torch
+
+    from ptylab_amd import _native
+    from ptylab_amd import synthetic as S
+
+    inst = _native.PtyInstance()
+    inst.engine, inst.propagator, inst.Np, inst.nlambda = _native.PTY_MPIE, _native.PTY_ASP, 256, 1
+    H = S.asp_tf_fftorder(0.05, S.WAVELENGTH, 256, 256 * 4e-6)
+    assert A.band_of(inst, {"tf": torch.as_tensor(H.astype(np.complex64))}) == 33
+    assert A.band_of(inst, {"tf": torch.ones(256, 256, dtype=torch.complex64)}) == 0
+    assert A.band_of(inst, None) == 0
+    inst.propagator = _native.PTY_FRAUNHOFER
+    assert A.band_of(inst, {"tf": torch.as_tensor(H.astype(np.complex64))}) == 0
+    assert A.Shape(1, 2, 256, 1024, 4, 33).key(8) != A.Shape(1, 2, 256, 1024, 4).key(8)
