@@ -6,9 +6,9 @@
 // lanes apart in the wave (LS = 1: a line occupies adjacent lanes; LS = CW:
 // CW lines interleaved lane by lane, as the column passes use).
 // Butterflies whose span is a multiple of LPF are done inside a lane; smaller
-// spans exchange values between lanes with one ds_bpermute per float
-// (__shfl_xor with mask span * LS).  No LDS and no workgroup barrier is
-// needed inside a transform.
+// spans exchange values between lanes (mask span * LS) with DPP lane moves
+// and the permlane swaps (xlane).  No LDS and no workgroup barrier is needed
+// inside a transform.
 //
 // fft_fwd: radix-2 decimation in frequency, natural order in, bit-reversed
 //          order out, kernel exp(-2 pi i nk / N), unnormalised.
@@ -42,6 +42,40 @@ __device__ __forceinline__ float2 cmul_f(float2 a, float2 b) {
 __device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
   return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
+
+// The value of lane (lane ^ m) for m = 1, 2, ..., 32, without LDS: DPP lane
+// moves inside a row of 16 lanes, the gfx950 permlane swaps across rows.
+// m must be a constant once the caller's loops are unrolled (every branch
+// below but one folds away).  Exact data movement: results are the same as
+// with ds_bpermute.
+//   m = 1, 2: quad_perm [1,0,3,2] / [2,3,0,1];
+//   m = 4: half-row mirror (lane i of 8 reads 7 - i = i ^ 7), then
+//          quad_perm [3,2,1,0] (i ^ 3): i ^ 7 ^ 3 = i ^ 4;
+//   m = 8: row mirror (15 - i = i ^ 15), then half-row mirror (i ^ 7);
+//   m = 16: v_permlane16_swap of x with itself leaves row r's partner row
+//           (r ^ 1) in the second result for even rows, in the first for odd;
+//   m = 32: v_permlane32_swap likewise for the two halves of the wave.
+__device__ __forceinline__ float xlane(float v, int m) {
+  const int x = __float_as_int(v);
+  int r;
+  if (m == 1) {
+    r = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+  } else if (m == 2) {
+    r = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+  } else if (m == 4) {
+    r = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
+  } else if (m == 8) {
+    r = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false), 0x141, 0xF, 0xF, false);
+  } else if (m == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    r = (__lane_id() & 16) ? (int)p[0] : (int)p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    r = (__lane_id() & 32) ? (int)p[0] : (int)p[1];
+  }
+  return __int_as_float(r);
+}
+__device__ __forceinline__ float2 xlane2(float2 v, int m) { return make_float2(xlane(v.x, m), xlane(v.y, m)); }
 
 template <int N>
 struct Line {
@@ -84,7 +118,7 @@ __device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const float2 x = v[l];
-      const float2 p = shfl_xor2(x, s * LS);
+      const float2 p = xlane2(x, s * LS);
       const float2 y = make_float2(__builtin_fmaf(sgn, x.x, p.x), __builtin_fmaf(sgn, x.y, p.y));
       v[l] = cmul_f(y, w);
     }
@@ -111,7 +145,7 @@ __device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const float2 x = cmul_f(v[l], w);
-      const float2 p = shfl_xor2(x, s * LS);
+      const float2 p = xlane2(x, s * LS);
       v[l] = make_float2(__builtin_fmaf(sgn, x.x, p.x), __builtin_fmaf(sgn, x.y, p.y));
     }
   }
