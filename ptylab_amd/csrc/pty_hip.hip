@@ -230,9 +230,10 @@ __device__ __forceinline__ void clear_slot(const Run& R, unsigned* slot) {
 // Cross-workgroup data of the persistent kernel (exit waves, object, probe,
 // momenta, maxima, error partials) is only ever read and written with
 // coherent accesses: agent-scope relaxed atomics (global_load/store ... sc1)
-// in general, or, for a cluster on one XCD, plain stores and L1-missing sc0
-// loads served by that XCD's L2; no cache the readers use holds a stale
-// copy.  A barrier then needs no cache maintenance: every wave drains its
+// in general, or, for a cluster on one XCD, plain stores and sc0 loads that
+// the XCD's L2 serves once the barrier has invalidated the CU's L1 (below);
+// no cache the readers use holds a stale copy.  A barrier then needs no L2
+// maintenance: every wave drains its
 // outstanding stores (s_waitcnt vmcnt(0): the stores have been performed at
 // the coherence point), the workgroup barrier orders that before the
 // workgroup's arrival, and once every workgroup has arrived the cluster
@@ -244,8 +245,12 @@ __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0x0F
 
 // XCD-local accessors.  When every workgroup of a cluster runs on one XCD,
 // cross-workgroup data need only be coherent in that XCD's L2: stores are
-// plain (the CU's L1 is write-through) and loads are buffer loads with the
-// sc0 bit, which miss in L1 and are served by L2.  The agent-scope forms
+// plain (the CU's L1 is write-through, so a drained store is in L2) and
+// loads are sc0 buffer loads.  An sc0 (workgroup-scope) load may still hit
+// the CU's L1, so the flag barrier invalidates the L1 (buffer_inv sc1, the
+// compiler's own acquire step for gfx942/gfx950) before any wave reads
+// again; without it Np = 32 clusters of 4 and 8 workgroups read stale
+// exit-wave lines (buffer_inv sc0 did not cure it).  The agent-scope forms
 // above instead miss in L2 as well.  The buffer resource is built from the
 // first active lane's address minus 2^31, so the 32-bit offsets of one
 // wave's accesses reach 2 GiB either side of it (a wave's accesses of a pass
@@ -312,9 +317,9 @@ __device__ __forceinline__ void counter_sync(Run& R) {
 // Flag barrier: workgroup g publishes the generation it has reached in its
 // own word D.bar[g] (a store, no atomic); wave 0 of every workgroup polls
 // the G words until all of them reach the generation.  Used by clusters on
-// one XCD: stores are plain and the polls sc0 buffer loads served by the
-// XCD's L2 (the volatile bit of the load keeps the compiler from hoisting it
-// out of the loop).  Measured against the counter barrier above it is the
+// one XCD: stores are plain and the polls volatile buffer loads, which the
+// compiler issues at system scope (sc0 sc1), so they neither hit a stale L1
+// line nor leave the loop.  Measured against the counter barrier above it is the
 // faster one within an XCD and the slower one across XCDs, where every poll
 // of G words goes past the L2s.
 __device__ __forceinline__ void flag_sync(Run& R) {
@@ -374,10 +379,15 @@ __device__ __forceinline__ bool cluster_sync(Run& R) {
   __syncthreads();
   if (R.G == 1) return true;
   R.gen += 1u;
-  if (R.loc)
+  if (R.loc) {
     flag_sync(R);
-  else
+    // lines this CU cached before the barrier may be stale: invalidate the
+    // L1 (it is shared by the CU's waves) before the workgroup barrier
+    // releases the other waves
+    if (R.wave == 0) __asm__ volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
+  } else {
     counter_sync(R);
+  }
   __syncthreads();
   return *R.fail == 0;
 }

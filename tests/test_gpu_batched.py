@@ -101,6 +101,15 @@ def test_batched_autotuned_equals_separate_runs(datasets, separate, tmp_path):
     _same(separate, got, "autotuned")
 
 
+def test_small_probe_clusters_equal_separate_runs(datasets, separate):
+    """Np = 32 on clusters of 4 and 8 workgroups, whose whole exit-wave
+    workspace fits a CU's L1: the XCD-local exchange must not read a stale
+    L1 line (without the L1 invalidate at its barrier it did, at 4.5e-2)."""
+    for G in ([1, 8, 2, 4, 2, 1], [1, 4, 4, 8, 4, 2]):
+        got, _ = _batched(datasets, workgroups=G)
+        _same(separate, got, f"clusters {G}")
+
+
 def test_batched_member_against_oracle(datasets):
     got, _ = _batched(datasets, workgroups=[4, 1, 1, 1, 1, 1])
     engine, prop, modes, Np, No, F, iters, order, seed = CASES[0]
