@@ -132,6 +132,8 @@ struct DevInst {
   double* pcpart;    // position correction: per-workgroup FP64 partials, (G, PC_TERMS)
   unsigned osel[32]; // fields whose object terms the owner of field f's object sums (bit g)
   unsigned psel[32]; // fields whose probe terms the owner of field f's probe sums (bit g)
+  int fl;            // the row passes may loop over the fields of a line in one wave: one
+                     // object field summing every field, probe field f taking field f alone
   int gsize[8];      // counter barrier: workgroups of this instance in each block group b % 8
   int ngroups;       // counter barrier: non-empty groups
   double* errpart;   // (2, F, G) per-position per-workgroup error partials
@@ -175,6 +177,7 @@ struct Run {
   float2* xb;
   int* fail;
   bool loc;  // every workgroup of the cluster runs on one XCD: exchange through its L2
+  bool fl;   // field-loop row passes (DevInst::fl, and at least one line per item slot)
 };
 
 // ---------------------------------------------------------------------------
@@ -975,6 +978,146 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   ITEMS_END
 }
 
+// Field-loop forms of the first and last row passes, for workgroups that own
+// at least one line per item slot (clusters of a few workgroups): one slot
+// takes a line and loops over its fields, so the field sums stay in
+// registers, in increasing field order as in the LDS forms (bit-identical
+// results), and a pass has no workgroup barrier.  Only for one object field
+// whose owner sums every field and probe field f taking field f alone
+// (ePIE / mPIE / multiPIE with nlambda = nosm = nslice = 1, DevInst::fl).
+template <int NP, bool ASPF, bool QPH>
+__device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot, unsigned* zslot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int No = D.a.No, nF = D.nF;
+  const size_t Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  const float2* __restrict__ O = (const float2*)D.a.object;
+  const float2* __restrict__ P = (const float2*)D.a.probe;
+  float2* __restrict__ W = D.W;
+  const bool cb = prop_checkerboard(D.a.propagator);
+  const float2* __restrict__ Qp = D.Q;
+  clear_slot(R, zslot);
+  float pmax = 0.f, omax = 0.f;
+  const int per = NP / R.G, line0 = R.g * per;
+  const int islot = R.wave * Line<NP>::SUB + R.sub;
+  for (int base = 0; base < per; base += R.ipr) {
+    const int li = base + islot;
+    const bool act = li < per;
+    const int r = line0 + (act ? li : per - 1);
+    float2 of[L];
+    load_row<NP>(R, of, O + (size_t)(row0 + r) * No + col0, R.t);
+    float sp[L], so[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) sp[l] = so[l] = 0.f;
+    for (int f = 0; f < nF; ++f) {
+      float2 pf[L], v[L];
+      load_row<NP>(R, pf, P + f * Np2 + (size_t)r * NP, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        sp[l] += act ? cabs2(pf[l]) : 0.f;
+        so[l] += act && f == 0 ? cabs2(of[l]) : 0.f;
+        float2 e = cmul(of[l], pf[l]);
+        if constexpr (!ASPF || QPH) {
+          if (Qp) e = cmul(e, ldg2(Qp + (size_t)(f / D.fpl) * Np2 + (size_t)r * NP + l * LPF + R.t));
+        }
+        if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
+        v[l] = e;
+      }
+      fft_fwd<NP>(v, R.tw, R.t);
+      if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+    }
+    if (act) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        pmax = fmaxf(pmax, sp[l]);
+        omax = fmaxf(omax, so[l]);
+      }
+    }
+  }
+  pmax = wave_max(pmax);
+  omax = wave_max(omax);
+  if (R.lane == 0) {
+    atomic_fmax_slot(R, slot, 0, pmax);
+    atomic_fmax_slot(R, slot, 1, omax);
+  }
+}
+
+template <int NP, bool ASPF, bool QPH>
+__device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned* slot) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
+  const DevInst& D = *R.D;
+  const int No = D.a.No, nF = D.nF;
+  const size_t Np2 = (size_t)NP * NP;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  float2* __restrict__ O = (float2*)D.a.object;
+  float2* __restrict__ P = (float2*)D.a.probe;
+  float2* __restrict__ W = D.W;
+  const bool cb = prop_checkerboard(D.a.propagator);
+  const float2* __restrict__ Qp = D.Q;
+  const float invN = 1.0f / (float)NP;
+  const int engine = D.a.engine;
+  const float Pmax = load_slot(R, slot, 0), Omax = load_slot(R, slot, 1);
+  const float bO = D.a.betaObject;
+  const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
+  const float bP = engine == PTY_EPIE ? D.a.betaProbe : stepP;
+  const float invP = 1.0f / Pmax, invO = 1.0f / Omax;
+  const float aPm = D.a.alphaObject * Pmax, aOm = D.a.alphaProbe * Omax;
+  const float omO = D.omAlphaO, omP = D.omAlphaP;
+  const int per = NP / R.G, line0 = R.g * per;
+  const int islot = R.wave * Line<NP>::SUB + R.sub;
+  for (int base = 0; base < per; base += R.ipr) {
+    const int li = base + islot;
+    const bool act = li < per;
+    const int r = line0 + (act ? li : per - 1);
+    const size_t orow_off = (size_t)(row0 + r) * No + col0;
+    const size_t prow_off = (size_t)r * NP;
+    float2 of[L], accO[L];
+    load_row<NP>(R, of, O + orow_off, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l) accO[l] = make_float2(0.f, 0.f);
+    for (int f = 0; f < nF; ++f) {
+      float2 d[L], pf[L], nv[L];
+      unsigned lb = ~0u;
+      if constexpr (ASPF) lb = live_bits<NP>(D, f, R.t);
+      load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+      load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
+      if constexpr (ASPF) zero_dead_columns<NP>(lb, d);
+      fft_inv<NP>(d, R.tw, R.t);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        float2 u = d[l];
+        if constexpr (!ASPF) {
+          u = cscale(u, invN);
+          if (cb && ((r + l * LPF + R.t) & 1)) u = make_float2(-u.x, -u.y);
+          if (Qp) u = cmul(u, conjf2(ldg2(Qp + prow_off + l * LPF + R.t)));
+        } else if constexpr (QPH) {
+          u = cmul(u, conjf2(ldg2(Qp + (size_t)(f / D.fpl) * Np2 + prow_off + l * LPF + R.t)));
+        }
+        const float2 dl = csub(u, cmul(of[l], pf[l]));
+        float wo, wp;
+        if (engine == PTY_EPIE) {
+          wo = invP;
+          wp = invO;
+        } else {
+          wo = 1.0f / (aPm + omO * cabs2(pf[l]));
+          wp = 1.0f / (aOm + omP * cabs2(of[l]));
+        }
+        accO[l] = cadd(accO[l], cmul(make_float2(pf[l].x * wo, -pf[l].y * wo), dl));
+        const float2 accP = cadd(make_float2(0.f, 0.f), cmul(make_float2(of[l].x * wp, -of[l].y * wp), dl));
+        nv[l] = cadd(pf[l], cscale(accP, bP));
+      }
+      if (act) store_row<NP>(R, P + f * Np2 + prow_off, nv, R.t);
+    }
+    if (act) {
+      float2 nv[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) nv[l] = cadd(of[l], cscale(accO[l], bO));
+      store_row<NP>(R, O + orow_off, nv, R.t);
+    }
+  }
+}
+
 // qNewton, after the object step: Omax = max over the whole object of the
 // sum over its fields of |O| (qNewton.py:134), each workgroup over its own
 // chunk of pixels, fields added in C order.
@@ -1429,11 +1572,18 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 // per-position drivers
 // ---------------------------------------------------------------------------
-template <int NP, bool ASP, int CWR, bool QN, bool QPH, bool PC>
+template <int NP, bool ASP, int CWR, bool QN, bool QPH, bool PC, bool BAT>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
                                              bool sync_after, int it) {
   const DevInst& D = *R.D;
-  pie_rows_first<NP, QN, ASP, QPH>(R, k, slot, zslot);
+  // only the angular-spectrum families take the field-loop forms: with them
+  // the Fraunhofer family spilled more registers and lost 6 % on one-mode
+  // instances, which have no field sums to save
+  constexpr bool FLK = BAT && ASP && !QN && !PC;
+  if (FLK && R.fl)
+    pie_rows_first_fl<NP, ASP, QPH>(R, k, slot, zslot);
+  else
+    pie_rows_first<NP, QN, ASP, QPH>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
     tf_cols<NP, CWR>(R, false);
@@ -1446,7 +1596,10 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs);
     if (!cluster_sync(R)) return false;
   }
-  pie_rows_update<NP, QN, ASP, QPH>(R, k, slot);
+  if (FLK && R.fl)
+    pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
+  else
+    pie_rows_update<NP, QN, ASP, QPH>(R, k, slot);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
@@ -1498,7 +1651,7 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
 // Iterations [it0, it1); the first iteration starts at position jb and the
 // last one stops before position je (full iterations: jb = 0, je >= F).  The
 // error of an iteration is reduced when its last position has run.
-template <int NP, int FAM, int CWR>
+template <int NP, int FAM, int CWR, bool BAT = false>
 __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb, int je) {
   constexpr int BF = base_family(FAM);  // pass family without the position correction
   constexpr bool PC = BF != FAM;
@@ -1555,8 +1708,8 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
         ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
       else
         ok = pie_position<NP, BF == FAM_PIE_ASP || BF == FAM_QN_ASP || BF == FAM_PIE_SASP, CWR,
-                          BF == FAM_QN_FRAUNHOFER || BF == FAM_QN_ASP, BF == FAM_PIE_SASP, PC>(R, k, j, slot, zslot, ep,
-                                                                                              sync_after, it);
+                          BF == FAM_QN_FRAUNHOFER || BF == FAM_QN_ASP, BF == FAM_PIE_SASP, PC, BAT>(
+            R, k, j, slot, zslot, ep, sync_after, it);
       if (!ok) return;
       if (mom) {
         momentum_pass<NP>(R);
@@ -1606,6 +1759,7 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   R.xb = s_x + R.ipr * NP;
   R.fail = s_fail;
   R.loc = false;
+  R.fl = false;
   for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
 }
@@ -1690,27 +1844,32 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
     case 16:
       init_run<16, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      run_iterations<16, FAM, BATCH_CW>(R, it0, it1, jb, je);
+      R.fl = D->fl && 16 / R.G >= R.ipr;
+      run_iterations<16, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 32:
       init_run<32, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      run_iterations<32, FAM, BATCH_CW>(R, it0, it1, jb, je);
+      R.fl = D->fl && 32 / R.G >= R.ipr;
+      run_iterations<32, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 64:
       init_run<64, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      run_iterations<64, FAM, BATCH_CW>(R, it0, it1, jb, je);
+      R.fl = D->fl && 64 / R.G >= R.ipr;
+      run_iterations<64, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 128:
       init_run<128, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      run_iterations<128, FAM, BATCH_CW>(R, it0, it1, jb, je);
+      R.fl = D->fl && 128 / R.G >= R.ipr;
+      run_iterations<128, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 256:
       init_run<256, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
       R.loc = true;
-      run_iterations<256, FAM, BATCH_CW>(R, it0, it1, jb, je);
+      R.fl = D->fl && 256 / R.G >= R.ipr;
+      run_iterations<256, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
   }
 }
@@ -2222,6 +2381,12 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     }
     d.nO = a.nlambda * a.nosm * a.nslice;
     d.nPr = a.nlambda * a.npsm * a.nslice;
+    d.fl = 0;
+    if ((a.engine == PTY_EPIE || a.engine == PTY_MPIE || a.engine == PTY_MULTIPIE) && d.nO == 1 && d.nF == d.nPr) {
+      bool ok = d.osel[0] == (d.nF >= 32 ? ~0u : (1u << d.nF) - 1u);
+      for (int f = 0; f < d.nF; ++f) ok = ok && d.psel[f] == (1u << f);
+      d.fl = ok ? 1 : 0;
+    }
     d.omAlphaO = (float)(1.0 - (double)a.alphaObject);
     d.omAlphaP = (float)(1.0 - (double)a.alphaProbe);
     // positions must keep every patch inside the object
