@@ -536,6 +536,11 @@ struct Cfg {
   }               \
   }
 #define ITEM_LEAD(act, frw) ((act) && (frw) == 0)  // the slot that speaks for its line
+// End of a round whose LDS slots the next round rewrites.  After the last
+// round the barrier that follows the pass (cluster barrier or block_sum)
+// orders the reuse, so the workgroup barrier is skipped there.
+#define ITEM_ROUND_SYNC() \
+  if (base_ + lpr_ < per_) __syncthreads()
 
 // ---------------------------------------------------------------------------
 // field bookkeeping
@@ -634,7 +639,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
         omax = fmaxf(omax, so[l]);
       }
     }
-    __syncthreads();
+    ITEM_ROUND_SYNC();
   ITEMS_END
   pmax = wave_max(pmax);
   omax = wave_max(omax);
@@ -671,6 +676,8 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
 #define CITEMS_END \
   }                \
   }
+#define CITEM_ROUND_SYNC() \
+  if (base_ + gpr_ < cgn_) __syncthreads()
 
 template <int NP, int CWR>
 __device__ __forceinline__ int col_rho(const Run& R) { return R.lane / Col<NP, CWR>::CW; }
@@ -745,7 +752,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
     }
     fft_inv_colg<NP, CWR>(v, R.tw, rho);
     if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
-    __syncthreads();
+    CITEM_ROUND_SYNC();
   CITEMS_END
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
@@ -844,7 +851,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     }
     fft_fwd<NP>(v, R.tw, R.t);
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
-    __syncthreads();
+    ITEM_ROUND_SYNC();
   ITEMS_END
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
@@ -858,7 +865,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
 // psm index 0, resp. osm index 0).  Each object pixel of the patch has one
 // owner.
 template <int NP, bool QN, bool ASPF, bool QPH>
-__device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot) {
+__device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot, bool sync_tail) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
@@ -974,7 +981,9 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       for (int l = 0; l < L; ++l) nv[l] = cadd(pf[l], cscale(acc[l], engine == PTY_EPIE ? D.a.betaProbe : stepP));
       store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
     }
-    __syncthreads();
+    // the next position's first pass rewrites the LDS slots at once when the
+    // cluster barrier after this pass is skipped
+    if (sync_tail) __syncthreads(); else ITEM_ROUND_SYNC();
   ITEMS_END
 }
 
@@ -1599,7 +1608,7 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (FLK && R.fl)
     pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
   else
-    pie_rows_update<NP, QN, ASP, QPH>(R, k, slot);
+    pie_rows_update<NP, QN, ASP, QPH>(R, k, slot, !sync_after);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
