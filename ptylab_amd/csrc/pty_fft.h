@@ -39,10 +39,6 @@ __device__ __forceinline__ float2 cmul_f(float2 a, float2 b) {
   return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 
-__device__ __forceinline__ float2 shfl_xor2(float2 v, int m) {
-  return make_float2(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
-}
-
 // The value of lane (lane ^ m) for m = 1, 2, ..., 32, without LDS: DPP lane
 // moves inside a row of 16 lanes, the gfx950 permlane swaps across rows.
 // m must be a constant once the caller's loops are unrolled (every branch
