@@ -51,8 +51,7 @@ __device__ __forceinline__ float2 cmul_f(float2 a, float2 b) {
 //   m = 16: v_permlane16_swap of x with itself leaves row r's partner row
 //           (r ^ 1) in the second result for even rows, in the first for odd;
 //   m = 32: v_permlane32_swap likewise for the two halves of the wave.
-__device__ __forceinline__ float xlane(float v, int m) {
-  const int x = __float_as_int(v);
+__device__ __forceinline__ int xlane_i(int x, int m) {
   int r;
   if (m == 1) {
     r = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
@@ -69,9 +68,15 @@ __device__ __forceinline__ float xlane(float v, int m) {
     const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
     r = (__lane_id() & 32) ? (int)p[0] : (int)p[1];
   }
-  return __int_as_float(r);
+  return r;
 }
+__device__ __forceinline__ float xlane(float v, int m) { return __int_as_float(xlane_i(__float_as_int(v), m)); }
 __device__ __forceinline__ float2 xlane2(float2 v, int m) { return make_float2(xlane(v.x, m), xlane(v.y, m)); }
+__device__ __forceinline__ double xlane_d(double v, int m) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)xlane_i((int)(unsigned)u, m), hi = (unsigned)xlane_i((int)(unsigned)(u >> 32), m);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 template <int N>
 struct Line {

@@ -185,12 +185,12 @@ struct Run {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, xlane(v, m));
   return v;
 }
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  for (int m = 32; m >= 1; m >>= 1) v += xlane_d(v, m);
   return v;
 }
 
@@ -274,7 +274,7 @@ __device__ __forceinline__ LocalRef local_ref(const void* p) {
 __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
-    const unsigned o = (unsigned)__shfl_xor((int)v, m);
+    const unsigned o = (unsigned)xlane_i((int)v, m);
     v = o < v ? o : v;
   }
   return v;
