@@ -248,3 +248,19 @@ def test_scaled_asp_phases_and_checkerboard_folding():
     want_b = O.ifft2c(O.fft2c(psi) * q2.conj()) * q1.conj()
     got_b = np.fft.ifft2(np.fft.fft2(c * psi) * pair[0].conj()) * pair[1].astype(np.complex128).conj()
     assert np.max(np.abs(got_b - want_b)) < 1e-12 * np.max(np.abs(want_b))
+
+
+def test_batched_instances_follow_the_named_inputs():
+    """The batched benchmark's instances: generated as the single mPIE ASP
+    workload (4 probe modes of radius Np/4) by default, as the plain-ePIE
+    family on request; Np, F and the seed per instance do not depend on the
+    inputs chosen."""
+    from ptylab_amd import synthetic as S
+
+    a = S.batched_instance(3, nps=(32,), fmin=5, fmax=8)
+    b = S.batched_instance(3, nps=(32,), fmin=5, fmax=8, inputs="fraunhofer")
+    assert a.Np == b.Np == 32 and a.No == b.No == 128
+    assert a.ptychogram.shape == b.ptychogram.shape
+    assert a.probe_true.shape[2] == 4 and b.probe_true.shape[2] == 1
+    with pytest.raises(ValueError):
+        S.batched_instance(3, nps=(32,), fmin=5, fmax=8, inputs="other")
