@@ -131,14 +131,48 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvFwd a) {
   const int t = threadIdx.x, wave = t >> 6, l = t & 63;
   const int M = a.B * a.H * a.W;
   const int m0 = blockIdx.x * CV_BM, n0 = blockIdx.y * CV_BN;
+  // This thread fills A-tile rows t / 4 and t / 4 + 64 (two pixels) at k
+  // chunk t % 4 of every 32-wide step: the pixels' coordinates are computed
+  // once and the chunk's (tap, channel) advances by 32 per step, so the
+  // loop does no integer division (cin_s is a multiple of 8: a chunk never
+  // straddles two taps).  Same operands as im2col_chunk.
+  const int kc = t & 3, row0 = t >> 2;
+  const int hw = a.H * a.W, half = a.ks / 2;
+  const bf16_t* img[2];
+  int py[2], pxx[2];
+  bool pin[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int m = m0 + row0 + 64 * h;
+    pin[h] = m < M;
+    const int mm = pin[h] ? m : 0;
+    const int n = mm / hw, r = mm - n * hw;
+    py[h] = r / a.W;
+    pxx[h] = r - py[h] * a.W;
+    img[h] = a.x + (size_t)n * hw * a.cin_s;
+  }
+  int tap = (8 * kc) / a.cin_s;
+  int ci = 8 * kc - tap * a.cin_s;
+  int ky = tap / a.ks, kx = tap - ky * a.ks;
   f32x16 acc0 = {}, acc1 = {};
   for (int k0 = 0; k0 < a.Kp; k0 += CV_KT) {
     // A tile: 128 pixels x 32 k, two 8-element chunks per thread
+    const bool kin = k0 + 8 * kc < a.K;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int c = t + 256 * h, row = c >> 2, kc = c & 3;
-      const uint4 v = im2col_chunk(a.x, m0 + row, k0 + 8 * kc, M, a.H, a.W, a.cin_s, a.ks, a.K, false);
-      *(uint4*)(As + row * CV_LD + 8 * kc) = v;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      const int yy = py[h] + ky - half, xs = pxx[h] + kx - half;
+      if (pin[h] && kin && yy >= 0 && yy < a.H && xs >= 0 && xs < a.W)
+        v = *(const uint4*)(img[h] + ((size_t)yy * a.W + xs) * a.cin_s + ci);
+      *(uint4*)(As + (row0 + 64 * h) * CV_LD + 8 * kc) = v;
+    }
+    ci += CV_KT;
+    while (ci >= a.cin_s) {
+      ci -= a.cin_s;
+      if (++kx == a.ks) {
+        kx = 0;
+        ++ky;
+      }
     }
     {
       const int row = t >> 2, kc = t & 3, co = n0 + row;

@@ -26,6 +26,7 @@ scanned region (union of the probe footprints above 1 % of peak intensity).
 from __future__ import annotations
 
 import math
+import sys
 import time
 
 import numpy as np
@@ -182,7 +183,7 @@ def train(net, steps, seed=0, lr=1e-3, lr_final=1e-4, log_every=0, sim=None):
         lr_s = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * s / max(1, steps)))
         losses.append(net.train_step(x, t, lr=lr_s))
         if log_every and (s + 1) % log_every == 0:
-            print(f"[train] step {s + 1}/{steps} loss {float(losses[-1]):.4e}", flush=True)
+            print(f"[train] step {s + 1}/{steps} loss {float(losses[-1]):.4e}", file=sys.stderr, flush=True)
     return torch.stack(losses).cpu().numpy()
 
 
