@@ -51,6 +51,9 @@ def parse_args():
                     help="c1: single mPIE ASP reconstruction (default); batched: 64 independent reconstructions; "
                          "learned_init: time to object NRMSE, learned-init vs plain mPIE")
     ap.add_argument("--train-patterns", type=int, default=200000)
+    ap.add_argument("--li-stride", type=int, default=4,
+                    help="learned_init: predict and stitch the patches of every n-th position (the fallback check "
+                         "uses all of them)")
     ap.add_argument("--engine", choices=("mPIE", "ePIE"), default="mPIE", help="engine of the batched workload")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true", help="batched: direct launches instead of graph replays")
@@ -417,7 +420,7 @@ def main_learned_init(args):
         rec.initializeObjectProbe()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info = LI.initialize_object(rec, data, net, probe) if learned else None
+        info = LI.initialize_object(rec, data, net, probe, stride=args.li_stride) if learned else None
         eng = P.Engines.mPIE(rec, data, params, P.DummyMonitor())
         eng.numIterations = max_iter
         np.random.seed(2)

@@ -191,10 +191,11 @@ def train(net, steps, seed=0, lr=1e-3, lr_final=1e-4, log_every=0, sim=None):
 # inference, stitching, NRMSE
 # ---------------------------------------------------------------------------
 def predict(net, ptychogram, probe_np):
-    """Patch estimates (F, 64, 64) complex64 on the GPU, batches of net.B."""
+    """Patch estimates (F, 64, 64) complex64 on the GPU, batches of net.B.
+    ``ptychogram``: (F, 128, 128) intensities, NumPy or a device tensor."""
     import torch
 
-    I = torch.as_tensor(np.asarray(ptychogram, np.float32), device="cuda")
+    I = torch.as_tensor(ptychogram, dtype=torch.float32, device="cuda")
     x = Features(probe_np)(I)
     F = x.shape[0]
     out = torch.empty((F, PATCH, PATCH), dtype=torch.complex64, device="cuda")
@@ -311,44 +312,57 @@ def run_to_target(engine, true_obj, roi, target=1e-2, max_iter=300):
     return (done if hit is not None else None), (hit if hit is not None else time.perf_counter() - t0), curve
 
 
-def data_misfit(obj, ptychogram, positions, probe_np):
+def data_misfit(obj, ptychogram, positions, probe_np, uniform=False):
     """sum_j ||sqrt(I_j) - |fft2c(P O_j)|||^2 / sum_j I_j for an object guess
     (GPU, the hot path's Fraunhofer propagator): how well a candidate
-    initial object explains the measured amplitudes with the known probe."""
+    initial object explains the measured amplitudes with the known probe.
+    ``uniform``: obj is all ones, so every exit wave is P and one transform,
+    expanded over the positions, gives the same values.  ``ptychogram`` may
+    be NumPy or a device tensor."""
     import torch
 
-    I = torch.as_tensor(np.asarray(ptychogram, np.float32), device=obj.device)
+    I = torch.as_tensor(ptychogram, dtype=torch.float32, device=obj.device)
     P = torch.as_tensor(np.asarray(probe_np, np.complex64), device=obj.device)
-    pos = torch.as_tensor(np.asarray(positions, np.int64), device=obj.device)
-    ar = torch.arange(NP, device=obj.device)
-    rows = (pos[:, 0, None] + ar[None])[:, :, None]
-    cols = (pos[:, 1, None] + ar[None])[:, None, :]
-    ex = (obj[rows, cols] * P[None]).contiguous()
-    _native.propagate(ex, _native.PTY_FRAUNHOFER, inverse=False)
+    F = I.shape[0]
+    if uniform:
+        far = P.clone().contiguous()
+        _native.propagate(far, _native.PTY_FRAUNHOFER, inverse=False)
+        ex = far[None].expand(F, NP, NP)
+    else:
+        pos = torch.as_tensor(np.asarray(positions, np.int64), device=obj.device)
+        ar = torch.arange(NP, device=obj.device)
+        rows = (pos[:, 0, None] + ar[None])[:, :, None]
+        cols = (pos[:, 1, None] + ar[None])[:, None, :]
+        ex = (obj[rows, cols] * P[None]).contiguous()
+        _native.propagate(ex, _native.PTY_FRAUNHOFER, inverse=False)
     s = torch.sqrt(I.sum() / (ex.abs() ** 2).sum())  # the data's global scale
     return float(((torch.sqrt(I.clamp_min(0)) - s * ex.abs()) ** 2).sum() / I.sum())
 
 
-def initialize_object(reconstruction, experimentalData, net, probe_np):
+def initialize_object(reconstruction, experimentalData, net, probe_np, stride=1):
     """Replace reconstruction.object by the learned initial object when it
     explains the data better than the uniform guess does (data_misfit with
-    the known probe); otherwise keep the uniform 'ones' object.  Returns a
-    dict: choice ('learned' or 'ones'), both misfits and the seconds spent.
-    The object keeps the reconstruction's 6D shape and complex64 dtype."""
+    the known probe, over every position); otherwise keep the uniform 'ones'
+    object.  ``stride``: predict and stitch the patches of every stride-th
+    position only (the scan overlaps many times over).  Returns a dict:
+    choice ('learned' or 'ones'), both misfits and the seconds spent.  The
+    object keeps the reconstruction's 6D shape and complex64 dtype."""
     import torch
 
     t0 = time.perf_counter()
     pos = np.asarray(reconstruction.positions, np.int64)
     No = int(reconstruction.No)
-    I = experimentalData.ptychogram
-    patches = predict(net, I, probe_np)
-    obj = stitch(patches, pos, probe_np, No)
+    I = torch.as_tensor(np.asarray(experimentalData.ptychogram, np.float32), device="cuda")
+    sel = np.arange(0, pos.shape[0], max(1, int(stride)))
+    patches = predict(net, I[torch.as_tensor(sel, device="cuda")] if stride > 1 else I, probe_np)
+    obj = stitch(patches, pos[sel], probe_np, No)
     ones = torch.ones_like(obj)
     e_l = data_misfit(obj, I, pos, probe_np)
-    e_o = data_misfit(ones, I, pos, probe_np)
+    e_o = data_misfit(ones, I, pos, probe_np, uniform=True)
     choice = "learned" if e_l < e_o else "ones"
     if choice == "learned":
         o = obj.cpu().numpy().astype(np.complex64)
         reconstruction.object = np.broadcast_to(o, reconstruction.shape_O).copy()
     torch.cuda.synchronize()
-    return dict(choice=choice, misfit_learned=e_l, misfit_ones=e_o, seconds=time.perf_counter() - t0)
+    return dict(choice=choice, misfit_learned=e_l, misfit_ones=e_o, stride=int(stride),
+                seconds=time.perf_counter() - t0)

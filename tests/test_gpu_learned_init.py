@@ -79,3 +79,18 @@ def test_mpie_from_the_learned_init_is_ahead(setup):
     assert curves[True][1] < curves[False][1]
     # both settle on the same noise floor
     assert abs(curves[True][-1] - curves[False][-1]) < 0.01
+
+
+def test_uniform_misfit_shortcut_equals_the_general_form():
+    """The fallback check's misfit of the all-ones object: one transform of
+    the probe expanded over the positions gives the same value as the
+    per-position exit waves."""
+    from ptylab_amd import learned_init as LI
+
+    w = LI.held_out_scan(F=40)
+    probe = LI.probe()
+    ones = torch.ones((w.No, w.No), dtype=torch.complex64, device="cuda")
+    pos = np.asarray(w.positions, np.int64)
+    a = LI.data_misfit(ones, w.ptychogram, pos, probe)
+    b = LI.data_misfit(ones, w.ptychogram, pos, probe, uniform=True)
+    assert a == pytest.approx(b, rel=1e-6)
