@@ -28,7 +28,7 @@ class ExperimentalData:
     def __init__(self, filename=None, operationMode="CPM"):
         self.logger = logging.getLogger("ExperimentalData")
         if operationMode != "CPM":
-            raise NotImplementedError("only operationMode='CPM' is implemented (FPM is outside this build's scope)")
+            raise NotImplementedError("only operationMode='CPM' is implemented (FPM is not)")
         self.operationMode = operationMode
         self.ptychogram = None
         self.encoder = None

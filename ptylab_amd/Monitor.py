@@ -1,5 +1,5 @@
 """Monitor interface of the engines (Monitor/Monitor.py:264-300): the
-engines only call these no-op hooks; live plotting is outside this build."""
+engines only call these no-op hooks; live plotting is not implemented."""
 
 
 class DummyMonitor:

@@ -3,7 +3,7 @@ of the object patch under the probe; the patches, stitched by probe-weighted
 averaging, replace the reference's 'ones' initial object
 (initialProbeOrObject, utils/initializationFunctions.py:10-62, used by
 Reconstruction.initializeObject, Reconstruction.py:356-374) and mPIE refines
-from there.  Plain uniform-init mPIE stays the fallback and the judge.
+from there.  Plain uniform-init mPIE stays the fallback and the yardstick.
 
 Workload (the learned-initialiser configuration): probe Np = 128 (the
 plain-ePIE family's aperture probe, radius 32 px, quadratic phase,

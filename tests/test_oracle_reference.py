@@ -1,8 +1,8 @@
 """Pin the oracle against the reference's own regression goldens.
 
 The goldens are read from the reference tree at test time (they are not
-copied into this repository); the tests skip where the tree is absent (the
-GPU box).  Tolerance is the reference's own (rtol 1e-5, atol 1e-7,
+copied into this repository); the tests skip where the tree is absent
+(the machines that run the GPU tests).  Tolerance is the reference's own (rtol 1e-5, atol 1e-7,
 test_engine_regression.py:153); the restatement in fact matches bit for bit.
 """
 
