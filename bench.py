@@ -34,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "position updates/s; time to object NRMSE 1e-2, learned-init vs plain mPIE"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = ROOT / "profiles" / "pty_iterate_kernel_mpie_asp_np256_traffic.txt"
+BATCH_TRAFFIC_FILE = ROOT / "profiles" / "pty_batch_kernel_mpie_asp_64inst_traffic.txt"
 
 
 def parse_args():
@@ -56,6 +57,9 @@ def parse_args():
                          "uses all of them)")
     ap.add_argument("--engine", choices=("mPIE", "ePIE"), default="mPIE", help="engine of the batched workload")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--gen-workers", type=int, default=16,
+                    help="batched: processes generating the inputs (1: in this process, no fork; for runs under a "
+                         "profiler, which initialises the GPU before the program starts)")
     ap.add_argument("--no-graph", action="store_true", help="batched: direct launches instead of graph replays")
     ap.add_argument("--batched-inputs", choices=("asp_modes", "fraunhofer"), default="asp_modes",
                     help="batched: instances generated as the single mPIE ASP workload (ASP, 4 probe modes; "
@@ -74,10 +78,10 @@ def bytes_per_position(Np, No, nl, no, npm, ns, momentum_fraction):
     return core + mom
 
 
-def read_traffic():
-    if not TRAFFIC_FILE.exists():
+def read_traffic(path=TRAFFIC_FILE):
+    if not path.exists():
         return None
-    txt = TRAFFIC_FILE.read_text()
+    txt = path.read_text()
     m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
     return float(m.group(1)) if m else None
 
@@ -267,7 +271,10 @@ def main_batched(args):
     # inputs first: the generator forks worker processes, which must happen
     # before this process initialises the GPU
     t_gen = time.perf_counter()
-    ws = S.batched_instances(n=args.instances, workers=min(16, os.cpu_count() or 1), inputs=args.batched_inputs)
+    ws = S.batched_instances(n=args.instances, workers=min(args.gen_workers, os.cpu_count() or 1),
+                             inputs=args.batched_inputs,
+                             progress=lambda i: print(f"[bench] input {i + 1}/{args.instances}", file=sys.stderr,
+                                                      flush=True))
     prop, npsm = ("ASP", 4) if args.batched_inputs == "asp_modes" else ("Fraunhofer", 1)
     print(f"[bench] {len(ws)} workloads generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     if not torch.cuda.is_available():
@@ -310,6 +317,12 @@ def main_batched(args):
     plan = P._native.Plan(insts, [s_[1] for s_ in staged], stream)
     plan.use_graph(not args.no_graph)
     Fs = np.array([e.experimentalData.numFrames for e in engines])
+    if args.calibrate_copy:
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for it in range(W):
         plan.run(it, it + 1)
     plan.check()
@@ -363,7 +376,8 @@ def main_batched(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": read_traffic(BATCH_TRAFFIC_FILE)
+            if args.engine == "mPIE" and args.batched_inputs == "asp_modes" and args.instances == 64 else None,
             "kernel": "pty_batch_kernel",
             "bytes_per_step": bytes_iter,
             "mean_launch_ms": mean_launch_s * 1e3,

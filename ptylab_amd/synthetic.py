@@ -222,13 +222,20 @@ def batched_instance(i, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, inpu
     raise ValueError(f"unknown batched inputs {inputs!r}")
 
 
-def batched_instances(n=64, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, workers=1, inputs="asp_modes"):
+def batched_instances(n=64, seed0=1000, nps=(64, 128, 256), fmin=100, fmax=800, workers=1, inputs="asp_modes",
+                      progress=None):
     """Config "batched": n instances (batched_instance(i), i < n).  With
     workers > 1 they are generated by a pool of forked processes; call it
-    before the process initialises the GPU."""
+    before the process initialises the GPU.  ``progress(i)``: called after
+    instance i when they are generated in this process."""
     args = [(i, seed0, nps, fmin, fmax, inputs) for i in range(n)]
     if workers <= 1:
-        return [batched_instance(*a) for a in args]
+        out = []
+        for a in args:
+            out.append(batched_instance(*a))
+            if progress is not None:
+                progress(a[0])
+        return out
     import multiprocessing as mp
 
     with mp.get_context("fork").Pool(workers) as pool:
