@@ -15,6 +15,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libptyhip.so"
 
 PTY_ABI_VERSION = 3
 PTY_FLAG_BATCHED = 1
+# error codes (include/pty_hip.h)
+PTY_OK, PTY_EINVAL, PTY_ENOMEM, PTY_EDEVICE, PTY_ETIMEDOUT = 0, -22, -12, -5, -110
 PTY_EPIE, PTY_MPIE, PTY_MULTIPIE, PTY_E3PIE, PTY_QNEWTON, PTY_MQNEWTON = 0, 1, 2, 3, 4, 5
 PTY_FRAUNHOFER, PTY_FRAUNHOFER_SHIFTED, PTY_ASP, PTY_FRESNEL, PTY_POLYCHROME_ASP, PTY_SCALED_ASP = 0, 1, 2, 3, 4, 5
 PTY_SCALED_POLYCHROME_ASP = 6
