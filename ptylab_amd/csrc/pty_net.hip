@@ -340,20 +340,21 @@ __global__ void pool_bwd_kernel(const bf16_t* __restrict__ dy, bf16_t* __restric
 // concatenation y (B, 2h, 2w, ca_s + cb_s); channels [ca_s, ...) copy skip
 __global__ void up_concat_kernel(const bf16_t* __restrict__ xa, int ca_s, const bf16_t* __restrict__ xb, int cb_s,
                                  bf16_t* __restrict__ y, int B, int H, int W) {
-  const int cs = ca_s + cb_s;
-  const size_t n = (size_t)B * H * W * cs;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % cs);
-    const size_t pix = i / cs;
-    if (c < ca_s) {
-      const int x = (int)(pix % W);
-      const size_t r = pix / W;
-      const int yy = (int)(r % H);
-      const int b = (int)(r / H);
-      y[i] = xa[(((size_t)b * (H / 2) + yy / 2) * (W / 2) + x / 2) * ca_s + c];
+  // one thread moves 8 channels (16 bytes); channel strides are multiples of
+  // 8 and the host keeps the chunk count below 2^31 (32-bit index math)
+  const unsigned ca8 = (unsigned)ca_s / 8u, cs8 = (unsigned)(ca_s + cb_s) / 8u;
+  const unsigned n = (unsigned)(B * H * W) * cs8;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned c8 = i % cs8, pix = i / cs8;
+    uint4 v;
+    if (c8 < ca8) {
+      const unsigned x = pix % (unsigned)W, r = pix / (unsigned)W;
+      const unsigned yy = r % (unsigned)H, b = r / (unsigned)H;
+      v = *(const uint4*)(xa + (((size_t)b * (H / 2) + yy / 2) * (W / 2) + x / 2) * ca_s + 8 * c8);
     } else {
-      y[i] = xb[pix * cb_s + (c - ca_s)];
+      v = *(const uint4*)(xb + (size_t)pix * cb_s + 8 * (c8 - ca8));
     }
+    *(uint4*)(y + (size_t)i * 8) = v;
   }
 }
 
@@ -563,7 +564,8 @@ int pty_net_pool_bwd(const void* dy, void* dx, int B, int H, int W, int c_s, voi
 
 int pty_net_up_concat(const void* xa, int ca_s, const void* xb, int cb_s, void* y, int B, int H, int W, void* stream) {
   if (!xa || !xb || !y || H % 2 || W % 2 || ca_s % 8 || cb_s % 8) return set_err(PTY_NET_EINVAL, "up_concat: bad arguments");
-  up_concat_kernel<<<grid_for((size_t)B * H * W * (ca_s + cb_s)), 256, 0, (hipStream_t)stream>>>(
+  if ((size_t)B * H * W * ((ca_s + cb_s) / 8) >= ((size_t)1 << 31)) return set_err(PTY_NET_EINVAL, "up_concat: too large");
+  up_concat_kernel<<<grid_for((size_t)B * H * W * ((ca_s + cb_s) / 8)), 256, 0, (hipStream_t)stream>>>(
       (const bf16_t*)xa, ca_s, (const bf16_t*)xb, cb_s, (bf16_t*)y, B, H, W);
   HIP_TRY(hipGetLastError());
   return PTY_NET_OK;
