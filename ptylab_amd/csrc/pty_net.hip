@@ -155,16 +155,18 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvFwd a) {
   int ci = 8 * kc - tap * a.cin_s;
   int ky = tap / a.ks, kx = tap - ky * a.ks;
   f32x16 acc0 = {}, acc1 = {};
-  for (int k0 = 0; k0 < a.Kp; k0 += CV_KT) {
-    // A tile: 128 pixels x 32 k, two 8-element chunks per thread
-    const bool kin = k0 + 8 * kc < a.K;
+  // Software pipeline: the chunks of step k0 + 32 are loaded into registers
+  // while the MFMAs work on step k0 in LDS.
+  uint4 ra[2], rb;
+  const int brow = t >> 2, co_b = n0 + brow;
+  auto fetch = [&](int kb) {
+    const bool kin = kb + 8 * kc < a.K;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      ra[h] = make_uint4(0u, 0u, 0u, 0u);
       const int yy = py[h] + ky - half, xs = pxx[h] + kx - half;
       if (pin[h] && kin && yy >= 0 && yy < a.H && xs >= 0 && xs < a.W)
-        v = *(const uint4*)(img[h] + ((size_t)yy * a.W + xs) * a.cin_s + ci);
-      *(uint4*)(As + (row0 + 64 * h) * CV_LD + 8 * kc) = v;
+        ra[h] = *(const uint4*)(img[h] + ((size_t)yy * a.W + xs) * a.cin_s + ci);
     }
     ci += CV_KT;
     while (ci >= a.cin_s) {
@@ -174,13 +176,18 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(ConvFwd a) {
         ++ky;
       }
     }
-    {
-      const int row = t >> 2, kc = t & 3, co = n0 + row;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (co < a.cout) v = *(const uint4*)(a.w + (size_t)co * a.Kp + k0 + 8 * kc);
-      *(uint4*)(Bs + row * CV_LD + 8 * kc) = v;
-    }
+    rb = make_uint4(0u, 0u, 0u, 0u);
+    if (co_b < a.cout) rb = *(const uint4*)(a.w + (size_t)co_b * a.Kp + kb + 8 * kc);
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < a.Kp; k0 += CV_KT) {
+    // A tile: 128 pixels x 32 k, two 8-element chunks per thread; B tile:
+    // 64 output channels x 32 k, one chunk per thread
+#pragma unroll
+    for (int h = 0; h < 2; ++h) *(uint4*)(As + (row0 + 64 * h) * CV_LD + 8 * kc) = ra[h];
+    *(uint4*)(Bs + brow * CV_LD + 8 * kc) = rb;
     __syncthreads();
+    if (k0 + CV_KT < a.Kp) fetch(k0 + CV_KT);
 #pragma unroll
     for (int kk = 0; kk < CV_KT; kk += 16) {
       const bf16x8 av = *(const bf16x8*)(As + (32 * wave + (l & 31)) * CV_LD + kk + 8 * (l >> 5));
