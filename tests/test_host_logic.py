@@ -264,3 +264,17 @@ def test_batched_instances_follow_the_named_inputs():
     assert a.probe_true.shape[2] == 4 and b.probe_true.shape[2] == 1
     with pytest.raises(ValueError):
         S.batched_instance(3, nps=(32,), fmin=5, fmax=8, inputs="other")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and says why,
+    before touching a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
