@@ -33,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "position updates/s; time to object NRMSE 1e-2, learned-init vs plain mPIE"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector FMA) peak
 TRAFFIC_FILE = ROOT / "profiles" / "pty_iterate_kernel_mpie_asp_np256_traffic.txt"
 BATCH_TRAFFIC_FILE = ROOT / "profiles" / "pty_batch_kernel_mpie_asp_64inst_traffic.txt"
 
@@ -48,9 +49,14 @@ def parse_args():
                          "larger than its position count)")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--F", type=int, default=1000)
-    ap.add_argument("--workload", choices=("c1", "batched", "learned_init"), default="c1",
-                    help="c1: single mPIE ASP reconstruction (default); batched: 64 independent reconstructions; "
-                         "learned_init: time to object NRMSE, learned-init vs plain mPIE")
+    ap.add_argument("--workload", choices=("c1", "c0", "batched", "learned_init"), default="c1",
+                    help="c1: single mPIE ASP reconstruction (default); c0: single plain ePIE Fraunhofer "
+                         "reconstruction (Np=64, 100 positions; microseconds per position); batched: 64 independent "
+                         "reconstructions; learned_init: time to object NRMSE, learned-init vs plain mPIE")
+    ap.add_argument("--no-orthogonalization", action="store_true",
+                    help="c1: skip the probe-mode orthogonalisation every 10 iterations")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="batched: processes of the all-cores CPU baseline (one instance each)")
     ap.add_argument("--train-patterns", type=int, default=200000)
     ap.add_argument("--li-stride", type=int, default=4,
                     help="learned_init: predict and stitch the patches of every n-th position (the fallback check "
@@ -76,6 +82,30 @@ def bytes_per_position(Np, No, nl, no, npm, ns, momentum_fraction):
     core = 16 * Np * Np * (nl * no * ns + nl * npm * ns) + 4 * Np * Np
     mom = 48 * (nl * no * ns * No * No + nl * npm * ns * Np * Np) * momentum_fraction
     return core + mom
+
+
+def fft_flops_per_position(Np, n_fft):
+    """Nominal FFT work of one position update: n_fft 2D transforms of
+    5 Np^2 log2(Np^2) flops each (the radix-2 operation count)."""
+    return n_fft * 5.0 * Np * Np * np.log2(float(Np * Np))
+
+
+def host_info():
+    """CPU model, usable cores and NumPy version of the host that ran the
+    CPU baseline."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "host_cores": usable, "numpy": np.__version__}
 
 
 def read_traffic(path=TRAFFIC_FILE):
@@ -109,6 +139,7 @@ def cpu_baseline(workload, n_positions):
         "kind": "port",
         "sample": f"NumPy oracle, mPIE ASP npsm=4 Np=256 No=1024: {iters} iteration(s) of {per_iter} positions "
                   f"({n} updates, {dt:.1f} s)",
+        **host_info(),
     }
 
 
@@ -122,6 +153,8 @@ def main():
         return main_batched(args)
     if args.workload == "learned_init":
         return main_learned_init(args)
+    if args.workload == "c0":
+        return main_c0(args)
     import torch
 
     import ptylab_amd as P
@@ -141,6 +174,10 @@ def main():
     params = P.Params()
     params.propagatorType = "ASP"
     params.positionOrder = "random"
+    # "feedback every 10 iterations": the snapShots orthogonalisation of the
+    # four probe modes every 10 iterations (Params.py:78-79)
+    params.orthogonalizationSwitch = not args.no_orthogonalization
+    params.orthogonalizationFrequency = 10
     rec = P.Reconstruction(data, params)
     rec.No = w.No
     rec.npsm = 4
@@ -162,8 +199,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    orth = bool(params.orthogonalizationSwitch)
     for it in range(W):
         plan.run(it, it + 1)
+        if orth:
+            eng.applyConstraints(it)
     plan.check()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
@@ -172,6 +212,10 @@ def main():
         ev[i][0].record(stream)
         plan.run(W + i, W + i + 1)
         ev[i][1].record(stream)
+        if orth:
+            # the probe modes' snapShots orthogonalisation at every 10th
+            # iteration (Params.orthogonalizationFrequency), inside the clock
+            eng.applyConstraints(W + i)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     plan.check()
@@ -184,6 +228,7 @@ def main():
     mean_launch_s = float(np.mean(launch_ms)) / 1e3
     achieved = bpos * F / mean_launch_s / 1e9
     traffic = read_traffic()
+    fpos = fft_flops_per_position(256, 4 * 4)  # ASP: 4 transforms per mode and position, 4 modes
     out = {
         "metric": METRIC,
         "value": K * F / elapsed,
@@ -201,6 +246,15 @@ def main():
             "workload": "mPIE, ASP, npsm=4 modes, Np=256, No=1024, F=%d positions per step (configuration 1)" % F,
             "positions_per_step": F,
             "engine_workgroups": n_wg,
+            "orthogonalization_every": 10 if orth else None,
+            "us_per_position": mean_launch_s / F * 1e6,
+        },
+        "fft": {
+            "flops_per_position": fpos,
+            "achieved": fpos * F / mean_launch_s / 1e12,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": fpos * F / mean_launch_s / 1e12 / FP32_PEAK_TFLOPS,
         },
         "roofline": {
             "bound": "hbm",
@@ -251,6 +305,50 @@ def cpu_baseline_batched(workloads, engine, propagator, npsm, budget_s=15.0):
         "sample": f"NumPy oracle, {engine} {propagator} npsm={npsm}, one iteration of the first {len(names)} "
                   f"batch instances "
                   f"({', '.join(names)}; {done} positions, {t_all:.1f} s)",
+        **host_info(),
+    }
+
+
+def _oracle_instance_sample(args):
+    """One oracle iteration over the first `limit` positions of one batch
+    instance, in a worker process; returns (positions, seconds)."""
+    import oracle as O
+
+    w, engine, propagator, npsm, limit = args
+    geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No,
+                     npsm=npsm)
+    np.random.seed(1)
+    obj = O.initial_guess(geo.shape_O, "ones", geo)
+    pr = O.initial_guess(geo.shape_P, "circ", geo)
+    run = O.OracleRun(geo, obj, pr, engine=engine, propagator=propagator, numIterations=1)
+    n = min(limit, w.ptychogram.shape[0])
+    t0 = time.perf_counter()
+    run.reconstruct(positions_limit=n)
+    return n, time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(workloads, engine, propagator, npsm, procs, limit=150):
+    """All-cores CPU throughput of the batch: `procs` forked processes, one
+    instance each (the first `procs` instances), every process one oracle
+    iteration over at most `limit` positions; aggregate = positions / wall
+    time.  Runs before this process initialises the GPU (fork)."""
+    import multiprocessing as mp
+
+    procs = max(1, min(procs, len(workloads)))
+    jobs = [(w, engine, propagator, npsm, limit) for w in workloads[:procs]]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_oracle_instance_sample, jobs, chunksize=1)
+    wall = time.perf_counter() - t0
+    n = int(sum(r[0] for r in res))
+    return {
+        "value": n / wall,
+        "unit": "position updates/s",
+        "cores": procs,
+        "kind": "port",
+        "sample": f"NumPy oracle, {engine} {propagator} npsm={npsm}: {procs} processes, one batch instance each "
+                  f"(the first {procs}), one iteration of at most {limit} positions ({n} updates, {wall:.1f} s wall)",
+        **host_info(),
     }
 
 
@@ -277,6 +375,11 @@ def main_batched(args):
                                                       flush=True))
     prop, npsm = ("ASP", 4) if args.batched_inputs == "asp_modes" else ("Fraunhofer", 1)
     print(f"[bench] {len(ws)} workloads generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    all_cores = None
+    if not args.no_cpu_baseline and args.cpu_procs > 1:
+        # forked processes: before this process initialises the GPU
+        print("[bench] timing the all-cores CPU oracle baseline", file=sys.stderr, flush=True)
+        all_cores = cpu_baseline_all_cores(ws, args.engine, prop, npsm, args.cpu_procs)
     if not torch.cuda.is_available():
         print("no HIP GPU available", file=sys.stderr)
         sys.exit(1)
@@ -340,9 +443,12 @@ def main_batched(args):
     n_wg = plan.workgroups()
     momentum = args.engine == "mPIE"
     bytes_iter = 0.0
+    flops_iter = 0.0
+    n_fft = (4 if prop == "ASP" else 2) * npsm
     for e, w in zip(engines, ws):
         frac = float(e._masks[W:W + K].sum()) / (K * e.experimentalData.numFrames) if momentum else 0.0
         bytes_iter += e.experimentalData.numFrames * bytes_per_position(w.Np, w.No, 1, 1, npsm, 1, frac)
+        flops_iter += e.experimentalData.numFrames * fft_flops_per_position(w.Np, n_fft)
     for e in engines:
         e._collect(W + K)
     plan.close()
@@ -382,11 +488,130 @@ def main_batched(args):
             "bytes_per_step": bytes_iter,
             "mean_launch_ms": mean_launch_s * 1e3,
         },
+        "fft": {
+            "flops_per_step": flops_iter,
+            "achieved": flops_iter / mean_launch_s / 1e12,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": flops_iter / mean_launch_s / 1e12 / FP32_PEAK_TFLOPS,
+        },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_batched(ws, args.engine, prop, npsm)
+        if all_cores is not None:
+            out["cpu_baseline_all_cores"] = all_cores
+    print(json.dumps(out))
+
+
+def main_c0(args):
+    """Plain ePIE, Fraunhofer, one reconstruction (the CPU-runnable
+    configuration: Np = 64, No = 256, 100 positions, 1e4 photons): a step is
+    one iteration (one launch); reported as microseconds per position and
+    as the (small) HBM fraction of a latency-bound single reconstruction."""
+    import torch
+
+    import ptylab_amd as P
+    from ptylab_amd import synthetic as S
+
+    if not torch.cuda.is_available():
+        print("no HIP GPU available", file=sys.stderr)
+        sys.exit(1)
+    w = S.cpm_fraunhofer(Np=64, No=256, F=100, seed=0)
+    data = P.ExperimentalData()
+    for k, v in w.as_dict().items():
+        setattr(data, k, v)
+    data._setData()
+    params = P.Params()
+    params.propagatorType = "Fraunhofer"
+    params.positionOrder = "random"
+    rec = P.Reconstruction(data, params)
+    rec.No = w.No
+    np.random.seed(3)
+    rec.initializeObjectProbe()
+    eng = P.Engines.ePIE(rec, data, params, P.DummyMonitor())
+    eng.workgroups = args.workgroups
+    K, W = args.steps, args.warmup
+    eng.numIterations = W + K
+    np.random.seed(3)
+    eng._prepareReconstruction()
+    plan, T = eng._begin()
+    stream = torch.cuda.current_stream()
+    F = data.numFrames
+    for it in range(W):
+        plan.run(it, it + 1)
+    plan.check()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    t0 = time.perf_counter()
+    for i in range(K):
+        ev[i][0].record(stream)
+        plan.run(W + i, W + i + 1)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    plan.check()
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    n_wg = plan.workgroups()
+    eng._end(W + K)
+    bpos = bytes_per_position(64, w.No, 1, 1, 1, 1, 0.0)
+    mean_launch_s = float(np.mean(launch_ms)) / 1e3
+    achieved = bpos * F / mean_launch_s / 1e9
+    fpos = fft_flops_per_position(64, 2)
+    out = {
+        "metric": METRIC,
+        "value": K * F / elapsed,
+        "unit": "position updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "ePIE, Fraunhofer, Np=64, No=256, F=100 positions per step (configuration 0)",
+            "positions_per_step": F,
+            "engine_workgroups": n_wg,
+            "us_per_position": mean_launch_s / F * 1e6,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "pty_iterate_kernel",
+            "bytes_per_position": bpos,
+            "mean_launch_ms": mean_launch_s * 1e3,
+        },
+        "fft": {
+            "flops_per_position": fpos,
+            "achieved": fpos * F / mean_launch_s / 1e12,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": fpos * F / mean_launch_s / 1e12 / FP32_PEAK_TFLOPS,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+
+        geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+        np.random.seed(3)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        run = O.OracleRun(geo, obj, pr, engine="ePIE", propagator="Fraunhofer", numIterations=20)
+        t0 = time.perf_counter()
+        run.reconstruct()
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 20 * F / dt, "unit": "position updates/s", "cores": 1, "kind": "port",
+                               "sample": f"NumPy oracle, ePIE Fraunhofer Np=64 No=256: 20 iterations of {F} positions "
+                                         f"({20 * F} updates, {dt:.1f} s)", **host_info()}
     print(json.dumps(out))
 
 

@@ -435,6 +435,10 @@ class OracleRun:
         self.object = np.asarray(self.object).astype(c)
         self.probe = np.asarray(self.probe).astype(c)
         self.ptychogram = g.ptychogram.astype(np.float64) if self.precision == "c128" else g.ptychogram
+        if self.fftshiftSwitch:
+            # _checkFFT (BaseEngine.py:405-422): the detector data are
+            # ifftshifted once so the unshifted transforms see them in FFT order
+            self.ptychogram = np.fft.ifftshift(self.ptychogram, axes=(-1, -2))
         self.energy = energy_at_pos(self.ptychogram)
         self.err_at_pos_sum = np.zeros(self.ptychogram.shape[0])
         self.errorAtPos = None

@@ -7,8 +7,11 @@ missing or a GPU is absent every entry point raises.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libptyhip.so"
@@ -199,8 +202,24 @@ def coresident_workgroups(Np, engine, propagator, batched):
     return n.value
 
 
+_live_plans = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_plans():
+    """Destroy the plans still open when the interpreter exits, while the
+    HIP runtime (and any profiler layered on it) is still up: Python's
+    atexit handlers run before the C-level exit handlers of the loaded
+    libraries, so no HIP call is left for module teardown."""
+    for plan in list(_live_plans):
+        plan.close()
+
+
 class Plan:
-    """RAII wrapper of a pty_plan; keeps the tensors it points to alive."""
+    """RAII wrapper of a pty_plan; keeps the tensors it points to alive.
+    Close it explicitly (``close()``); plans still open at interpreter exit
+    are closed by an atexit hook, and a plan collected during interpreter
+    finalisation is left to the process exit (no HIP call then)."""
 
     def __init__(self, instances, keepalive, stream=None):
         self._lib = load()
@@ -211,6 +230,7 @@ class Plan:
         h = ctypes.c_void_p()
         check(self._lib.pty_plan_create(ctypes.byref(h), arr, len(instances), _stream_handle(stream)))
         self._h = h
+        _live_plans.add(self)
 
     def run(self, it_begin, it_end):
         check(self._lib.pty_plan_run(self._h, int(it_begin), int(it_end), _stream_handle(self.stream)))
@@ -234,8 +254,11 @@ class Plan:
         if getattr(self, "_h", None):
             self._lib.pty_plan_destroy(self._h)
             self._h = None
+        _live_plans.discard(self)
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

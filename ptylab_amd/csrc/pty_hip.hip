@@ -2323,6 +2323,17 @@ int pty_abi_version(void) { return PTY_ABI_VERSION; }
 
 const char* pty_last_error(void) { return g_err.c_str(); }
 
+// HIP_TRY inside pty_plan_create: record the error and leave through the
+// function's `fail` label, which frees the half-built plan
+#define PLAN_TRY(expr)                                                                       \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      rc = set_err(PTY_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));              \
+      goto fail;                                                                             \
+    }                                                                                        \
+  } while (0)
+
 int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, void* hip_stream) {
   if (!out || !insts || ninst < 1) return set_err(PTY_EINVAL, "pty_plan_create: bad arguments");
   *out = nullptr;
@@ -2330,7 +2341,13 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
   if (rc) return rc;
   hipStream_t st = (hipStream_t)hip_stream;
   pty_plan* p = new pty_plan();
-  HIP_TRY(hipGetDevice(&p->device));
+  {
+    const hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) {
+      delete p;
+      return set_err(PTY_EDEVICE, "hipGetDevice failed: %s", hipGetErrorString(e));
+    }
+  }
   p->ninst = ninst;
   p->h_insts.resize(ninst);
   std::vector<std::vector<int2>> cmaps;  // per launch class
@@ -2400,7 +2417,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     d.omAlphaP = (float)(1.0 - (double)a.alphaProbe);
     // positions must keep every patch inside the object
     std::vector<int32_t> pos((size_t)a.F * 2);
-    HIP_TRY(hipMemcpy(pos.data(), a.positions, pos.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PLAN_TRY(hipMemcpy(pos.data(), a.positions, pos.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int f = 0; f < a.F; ++f) {
       const int r = pos[2 * f], c = pos[2 * f + 1];
       if (r < 0 || c < 0 || r + np > a.No || c + np > a.No) {
@@ -2411,7 +2428,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       }
     }
     std::vector<int32_t> ord((size_t)a.F * a.n_iter);
-    HIP_TRY(hipMemcpy(ord.data(), a.order, ord.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PLAN_TRY(hipMemcpy(ord.data(), a.order, ord.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < ord.size(); ++q) {
       if (ord[q] < 0 || ord[q] >= a.F) {
         free_plan(p);
@@ -2440,7 +2457,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       // the second moments start at zero (a fresh engine's momentum copy,
       // mqNewton.py:76-84) and persist across the plan's launches
       if ((rc = dalloc(p, &mem, 4 * sizeof(double)))) goto fail;
-      HIP_TRY(hipMemsetAsync(mem, 0, 4 * sizeof(double), st));
+      PLAN_TRY(hipMemsetAsync(mem, 0, 4 * sizeof(double), st));
       d.adamv = (double*)mem;
     }
     if ((rc = dalloc(p, &mem, CTRL_WORDS * sizeof(unsigned)))) goto fail;
@@ -2456,7 +2473,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     } else {
       if ((rc = dalloc(p, &mem, (size_t)a.F * np2 * sizeof(float)))) goto fail;
       pty_permute_frames_kernel<<<1024, 256, 0, st>>>(a.ptychogram, (float*)mem, a.F, np, log2_np(np));
-      HIP_TRY(hipGetLastError());
+      PLAN_TRY(hipGetLastError());
       d.Ik = (const float*)mem;
     }
     d.fpl = d.nF;
@@ -2470,14 +2487,14 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       for (int t = 0; t < ntf; ++t)
         pty_permute_tf_kernel<<<256, 256, 0, st>>>((const float2*)a.tf + (size_t)t * np2, (float2*)mem + (size_t)t * np2,
                                                    np, log2_np(np));
-      HIP_TRY(hipGetLastError());
+      PLAN_TRY(hipGetLastError());
       d.HT = (const float2*)mem;
       if (a.propagator == PTY_SCALED_ASP || a.propagator == PTY_SCALED_POLYCHROME_ASP)
         d.Q = (const float2*)a.tf + (size_t)ntf * np2;  // c Q1 per wavelength, natural order
       if (prop_asp(a.propagator) && a.engine != PTY_E3PIE) {
         if ((rc = dalloc(p, &mem, (size_t)ntf * np))) goto fail;
         pty_live_columns_kernel<<<(ntf * np + 255) / 256, 256, 0, st>>>(d.HT, (unsigned char*)mem, np, ntf);
-        HIP_TRY(hipGetLastError());
+        PLAN_TRY(hipGetLastError());
         d.live = (const unsigned char*)mem;
       }
     }
@@ -2547,12 +2564,12 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     p->nclaim = (int)p->classes.size() * N_XCD * 32;
     if ((rc = dalloc(p, &mem, sizeof(unsigned) * p->nclaim))) goto fail;
     p->d_claims = (unsigned*)mem;
-    HIP_TRY(hipMemsetAsync(p->d_claims, 0, sizeof(unsigned) * p->nclaim, st));
+    PLAN_TRY(hipMemsetAsync(p->d_claims, 0, sizeof(unsigned) * p->nclaim, st));
     for (size_t c = 0; c < p->classes.size(); ++c) p->classes[c].d_claim = p->d_claims + c * N_XCD * 32;
-    HIP_TRY(hipMemcpyAsync(p->d_insts, p->h_insts.data(), sizeof(DevInst) * ninst, hipMemcpyHostToDevice, st));
+    PLAN_TRY(hipMemcpyAsync(p->d_insts, p->h_insts.data(), sizeof(DevInst) * ninst, hipMemcpyHostToDevice, st));
     int cus = 0, coop = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, p->device));
+    PLAN_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+    PLAN_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, p->device));
     p->nblocks = 0;
     for (size_t c = 0; c < p->classes.size(); ++c) {
       LaunchClass& lc = p->classes[c];
@@ -2560,11 +2577,11 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       for (const int2& e : cmaps[c]) p->nblocks += e.x >= 0;
       if ((rc = dalloc(p, &mem, sizeof(int2) * cmaps[c].size()))) goto fail;
       lc.d_bmap = (int2*)mem;
-      HIP_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
+      PLAN_TRY(hipMemcpyAsync(lc.d_bmap, cmaps[c].data(), sizeof(int2) * cmaps[c].size(), hipMemcpyHostToDevice, st));
       if (lc.cooperative) {
         // clusters spin on each other: the whole launch must be co-resident
         int per_cu = 0;
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np, lc.fam), lc.nthreads, 0));
+        PLAN_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, iterate_kernel_ptr(lc.np, lc.fam), lc.nthreads, 0));
         if (!coop) {
           rc = set_err(PTY_EDEVICE, "device does not support cooperative launches");
           goto fail;
@@ -2576,7 +2593,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
         }
       }
     }
-    HIP_TRY(hipStreamSynchronize(st));
+    PLAN_TRY(hipStreamSynchronize(st));
   }
   *out = p;
   return PTY_OK;

@@ -263,3 +263,97 @@ def test_position_correction_update_moves_the_encoder():
     np.testing.assert_allclose(shift, np.array([2.0, -1.0]) * geo.dxo - mean_shift, rtol=1e-9)
     np.testing.assert_allclose(run.encoder_corrected.mean(axis=0), geo.encoder.mean(axis=0), rtol=1e-9, atol=1e-18)
     np.testing.assert_array_equal(run.positions, O.pixel_positions(run.encoder_corrected, geo.dxo, geo.No, geo.Np))
+
+
+def _dup_channel_run(d, No, engine, ptychogram, nl=1, no=1, ns=1, **kw):
+    """An oracle run whose nl wavelengths, no object modes or ns slices are
+    identical copies of one channel (same seeded initial object and probe)."""
+    sd = np.full(nl, float(d["wavelength"])) if nl > 1 else None
+    geo = O.Geometry(ptychogram, d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
+                     No=No, nlambda=nl, nosm=no, nslice=ns, spectralDensity=sd)
+    one = O.Geometry(ptychogram, d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"], No=No)
+    np.random.seed(4)
+    obj = np.broadcast_to(O.initial_guess(one.shape_O, "ones", one), geo.shape_O).copy()
+    pr = np.broadcast_to(O.initial_guess(one.shape_P, "circ", one), geo.shape_P).copy()
+    run = O.OracleRun(geo, obj, pr, engine=engine, numIterations=3, alphaObject=1.0, alphaProbe=1.0, **kw)
+    np.random.seed(5)
+    return run.reconstruct()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+def test_multipie_duplicated_channels_reduce_to_one_channel():
+    """multiPIE's own axes (multiPIE.py:153-205), pinned by what duplicated
+    channels must give.  With alphaObject = alphaProbe = 1 the update weights
+    are 1/Pmax and 1/Omax, and Pmax, Omax sum over wavelengths, object modes
+    and slices (axes 0-3).  Two identical wavelengths or object modes on data
+    I see Ihat = 2|Psi|^2, i.e. the same amplitude factor as one channel on
+    I/2; Omax doubles and the probe increment sums over axes (0, 1), so each
+    copy gets exactly the one-channel probe increment.  For wavelengths Pmax
+    doubles too, which betaObject = 2 x 0.25 compensates.  Four identical
+    slices see Ihat of the last slice only, so the data stay I; Pmax and Omax
+    quadruple, compensated by betaObject = betaProbe = 4 x 0.25, and the
+    probe sum runs over axes (0, 1) only, so every slice copy equals the
+    one-slice run.  mPIE's probe sum over axis 1 alone fails the wavelength
+    case by tens of percent: the check separates the two sums."""
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=32, No=96, F=12, seed=3, step_frac=0.3)
+    d = w.as_dict()
+    I = d["ptychogram"]
+    half = _dup_channel_run(d, w.No, "multiPIE", I / 2)
+    lam = _dup_channel_run(d, w.No, "multiPIE", I, nl=2, betaObject=0.5)
+    osm = _dup_channel_run(d, w.No, "multiPIE", I, no=2)
+    for i in range(2):
+        # the only difference is the 1e-10 guard of Ihat, halved on I/2
+        assert _rel(lam.object[i], half.object[0]) < 1e-5 and _rel(lam.probe[i], half.probe[0]) < 1e-5
+        assert _rel(osm.object[0, i], half.object[0, 0]) < 1e-5
+    assert _rel(osm.probe, half.probe) < 1e-5
+    np.testing.assert_allclose(lam.error, half.error, rtol=1e-5)
+    np.testing.assert_allclose(osm.error, half.error, rtol=1e-5)
+    one = _dup_channel_run(d, w.No, "multiPIE", I)
+    sl = _dup_channel_run(d, w.No, "multiPIE", I, ns=4, betaObject=1.0, betaProbe=1.0)
+    for s in range(4):
+        assert _rel(sl.object[0, 0, 0, s], one.object[0, 0, 0, 0]) < 1e-6
+        assert _rel(sl.probe[0, 0, 0, s], one.probe[0, 0, 0, 0]) < 1e-6
+    np.testing.assert_allclose(sl.error, one.error, rtol=1e-6)
+    # the same duplication through mPIE's probe sum (axis 1 only) halves
+    # every wavelength's probe increment: the identity breaks
+    mp = _dup_channel_run(d, w.No, "mPIE", I, nl=2, betaObject=0.5)
+    assert _rel(mp.probe[0], half.probe[0]) > 1e-2
+
+
+def test_fftshift_switch_is_the_same_reconstruction():
+    """fftshiftSwitch=True (BaseEngine.py:405-422, utils.py:20-47): the data
+    are ifftshifted once and the transforms run without shifts.  For even Np
+    the shift of the exit wave only multiplies its spectrum by (-1)^(kx+ky),
+    which |.|^2 removes and the back-transform undoes, so in exact arithmetic
+    the reconstruction is the unshifted one; in complex64 they agree to
+    rounding.  Forgetting the data shift breaks this by orders of magnitude."""
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=32, No=96, F=12, seed=3, step_frac=0.3)
+    d = w.as_dict()
+    geo = O.Geometry(d["ptychogram"], d["encoder"], d["wavelength"], d["dxd"], d["zo"], d["entrancePupilDiameter"],
+                     No=w.No)
+    runs = []
+    for sw in (False, True):
+        np.random.seed(4)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        r = O.OracleRun(geo, obj, pr, engine="mPIE", numIterations=3, fftshiftSwitch=sw)
+        np.random.seed(5)
+        runs.append(r.reconstruct())
+    assert _rel(runs[1].object, runs[0].object) < 1e-5
+    assert _rel(runs[1].probe, runs[0].probe) < 1e-5
+    np.testing.assert_allclose(runs[1].error, runs[0].error, rtol=1e-5)
+    # the transforms alone, without the data shift, are a different problem
+    np.random.seed(4)
+    bad = O.OracleRun(geo, O.initial_guess(geo.shape_O, "ones", geo), O.initial_guess(geo.shape_P, "circ", geo),
+                      engine="mPIE", numIterations=3, fftshiftSwitch=True)
+    bad.ptychogram = np.fft.fftshift(bad.ptychogram, axes=(-1, -2))
+    np.random.seed(5)
+    bad.reconstruct()
+    assert _rel(bad.object, runs[0].object) > 1e-2
