@@ -473,7 +473,11 @@ class BaseEngine:
         for il in range(r.nlambda):
             for isl in range(r.nslice):
                 p2 = sel(X, il, isl).reshape(n, npx * npx)
-                G = torch.matmul(p2.conj(), p2.transpose(0, 1).contiguous())
+                # Gram matrix G[i, j] = sum_x conj(p_i) p_j as an elementwise
+                # product and a reduction, accumulated in complex128 (a library
+                # GEMM with K = Np^2 and an n x n output ran on one workgroup)
+                q = p2.to(torch.complex128)
+                G = (q.conj()[:, None, :] * q[None, :, :]).sum(dim=-1)
                 w, V = np.linalg.eigh(G.cpu().numpy())
                 order = np.argsort(w)[::-1]
                 s = np.sqrt(np.clip(w[order].real, 0.0, None))
