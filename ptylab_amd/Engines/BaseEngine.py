@@ -482,7 +482,7 @@ class BaseEngine:
                 order = np.argsort(w)[::-1]
                 s = np.sqrt(np.clip(w[order].real, 0.0, None))
                 ev = s ** 2 / np.sum(s ** 2)
-                Vt = torch.as_tensor(np.ascontiguousarray(V[:, order].T), device=X.device)
+                Vt = torch.as_tensor(np.ascontiguousarray(V[:, order].T, dtype=np.complex64), device=X.device)
                 sel(X, il, isl).copy_(torch.matmul(Vt, p2).reshape(n, npx, npx))
                 if key == "probe":
                     r.purityProbe = float(np.sqrt(np.sum(ev ** 2)))
