@@ -251,6 +251,18 @@ int pty_field_moments(const void* x, int32_t N, const float* X, const float* Y, 
 int pty_roll_fields(void* x, void* scratch, int32_t nfields, int32_t Ny, int32_t Nx, int32_t dy, int32_t dx,
                     void* hip_stream);
 
+/* pty_roll_fields without a workspace and without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * scratch is a static device array of PTY_ROLL_SCRATCH_MAX_VALUES complex64
+ * values (16 MiB) per GPU, held until the process ends; nfields*Ny*Nx above
+ * that returns PTY_EINVAL.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams or host threads) race.
+ * Two kernel launches, no memset or memcpy nodes. */
+#define PTY_ROLL_SCRATCH_MAX_VALUES 2097152 /* 2 Mi complex64 values = 16 MiB */
+int pty_roll_fields_noalloc(void* x, int32_t nfields, int32_t Ny, int32_t Nx, int32_t dy, int32_t dx,
+                            void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,7 +46,9 @@ EXPORTED = (
     "pty_window_fields",
     "pty_field_moments",
     "pty_roll_fields",
+    "pty_roll_fields_noalloc",
 )
+PTY_ROLL_SCRATCH_MAX_VALUES = 2 * 1024 * 1024  # include/pty_hip.h
 
 
 class PtyInstance(ctypes.Structure):
@@ -140,6 +142,7 @@ def load(path: os.PathLike | None = None):
     lib.pty_window_fields.argtypes = [vp, i32, i32, vp, f32, vp]
     lib.pty_field_moments.argtypes = [vp, i32, vp, vp, vp, vp]
     lib.pty_roll_fields.argtypes = [vp, vp, i32, i32, i32, i32, i32, vp]
+    lib.pty_roll_fields_noalloc.argtypes = [vp, i32, i32, i32, i32, i32, vp]
     for name in EXPORTED[2:]:
         getattr(lib, name).restype = ctypes.c_int
     if lib.pty_abi_version() != PTY_ABI_VERSION:

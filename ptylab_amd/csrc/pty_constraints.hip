@@ -6,6 +6,8 @@
 // are deterministic.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "pty_hip.h"
 
 // pty_last_error() message setter of libptyhip (pty_hip.hip), not exported
@@ -95,10 +97,43 @@ __global__ void roll_kernel(const float2* __restrict__ src, float2* __restrict__
   }
 }
 
+__global__ void copy_kernel(const float2* __restrict__ src, float2* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 int grid_for(size_t n) {
   size_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
   return (int)(g ? g : 1);
+}
+
+// Scratch of pty_roll_fields_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never uses its name as an
+// address: the device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t ROLL_SCRATCH_MAX = PTY_ROLL_SCRATCH_MAX_VALUES;
+static __device__ __align__(256) float2 scratch[ROLL_SCRATCH_MAX];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float2* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The calling thread's current device's copy of `scratch`, looked up on the
+// first call on that device.
+int device_scratch(float2** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+    return pty_internal_set_error(PTY_EDEVICE, "pty_roll_fields_noalloc: no current HIP device");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return pty_internal_set_error(PTY_EDEVICE, "pty_roll_fields_noalloc: hipGetSymbolAddress(scratch) failed");
+    g_scratch_ptr[dev] = (float2*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PTY_OK;
 }
 
 }  // namespace
@@ -140,6 +175,25 @@ int pty_roll_fields(void* x, void* scratch, int32_t nfields, int32_t Ny, int32_t
   if (hipGetLastError() != hipSuccess) return pty_internal_set_error(PTY_EDEVICE, "pty_roll_fields: launch failed");
   if (hipMemcpyAsync(x, scratch, n * sizeof(float2), hipMemcpyDeviceToDevice, st) != hipSuccess)
     return pty_internal_set_error(PTY_EDEVICE, "pty_roll_fields: copy back failed");
+  return PTY_OK;
+}
+
+int pty_roll_fields_noalloc(void* x, int32_t nfields, int32_t Ny, int32_t Nx, int32_t dy, int32_t dx, void* stream) {
+  if (!x || nfields < 1 || Ny < 1 || Nx < 1)
+    return pty_internal_set_error(PTY_EINVAL, "pty_roll_fields_noalloc: bad arguments");
+  const size_t n = (size_t)nfields * Ny * Nx;
+  if (n > ROLL_SCRATCH_MAX)
+    return pty_internal_set_error(PTY_EINVAL, "pty_roll_fields_noalloc: more than PTY_ROLL_SCRATCH_MAX_VALUES "
+                                              "complex values (use pty_roll_fields with a workspace)");
+  float2* buf = nullptr;
+  const int rc = device_scratch(&buf);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  // the roll writes every element of buf[0, n) that the copy reads: nothing to clear
+  roll_kernel<<<grid_for(n), 256, 0, st>>>((const float2*)x, buf, (size_t)nfields, Ny, Nx, dy, dx);
+  if (hipGetLastError() != hipSuccess) return pty_internal_set_error(PTY_EDEVICE, "pty_roll_fields_noalloc: launch failed");
+  copy_kernel<<<grid_for(n), 256, 0, st>>>(buf, (float2*)x, n);
+  if (hipGetLastError() != hipSuccess) return pty_internal_set_error(PTY_EDEVICE, "pty_roll_fields_noalloc: launch failed");
   return PTY_OK;
 }
 
