@@ -36,6 +36,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector FMA) peak
 TRAFFIC_FILE = ROOT / "profiles" / "pty_iterate_kernel_mpie_asp_np256_traffic.txt"
 BATCH_TRAFFIC_FILE = ROOT / "profiles" / "pty_batch_kernel_mpie_asp_64inst_traffic.txt"
+BATCH_EPIE_TRAFFIC_FILE = ROOT / "profiles" / "pty_batch_kernel_epie_fraunhofer_64inst_traffic.txt"
 
 
 def parse_args():
@@ -114,6 +115,18 @@ def read_traffic(path=TRAFFIC_FILE):
     txt = path.read_text()
     m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
     return float(m.group(1)) if m else None
+
+
+def batch_traffic(args):
+    """Measured HBM bytes per launch of the batched workload, where a
+    profile of that exact workload is committed."""
+    if args.instances != 64:
+        return None
+    if args.engine == "mPIE" and args.batched_inputs == "asp_modes":
+        return read_traffic(BATCH_TRAFFIC_FILE)
+    if args.engine == "ePIE" and args.batched_inputs == "fraunhofer":
+        return read_traffic(BATCH_EPIE_TRAFFIC_FILE)
+    return None
 
 
 def cpu_baseline(workload, n_positions):
@@ -489,8 +502,7 @@ def main_batched(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(BATCH_TRAFFIC_FILE)
-            if args.engine == "mPIE" and args.batched_inputs == "asp_modes" and args.instances == 64 else None,
+            "traffic": batch_traffic(args),
             "kernel": "pty_batch_kernel",
             "bytes_per_step": bytes_iter,
             "mean_launch_ms": mean_launch_s * 1e3,

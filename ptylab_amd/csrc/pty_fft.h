@@ -7,8 +7,11 @@
 // CW lines interleaved lane by lane, as the column passes use).
 // Butterflies whose span is a multiple of LPF are done inside a lane; smaller
 // spans exchange values between lanes (mask span * LS) with DPP lane moves
-// and the permlane swaps (xlane).  No LDS and no workgroup barrier is needed
-// inside a transform.
+// and the permlane swaps (xlane) -- the cross-lane form, used when a lane
+// holds fewer than 4 values of the line.  With 4 or more values per lane the
+// transposed form below runs every butterfly inside a lane and moves the line
+// between register layouts through a wave-private LDS scratch.  No workgroup
+// barrier is needed inside a transform.
 //
 // fft_fwd: radix-2 decimation in frequency, natural order in, bit-reversed
 //          order out, kernel exp(-2 pi i nk / N), unnormalised.
@@ -87,7 +90,7 @@ struct Line {
 };
 
 template <int N, int LPF, int LS>
-__device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t) {
+__device__ __forceinline__ void fft_fwd_x(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t) {
   constexpr int L = N / LPF;
   // Opaque zero offset: keeps the compiler from hoisting the (loop-invariant)
   // twiddle loads out of the caller's loops into registers.
@@ -127,7 +130,7 @@ __device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __
 }
 
 template <int N, int LPF, int LS>
-__device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t) {
+__device__ __forceinline__ void fft_inv_x(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t) {
   constexpr int L = N / LPF;
   // Opaque zero offset: keeps the compiler from hoisting the (loop-invariant)
   // twiddle loads out of the caller's loops into registers.
@@ -166,14 +169,287 @@ __device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __
   }
 }
 
+// ---------------------------------------------------------------------------
+// Transposed form: every butterfly inside a lane.
+// ---------------------------------------------------------------------------
+// With L >= 4 values per lane the same radix-2 stages run without any
+// cross-lane arithmetic: the line is rewritten through a wave-private LDS
+// scratch so that the lg = log2(L) bits of n a group of stages pairs on are
+// register bits, and every butterfly is the in-lane one.  Layout k keeps bits
+// [k, k + lg) of n in the register index and the other bits in the lane:
+//   n(l, t; k) = ((t >> k) << (k + lg)) | (l << k) | (t & (2^k - 1)).
+// Layout k0 = log2(N) - lg is the caller's (n = l * P + t).  The forward
+// transform (decimation in frequency, bit 7 ... bit 0 for N = 256) visits
+// k0, k0 - lg, ..., 0 and returns to k0; the inverse visits them in reverse.
+// Each stage computes exactly what the cross-lane form computes (same
+// twiddle, same operations), so the results are the same bit for bit (up to
+// the sign of exact zeros: the cross-lane form multiplies the lower output
+// by 1 + 0i).
+//
+// Scratch addresses: element n of line c (the LS lines of a wave, lanes
+// t * LS + c) lives at c * STRIDE + n + cc * ((n >> s) & msk), with (s, msk,
+// cc) chosen per transpose so that the map is one-to-one and both the write
+// layout and the read layout are free of bank conflicts (a 64-bit access is
+// served 16 lanes at a time from 16 distinct 8-byte banks); the tables below
+// were found by exhaustive search over that family and are re-checked by
+// static_assert.  The address of n = (t part) | (l part) splits into a per-lane
+// base plus a constant offset per register, so every LDS access uses an
+// immediate offset.
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
+
+struct TPad {
+  int s, msk, cc, size, stride;  // padding n + cc * ((n >> s) & msk); elements of one line; line stride
+};
+
+// Padding of the transpose between layouts k1 and k2 (symmetric).
+constexpr TPad tpad(int N, int P, int LS, int k1, int k2) {
+  const int a = k1 < k2 ? k1 : k2, b = k1 < k2 ? k2 : k1;
+  if (N == 256 && P == 64 && LS == 1) {
+    if (a == 4 && b == 6) return {0, 0, 0, 256, 256};
+    if (a == 2 && b == 4) return {4, 15, 4, 316, 316};
+    if (a == 0 && b == 2) return {2, 63, 1, 319, 319};
+    if (a == 0 && b == 6) return {4, 15, 1, 271, 271};
+  }
+  if (N == 512 && P == 64 && LS == 1) {
+    if (a == 3 && b == 6) return {4, 31, 2, 574, 574};
+    if (a == 0 && b == 3) return {3, 63, 1, 575, 575};
+    if (a == 0 && b == 6) return {4, 31, 1, 543, 543};
+  }
+  if (N == 512 && P == 32 && LS == 2) return {4, 31, 1, 543, 552};
+  return {-1, 0, 0, 0, 0};
+}
+
+template <int N, int P, int LS>
+struct TPlan {
+  static constexpr int L = N / P, LG = ilog2c(L), M = ilog2c(N), K0 = M - LG;
+  // layout j of the forward sequence K0, K0 - LG, ... (clamped at 0)
+  static constexpr int kseq(int j) {
+    int k = K0;
+    for (int i = 0; i < j; ++i) k = k - LG > 0 ? k - LG : 0;
+    return k;
+  }
+  static constexpr int nlayouts() {
+    if (LG == 0) return 1;  // one value per lane: the cross-lane form
+    int j = 0;
+    while (kseq(j) > 0) ++j;
+    return j + 1;
+  }
+  static constexpr int NK = nlayouts();
+  static constexpr bool ENABLED = L >= 4;
+  // scratch elements per wave: LS lines of the largest stride used
+  static constexpr int scratch() {
+    int m = 0;
+    for (int j = 0; j < NK; ++j) {
+      const TPad a = tpad(N, P, LS, kseq(j), kseq(j + 1 < NK ? j + 1 : 0));
+      const int e = (LS == 1 ? a.size : a.stride * LS);
+      m = e > m ? e : m;
+    }
+    return m;
+  }
+  static constexpr int SCRATCH = ENABLED ? scratch() : 0;
+};
+
+__host__ __device__ constexpr int tpos(int l, int t, int k, int lg) {
+  return ((t >> k) << (k + lg)) | (l << k) | (t & ((1 << k) - 1));
+}
+__host__ __device__ constexpr int tpadded(const TPad& d, int n) { return n + d.cc * ((n >> d.s) & d.msk); }
+
+// one-to-one and conflict-free for both layouts (compile-time check)
+constexpr bool tpad_ok(int N, int P, int LS, int k1, int k2) {
+  const TPad d = tpad(N, P, LS, k1, k2);
+  if (d.s < 0) return false;
+  const int lg = ilog2c(N / P);
+  // one-to-one into [0, size): every padded index below size and distinct
+  if (d.size > 2048) return false;
+  bool seen[2048] = {};
+  for (int n = 0; n < N; ++n) {
+    const int a = tpadded(d, n);
+    if (a < 0 || a >= d.size || seen[a]) return false;
+    seen[a] = true;
+  }
+  if (LS > 1 && d.stride < d.size) return false;
+  const int ks[2] = {k1, k2};
+  for (int ki = 0; ki < 2; ++ki)
+    for (int l = 0; l < N / P; ++l)
+      for (int g = 0; g < 4; ++g) {
+        unsigned used = 0;
+        for (int lane = 16 * g; lane < 16 * g + 16; ++lane) {
+          const int t = lane / LS, c = lane % LS;
+          const int bank = (c * d.stride + tpadded(d, tpos(l, t, ks[ki], lg))) & 15;
+          if (used & (1u << bank)) return false;
+          used |= 1u << bank;
+        }
+      }
+  return true;
+}
+
+template <int N, int P, int LS>
+constexpr bool tplan_ok() {
+  using T = TPlan<N, P, LS>;
+  if (!T::ENABLED) return true;
+  for (int j = 0; j < T::NK; ++j)
+    if (!tpad_ok(N, P, LS, T::kseq(j), T::kseq(j + 1 < T::NK ? j + 1 : 0))) return false;
+  return true;
+}
+
+// Lane part of layout k's element index: n(l, t; k) = tpart + (l << k).
+template <int LG>
+__device__ __forceinline__ int tpart(int t, int k) {
+  return ((t >> k) << (k + LG)) | (t & ((1 << k) - 1));
+}
+
+// Rewrite the lane's L values from layout K1 to layout K2 through the
+// wave-private scratch `sx` (line c of the wave at c * stride).  LDS
+// instructions of one wave are performed in order, so the reads see the
+// other lanes' writes; the compiler barriers keep the accesses in program
+// order.
+template <int N, int P, int LS, int K1, int K2>
+__device__ __forceinline__ void ttranspose(float2 (&v)[N / P], float2* __restrict__ sx, int t, int c) {
+  using T = TPlan<N, P, LS>;
+  constexpr TPad d = tpad(N, P, LS, K1, K2);
+  static_assert(d.s >= 0, "no transpose padding for this layout pair");
+  float2* base = sx + c * d.stride;
+  const int t1 = tpart<T::LG>(t, K1), t2 = tpart<T::LG>(t, K2);
+  float2* w = base + tpadded(d, t1);
+  const float2* r = base + tpadded(d, t2);
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) w[tpadded(d, l << K1)] = v[l];
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int l = 0; l < T::L; ++l) v[l] = r[tpadded(d, l << K2)];
+  __asm__ volatile("" ::: "memory");
+}
+
+// Forward (DIF) stages of bits hi-1 ... lo inside layout K.
+template <int N, int P, int LS, int K>
+__device__ __forceinline__ void tstages_fwd(float2 (&v)[N / P], const float2* __restrict__ tw, int t, int hi, int lo) {
+  using T = TPlan<N, P, LS>;
+  const int tlo = t & ((1 << K) - 1);
+#pragma unroll
+  for (int b = K + T::LG - 1; b >= K; --b) {
+    if (b >= hi || b < lo) continue;
+    const int sh = T::M - 1 - b;
+    const float2* __restrict__ twb = tw + (tlo << sh);
+    const int d = 1 << (b - K);
+#pragma unroll
+    for (int l = 0; l < T::L; ++l) {
+      if ((l & d) == 0) {
+        const float2 w = twb[((l & (d - 1)) << K) << sh];
+        const float2 a = v[l], q = v[l + d];
+        v[l] = cadd(a, q);
+        v[l + d] = cmul_f(csub(a, q), w);
+      }
+    }
+  }
+}
+
+// Inverse (DIT) stages of bits lo ... hi-1 inside layout K.
+template <int N, int P, int LS, int K>
+__device__ __forceinline__ void tstages_inv(float2 (&v)[N / P], const float2* __restrict__ tw, int t, int hi, int lo) {
+  using T = TPlan<N, P, LS>;
+  const int tlo = t & ((1 << K) - 1);
+#pragma unroll
+  for (int b = K; b < K + T::LG; ++b) {
+    if (b >= hi || b < lo) continue;
+    const int sh = T::M - 1 - b;
+    const float2* __restrict__ twb = tw + (tlo << sh);
+    const int d = 1 << (b - K);
+#pragma unroll
+    for (int l = 0; l < T::L; ++l) {
+      if ((l & d) == 0) {
+        const float2 w = conjf2(twb[((l & (d - 1)) << K) << sh]);
+        const float2 a = v[l];
+        const float2 q = cmul_f(v[l + d], w);
+        v[l] = cadd(a, q);
+        v[l + d] = csub(a, q);
+      }
+    }
+  }
+}
+
+template <int N, int P, int LS, int J>
+__device__ __forceinline__ void tfwd_from(float2 (&v)[N / P], const float2* __restrict__ tw, float2* __restrict__ sx,
+                                          int t, int c) {
+  using T = TPlan<N, P, LS>;
+  constexpr int K = T::kseq(J);
+  constexpr int HI = J == 0 ? T::M : T::kseq(J - 1);
+  tstages_fwd<N, P, LS, K>(v, tw, t, HI, K);
+  if constexpr (J + 1 < T::NK) {
+    ttranspose<N, P, LS, K, T::kseq(J + 1)>(v, sx, t, c);
+    tfwd_from<N, P, LS, J + 1>(v, tw, sx, t, c);
+  } else {
+    ttranspose<N, P, LS, K, T::K0>(v, sx, t, c);
+  }
+}
+
+template <int N, int P, int LS, int J>
+__device__ __forceinline__ void tinv_from(float2 (&v)[N / P], const float2* __restrict__ tw, float2* __restrict__ sx,
+                                          int t, int c) {
+  // layouts in reverse order: J = NK - 1 first (lowest bits), J = 0 last
+  using T = TPlan<N, P, LS>;
+  constexpr int K = T::kseq(J);
+  constexpr int HI = J == 0 ? T::M : T::kseq(J - 1);
+  tstages_inv<N, P, LS, K>(v, tw, t, HI, K);
+  if constexpr (J > 0) {
+    ttranspose<N, P, LS, K, T::kseq(J - 1)>(v, sx, t, c);
+    tinv_from<N, P, LS, J - 1>(v, tw, sx, t, c);
+  }
+}
+
+// fft_fwd / fft_inv for a line on P lanes (lane stride LS): the transposed
+// form where it is enabled (L >= 4 values per lane; `sx` then points to the
+// calling wave's TPlan::SCRATCH elements of LDS, `c` = lane % LS), the
+// cross-lane form otherwise.
+template <int N, int LPF, int LS>
+__device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t,
+                                          float2* __restrict__ sx, int c) {
+  if constexpr (TPlan<N, LPF, LS>::ENABLED) {
+    static_assert(tplan_ok<N, LPF, LS>(), "transpose paddings are not conflict-free");
+    unsigned off = 0;
+    __asm__ volatile("" : "+s"(off));
+    tfwd_from<N, LPF, LS, 0>(v, tw_ + off, sx, t, c);
+  } else {
+    fft_fwd_x<N, LPF, LS>(v, tw_, t);
+  }
+}
+template <int N, int LPF, int LS>
+__device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t,
+                                          float2* __restrict__ sx, int c) {
+  if constexpr (TPlan<N, LPF, LS>::ENABLED) {
+    static_assert(tplan_ok<N, LPF, LS>(), "transpose paddings are not conflict-free");
+    using T = TPlan<N, LPF, LS>;
+    unsigned off = 0;
+    __asm__ volatile("" : "+s"(off));
+    // the caller's layout is K0; the inverse starts in the last layout
+    ttranspose<N, LPF, LS, T::K0, T::kseq(T::NK - 1)>(v, sx, t, c);
+    tinv_from<N, LPF, LS, T::NK - 1>(v, tw_ + off, sx, t, c);
+  } else {
+    fft_inv_x<N, LPF, LS>(v, tw_, t);
+  }
+}
+
+// per-wave scratch (float2 elements) the transforms of a probe size need:
+// rows (LPF lanes) and column groups of CW lines
+template <int N, int CW>
+constexpr int fft_scratch() {
+  constexpr int LPF = N >= 64 ? 64 : N;
+  constexpr int cw = CW * N >= 64 ? CW : 64 / N;
+  constexpr int a = TPlan<N, LPF, 1>::SCRATCH;
+  constexpr int b = TPlan<N, 64 / cw, cw>::SCRATCH;
+  constexpr int m = a > b ? a : b;
+  return m > 0 ? m : 1;
+}
+
 // row layout: a line on LPF adjacent lanes
 template <int N>
-__device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
-  fft_fwd_g<N, Line<N>::LPF, 1>(v, tw, t);
+__device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t, float2* sx) {
+  fft_fwd_g<N, Line<N>::LPF, 1>(v, tw, t, sx, 0);
 }
 template <int N>
-__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t) {
-  fft_inv_g<N, Line<N>::LPF, 1>(v, tw, t);
+__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t, float2* sx) {
+  fft_inv_g<N, Line<N>::LPF, 1>(v, tw, t, sx, 0);
 }
 
 __device__ __forceinline__ int brev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }

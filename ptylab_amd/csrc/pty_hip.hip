@@ -175,6 +175,7 @@ struct Run {
   double* red;
   float2* xa;  // LDS exchange buffers, one line-field slot per item
   float2* xb;
+  float2* fx;  // this wave's LDS scratch of the transposed line transforms
   int* fail;
   bool loc;  // every workgroup of the cluster runs on one XCD: exchange through its L2
   bool fl;   // field-loop row passes (DevInst::fl, and at least one line per item slot)
@@ -616,7 +617,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
       v[l] = e;
     }
-    fft_fwd<NP>(v, R.tw, R.t);
+    fft_fwd<NP>(v, R.tw, R.t, R.fx);
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
@@ -703,13 +704,18 @@ __device__ __forceinline__ void store_colg(const Run& R, float2* f, const ColVal
   for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(R, f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p, v[l]);
 }
 template <int NP, int CWR>
-__device__ __forceinline__ void fft_fwd_colg(ColVals<NP, CWR>& v, const float2* tw, int rho) {
-  fft_fwd_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, tw, rho);
+__device__ __forceinline__ void fft_fwd_colg(const Run& R, ColVals<NP, CWR>& v, int rho) {
+  fft_fwd_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, R.tw, rho, R.fx, R.lane % Col<NP, CWR>::CW);
 }
 template <int NP, int CWR>
-__device__ __forceinline__ void fft_inv_colg(ColVals<NP, CWR>& v, const float2* tw, int rho) {
-  fft_inv_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, tw, rho);
+__device__ __forceinline__ void fft_inv_colg(const Run& R, ColVals<NP, CWR>& v, int rho) {
+  fft_inv_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, R.tw, rho, R.fx, R.lane % Col<NP, CWR>::CW);
 }
+
+// LDS scratch per wave of the line transforms of probe size NP (rows and
+// column groups of the kernel's column width)
+template <int NP, int CWR>
+constexpr int wave_scratch() { return fft_scratch<NP, Col<NP, CWR>::CW>(); }
 
 // Fraunhofer detector column pass: forward column transform, unitary scale,
 // intensity constraint with the measured frame, inverse column transform.
@@ -730,7 +736,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     float2 v[CL];
     load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
-    fft_fwd_colg<NP, CWR>(v, R.tw, rho);
+    fft_fwd_colg<NP, CWR>(R, v, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
       v[l] = cscale(v[l], invN);
@@ -750,7 +756,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
       if (ITEM_LEAD(act, frw)) err += (double)fabsf(im - iest[l]);
       v[l] = cscale(v[l], sqrtf(im / (iest[l] + 1e-10f)));
     }
-    fft_inv_colg<NP, CWR>(v, R.tw, rho);
+    fft_inv_colg<NP, CWR>(R, v, rho);
     if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
     CITEM_ROUND_SYNC();
   CITEMS_END
@@ -799,14 +805,14 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
     if (D.live && __builtin_amdgcn_ballot_w64(D.live[(f / D.fpl) * NP + p] != 0) == 0) continue;
     float2 v[CL];
     load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
-    fft_fwd_colg<NP, CWR>(v, R.tw, rho);
+    fft_fwd_colg<NP, CWR>(R, v, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
       float2 h = ldg2(HT + (size_t)(l * LPC + rho) * NP + p);
       if (conj) h = conjf2(h);
       v[l] = cmul(v[l], h);
     }
-    fft_inv_colg<NP, CWR>(v, R.tw, rho);
+    fft_inv_colg<NP, CWR>(R, v, rho);
     if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
   CITEMS_END
 }
@@ -831,7 +837,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     zero_dead_columns<NP>(lb, v);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
-    fft_inv<NP>(v, R.tw, R.t);
+    fft_inv<NP>(v, R.tw, R.t, R.fx);
     float2 q[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) q[l] = make_float2(act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f, 0.f);
@@ -849,7 +855,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
       if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest[l]);
       v[l] = cscale(v[l], sqrtf(im[l] / (iest[l] + 1e-10f)));
     }
-    fft_fwd<NP>(v, R.tw, R.t);
+    fft_fwd<NP>(v, R.tw, R.t, R.fx);
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     ITEM_ROUND_SYNC();
   ITEMS_END
@@ -902,7 +908,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
     load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
     if constexpr (ASPF) zero_dead_columns<NP>(lb, d);
-    fft_inv<NP>(d, R.tw, R.t);
+    fft_inv<NP>(d, R.tw, R.t, R.fx);
     float2 tO[L], tP[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -1033,7 +1039,7 @@ __device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot,
         if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
         v[l] = e;
       }
-      fft_fwd<NP>(v, R.tw, R.t);
+      fft_fwd<NP>(v, R.tw, R.t, R.fx);
       if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     }
     if (act) {
@@ -1092,7 +1098,7 @@ __device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned
       load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
       load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
       if constexpr (ASPF) zero_dead_columns<NP>(lb, d);
-      fft_inv<NP>(d, R.tw, R.t);
+      fft_inv<NP>(d, R.tw, R.t, R.fx);
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 u = d[l];
@@ -1405,7 +1411,7 @@ __device__ __forceinline__ void e3_rows_first(Run& R, int k, unsigned* slot, uns
 #pragma unroll
     for (int l = 0; l < L; ++l)
       if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
-    fft_fwd<NP>(v, R.tw, R.t);
+    fft_fwd<NP>(v, R.tw, R.t, R.fx);
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
@@ -1467,7 +1473,7 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
     float2 v[L], os[L], q[L];
     load_row<NP>(R, v, W + f * Np2 + (size_t)r * NP, R.t);
     load_row<NP>(R, os, O + (size_t)((f / npm) * ns + s) * No2 + orow_off, R.t);
-    fft_inv<NP>(v, R.tw, R.t);
+    fft_inv<NP>(v, R.tw, R.t, R.fx);
     if (act) store_row<NP>(R, P + (size_t)(f * ns + s) * Np2 + (size_t)r * NP, v, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -1479,7 +1485,7 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
 #pragma unroll
     for (int l = 0; l < L; ++l)
       if (cb && ((r + l * LPF + R.t) & 1)) v[l] = make_float2(-v[l].x, -v[l].y);
-    fft_fwd<NP>(v, R.tw, R.t);
+    fft_fwd<NP>(v, R.tw, R.t, R.fx);
     if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
@@ -1532,7 +1538,7 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
     load_row<NP>(R, pv, P + (size_t)(f * ns + su) * Np2 + prow_off, R.t);
     load_row<NP>(R, ev, E + (size_t)(esl * nF + f) * Np2 + prow_off, R.t);
     load_row<NP>(R, ov, O + (size_t)((f / npm) * ns + su) * No2 + orow_off, R.t);
-    fft_inv<NP>(d, R.tw, R.t);
+    fft_inv<NP>(d, R.tw, R.t, R.fx);
     float2 tO[L], tP[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -1571,7 +1577,7 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
     }
     if (act) store_row<NP>(R, P + (size_t)(f * ns + su) * Np2 + prow_off, nv, R.t);
     if (su >= 1) {
-      fft_fwd<NP>(nv, R.tw, R.t);
+      fft_fwd<NP>(nv, R.tw, R.t, R.fx);
       if (act) store_row<NP>(R, W + f * Np2 + prow_off, nv, R.t);
     }
     __syncthreads();
@@ -1746,9 +1752,9 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
 
 // Per-workgroup set-up for probe size NP on a workgroup of NT threads:
 // lane roles, LDS views, the twiddles of length NP.
-template <int NP, int NT>
+template <int NP, int NT, int CWR>
 __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2* s_tw, double* s_red, float2* s_x,
-                                         int* s_fail) {
+                                         float2* s_fx, int* s_fail) {
   R.D = D;
   R.g = g;
   R.G = D->G;
@@ -1766,6 +1772,7 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   R.ipr = (NT / 64) * Line<NP>::SUB;
   R.xa = s_x;
   R.xb = s_x + R.ipr * NP;
+  R.fx = s_fx + R.wave * wave_scratch<NP, CWR>();
   R.fail = s_fail;
   R.loc = false;
   R.fl = false;
@@ -1783,11 +1790,12 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   __shared__ float2 s_tw[256];
   __shared__ double s_red[Cfg<NP>::NW];
   __shared__ float2 s_x[Cfg<NP>::XBUF];
+  __shared__ float2 s_fx[Cfg<NP>::NW * wave_scratch<NP, 0>()];
   __shared__ int s_fail;
   const int2 bm = bmap[blockIdx.x];
   if (bm.x < 0) return;
   Run R;
-  init_run<NP, Cfg<NP>::NT>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, &s_fail);
+  init_run<NP, Cfg<NP>::NT, 0>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
   run_iterations<NP, FAM, 0>(R, it0, it1, jb, je);
 }
 
@@ -1812,6 +1820,9 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int BATCH_XBUF =
     cmax(cmax(cmax(BatchCfg<16>::XBUF, BatchCfg<32>::XBUF), cmax(BatchCfg<64>::XBUF, BatchCfg<128>::XBUF)),
          BatchCfg<256>::XBUF);
+constexpr int BATCH_FX = cmax(cmax(cmax(wave_scratch<16, BATCH_CW>(), wave_scratch<32, BATCH_CW>()),
+                                   cmax(wave_scratch<64, BATCH_CW>(), wave_scratch<128, BATCH_CW>())),
+                              wave_scratch<256, BATCH_CW>());
 
 template <int FAM>
 __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) void pty_batch_kernel(
@@ -1820,6 +1831,7 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ float2 s_tw[256];
   __shared__ double s_red[BATCH_NT / 64];
   __shared__ float2 s_x[BATCH_XBUF];
+  __shared__ float2 s_fx[(BATCH_NT / 64) * BATCH_FX];
   __shared__ int s_fail;
   __shared__ int s_blk;
   // The plan packs every cluster onto one XCD: bmap entry N_XCD * s + x is
@@ -1851,31 +1863,31 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   Run R;
   switch (D->a.Np) {
     case 16:
-      init_run<16, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      init_run<16, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
       R.loc = true;
       R.fl = D->fl && 16 / R.G >= R.ipr;
       run_iterations<16, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 32:
-      init_run<32, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      init_run<32, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
       R.loc = true;
       R.fl = D->fl && 32 / R.G >= R.ipr;
       run_iterations<32, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 64:
-      init_run<64, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      init_run<64, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
       R.loc = true;
       R.fl = D->fl && 64 / R.G >= R.ipr;
       run_iterations<64, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 128:
-      init_run<128, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      init_run<128, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
       R.loc = true;
       R.fl = D->fl && 128 / R.G >= R.ipr;
       run_iterations<128, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 256:
-      init_run<256, BATCH_NT>(R, D, bm.y, s_tw, s_red, s_x, &s_fail);
+      init_run<256, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
       R.loc = true;
       R.fl = D->fl && 256 / R.G >= R.ipr;
       run_iterations<256, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
@@ -1956,6 +1968,7 @@ template <int NP>
 __global__ __launch_bounds__(NTHREADS) void pty_lines_kernel(float2* __restrict__ data, int nfields, int mode) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB, LG = Line<NP>::LOG2;
   __shared__ float2 s_tw[256];
+  __shared__ float2 s_fx[NWAVE * fft_scratch<NP, 1>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane & (LPF - 1), sub = lane / LPF;
   const bool inv = mode & LN_INVERSE;
@@ -1978,7 +1991,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_lines_kernel(float2* __restrict_
     if ((mode & LN_PRE_CB) && ((n + li) & 1)) x = make_float2(-x.x, -x.y);
     v[l] = x;
   }
-  fft_fwd<NP>(v, s_tw, t);
+  fft_fwd<NP>(v, s_tw, t, s_fx + wave * fft_scratch<NP, 1>());
   const float sc = 1.0f / (float)NP;
 #pragma unroll
   for (int l = 0; l < L; ++l) {
@@ -2000,6 +2013,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_asp_cols_kernel(float2* __restri
                                                                 const float2* __restrict__ tf, int conj) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB, LG = Line<NP>::LOG2;
   __shared__ float2 s_tw[256];
+  __shared__ float2 s_fx[NWAVE * fft_scratch<NP, 1>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane & (LPF - 1), sub = lane / LPF;
   for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[LG - 4][i];
@@ -2010,7 +2024,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_asp_cols_kernel(float2* __restri
   float2* base = data + (size_t)f * NP * NP;
   float2 v[L];
   load_col<NP>(v, base, p, t);
-  fft_fwd<NP>(v, s_tw, t);
+  fft_fwd<NP>(v, s_tw, t, s_fx + wave * fft_scratch<NP, 1>());
   const float sc = 1.0f / ((float)NP * (float)NP);
 #pragma unroll
   for (int l = 0; l < L; ++l) {
@@ -2018,7 +2032,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_asp_cols_kernel(float2* __restri
     if (conj) h = conjf2(h);
     v[l] = cmul(v[l], cscale(h, sc));
   }
-  fft_inv<NP>(v, s_tw, t);
+  fft_inv<NP>(v, s_tw, t, s_fx + wave * fft_scratch<NP, 1>());
   store_col<NP>(base, v, p, t);
 }
 
@@ -2026,6 +2040,7 @@ template <int NP>
 __global__ __launch_bounds__(NTHREADS) void pty_rows_fwd_kernel(float2* __restrict__ data, int nfields) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
   __shared__ float2 s_tw[256];
+  __shared__ float2 s_fx[NWAVE * fft_scratch<NP, 1>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane & (LPF - 1), sub = lane / LPF;
   for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
@@ -2035,7 +2050,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_rows_fwd_kernel(float2* __restri
   float2* row = data + (size_t)line * NP;
   float2 v[L];
   load_row<NP>(v, row, t);
-  fft_fwd<NP>(v, s_tw, t);
+  fft_fwd<NP>(v, s_tw, t, s_fx + wave * fft_scratch<NP, 1>());
   store_row<NP>(row, v, t);
 }
 
@@ -2043,6 +2058,7 @@ template <int NP>
 __global__ __launch_bounds__(NTHREADS) void pty_rows_inv_kernel(float2* __restrict__ data, int nfields) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
   __shared__ float2 s_tw[256];
+  __shared__ float2 s_fx[NWAVE * fft_scratch<NP, 1>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane & (LPF - 1), sub = lane / LPF;
   for (int i = threadIdx.x; i < NP / 2; i += NTHREADS) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
@@ -2052,7 +2068,7 @@ __global__ __launch_bounds__(NTHREADS) void pty_rows_inv_kernel(float2* __restri
   float2* row = data + (size_t)line * NP;
   float2 v[L];
   load_row<NP>(v, row, t);
-  fft_inv<NP>(v, s_tw, t);
+  fft_inv<NP>(v, s_tw, t, s_fx + wave * fft_scratch<NP, 1>());
   store_row<NP>(row, v, t);
 }
 
