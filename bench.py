@@ -429,7 +429,7 @@ def main_batched(args):
     staged = [e._stage() for e in engines]
     insts = [s_[0] for s_ in staged]
     for inst in insts:
-        inst.flags = P._native.PTY_FLAG_BATCHED
+        inst.flags |= P._native.PTY_FLAG_BATCHED
     t_tune = time.perf_counter()
     G = autotune.tune(insts, tensors=[s_[1] for s_ in staged])
     print(f"[bench] autotuned cluster sizes in {time.perf_counter() - t_tune:.1f} s: total {sum(G)} workgroups",

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PTY_ABI_VERSION 3
+#define PTY_ABI_VERSION 4
 
 /* error codes */
 #define PTY_OK 0
@@ -120,12 +120,26 @@ typedef struct pty_instance {
    * position.  Instances with pc_direction never join the batched launch. */
   double* pc_direction;
   int32_t pc_start;
+  /* params.positionCorrectionSwitch_radius: below 2 the direct correlation
+   * above; from 2 on the Fourier branch (BaseEngine.py:1278-1313): the
+   * cross-correlation of the mean-free patches over shifts -r..r in both
+   * axes, betaGrad 5, clipped at 10 px. */
+  int32_t pc_radius;
+  /* Optional (F, Np, Np) float32 output, natural detector order: the last
+   * iteration's |I_k - Iest| of every position k, the reference's
+   * reconstruction.detectorError (getRMSD, BaseEngine.py:851-868; the
+   * reference rewrites it at every iteration, so after a run it holds the
+   * last iteration's values).  NULL: not written. */
+  float* detector_error;
 } pty_instance;
 
 /* Run the instance in the batched launch (the one kernel that serves every
  * Np <= 256 of a pass family) even when it is alone in its plan.  In a plan
  * of several instances every instance with Np <= 256 runs there anyway. */
 #define PTY_FLAG_BATCHED 1
+#define PTY_FLAG_MQN_NADAM 2    /* PTY_MQNEWTON with momentum_method "NADAM" (mqNewton.py:155-174) */
+#define PTY_FLAG_MQN_MOMENTUM 4 /* PTY_MQNEWTON with momentum_method "momentum" (mqNewton.py:176-182):
+                                   momFriction carries frictionM, momFeedback is unused */
 
 typedef struct pty_plan pty_plan;
 

@@ -399,6 +399,7 @@ class OracleRun:
     betaProbe: Optional[float] = None
     regObject: float = 1.0   # qNewton regularisers (qNewton.py:51-52)
     regProbe: float = 1.0
+    momentum_method: str = "ADAM"  # mqNewton: "ADAM", "NADAM" or "momentum" (mqNewton.py:65)
     beta1: float = 0.5        # mqNewton ADAM coefficients (mqNewton.py:57-60)
     beta2: float = 0.5
     betaObject_m: float = 0.25
@@ -417,6 +418,7 @@ class OracleRun:
     absorbingProbeBoundary: bool = False
     absorbingProbeBoundaryAleph: float = 5e-2
     positionCorrectionSwitch: bool = False
+    saveMemory: bool = False
     precision: str = "c64"
     error: List[float] = field(default_factory=list)
 
@@ -442,6 +444,13 @@ class OracleRun:
         self.energy = energy_at_pos(self.ptychogram)
         self.err_at_pos_sum = np.zeros(self.ptychogram.shape[0])
         self.errorAtPos = None
+        # _initializeErrors (BaseEngine.py:199-225)
+        F, Nd = self.ptychogram.shape[0], self.ptychogram.shape[-1]
+        if self.saveMemory:
+            self.detectorError = 0
+            self.errorAtPos = np.zeros((F, 1), dtype=np.float32)
+        else:
+            self.detectorError = np.zeros((F, Nd, Nd))
         self._tf = {}
         self.schedule = []  # (order, momentum triggers) per iteration, for inspection
         # Reconstruction.encoder_corrected and the positions derived from it
@@ -450,11 +459,11 @@ class OracleRun:
         self.positions = g.positions.copy()
         self.D = np.zeros((self.ptychogram.shape[0], 2))  # _initializePCParameters (BaseEngine.py:178-197)
         if self.engine in ("mPIE", "multiPIE", "mqNewton", "pcPIE"):
-            # mPIE.py:87-94 / multiPIE.py:61-66 / mqNewton.py:42-46 / pcPIE.py:44-51
+            # mPIE.py:87-94 / multiPIE.py:61-66 / mqNewton.py:43-44 / pcPIE.py:44-51
             self.objectMomentum = np.zeros_like(self.object)
             self.probeMomentum = np.zeros_like(self.probe)
         if self.engine in ("mqNewton", "pcPIE"):
-            # buffers at construction (mqNewton.py:45-47, pcPIE.py:49-51)
+            # buffers at construction (mqNewton.py:46-47, pcPIE.py:49-51)
             self.objectBuffer = self.object.copy()
             self.probeBuffer = self.probe.copy()
         if self.engine == "e3PIE":
@@ -479,13 +488,23 @@ class OracleRun:
         Imeas = self.ptychogram[k]
         det_err = abs(Imeas - Iest)
         self.err_at_pos_sum[k] = np.sum(np.abs(det_err.astype(np.float64)))
+        # getRMSD (BaseEngine.py:851-868)
+        if self.saveMemory:
+            self.errorAtPos[k] = np.sum(det_err)
+        else:
+            self.detectorError[k] = det_err
         frac = np.sqrt(Imeas / (Iest + GEMMA))
         ESW = ESW * frac
         return self._bwd(ESW)
 
     def _error_metrics(self):
-        """getErrorMetrics (BaseEngine.py:819-841)."""
-        self.errorAtPos = self.err_at_pos_sum / (self.energy + 1e-20)
+        """getErrorMetrics (BaseEngine.py:819-841).  With saveMemory the
+        (numFrames, 1) errorAtPos divided by the (numFrames,) energies
+        broadcasts to a (numFrames, numFrames) array, whose sum is the error."""
+        if self.saveMemory:
+            self.errorAtPos = self.errorAtPos / (self.energy + 1e-20)
+        else:
+            self.errorAtPos = self.err_at_pos_sum / (self.energy + 1e-20)
         self.error.append(float(np.sum(self.errorAtPos)))
 
     @property
@@ -585,7 +604,7 @@ class OracleRun:
             self.objectBuffer = self.object.copy()
             self.probeBuffer = self.probe.copy()
         if self.engine == "mqNewton":
-            # initializeAdaptiveMomentum (mqNewton.py:70-84)
+            # initializeAdaptiveMomentum (mqNewton.py:68-79)
             self.objectMomentum_v = self.objectMomentum.copy()
             self.probeMomentum_v = self.probeMomentum.copy()
         F = self.ptychogram.shape[0]
@@ -709,7 +728,7 @@ class OracleRun:
         self.probeBuffer = self.probe.copy()
 
     def _adam(self, grad, mt, vt, itr):
-        """mqNewton.ADAM (mqNewton.py:160-174): the second moment takes the
+        """mqNewton.ADAM (mqNewton.py:142-153): the second moment takes the
         squared 2-norm of the whole flattened gradient."""
         beta1_scale = 1 - self.beta1 ** itr
         beta2_scale = 1 - self.beta2 ** itr
@@ -719,17 +738,41 @@ class OracleRun:
         v_hat = vt / beta2_scale
         return m_hat / (v_hat ** 0.5 + 1e-8), mt, vt
 
+    def _nadam(self, grad, mt, vt, itr):
+        """mqNewton.NADAM (mqNewton.py:155-174): ADAM's moments with the
+        Nesterov step (beta1 m_hat + (1 - beta1) g / (1 - beta1^itr)) /
+        (sqrt(v_hat) + 1e-8)."""
+        beta1_scale = 1 - self.beta1 ** itr
+        beta2_scale = 1 - self.beta2 ** itr
+        norm_sq = np.linalg.norm(grad.flatten(), 2) ** 2
+        mt = self.beta1 * mt + (1 - self.beta1) * grad
+        vt = self.beta2 * vt + (1 - self.beta2) * norm_sq
+        m_hat = mt / beta1_scale
+        v_hat = vt / beta2_scale
+        update = (self.beta1 * m_hat + grad * (1 - self.beta1) / beta1_scale) / (v_hat ** 0.5 + 1e-8)
+        return update, mt, vt
+
+    def _plain_momentum(self, grad, mt, vt, itr):
+        """mqNewton.momentum (mqNewton.py:176-182): m = g + frictionM m, the
+        step is m; the second moment is passed through untouched."""
+        mt = grad + self.frictionM * mt
+        return mt, mt, vt
+
+    def _momentum_engine(self):
+        return {"ADAM": self._adam, "NADAM": self._nadam, "momentum": self._plain_momentum}[self.momentum_method]
+
     def _adam_momentum(self, loop):
-        """objectMomentumUpdate / probeMomentumUpdate (mqNewton.py:197-235)
-        after every position, with itr = loop + 1."""
+        """objectMomentumUpdate / probeMomentumUpdate (mqNewton.py:184-224)
+        after every position, with itr = loop + 1, through the selected
+        momentum engine."""
+        step = self._momentum_engine()
         grad = self.objectBuffer - self.object
-        upd, self.objectMomentum, self.objectMomentum_v = self._adam(grad, self.objectMomentum,
-                                                                     self.objectMomentum_v, loop + 1)
+        upd, self.objectMomentum, self.objectMomentum_v = step(grad, self.objectMomentum, self.objectMomentum_v,
+                                                               loop + 1)
         self.object = self.object - self.betaObject_m * upd
         self.objectBuffer = self.object.copy()
         grad = self.probeBuffer - self.probe
-        upd, self.probeMomentum, self.probeMomentum_v = self._adam(grad, self.probeMomentum,
-                                                                   self.probeMomentum_v, loop + 1)
+        upd, self.probeMomentum, self.probeMomentum_v = step(grad, self.probeMomentum, self.probeMomentum_v, loop + 1)
         self.probe = self.probe - self.betaProbe_m * upd
         self.probeBuffer = self.probe.copy()
 

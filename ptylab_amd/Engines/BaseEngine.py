@@ -35,7 +35,7 @@ from ..Operators import (asp_tf_fftorder, aspw_transfer_function, polychrome_tf_
 # implement (BaseEngine.py:862, :1491)
 _UNSUPPORTED_TRUE = [
     "FourierMaskSwitch", "CPSCswitch", "backgroundModeSwitch", "adaptiveDenoisingSwitch",
-    "objectTVregSwitch", "OPRP", "saveMemory", "l2reg",
+    "objectTVregSwitch", "OPRP", "l2reg",
     "modulusEnforcedProbeSwitch", "probeSpectralPowerCorrectionSwitch", "PSDestimationSwitch",
     "probeSmoothenessSwitch", "objectSmoothenessSwitch", "absObjectSwitch", "absProbeSwitch",
     "objectContrastSwitch", "couplingSwitch", "binaryProbeSwitch", "TV_autofocus",
@@ -115,7 +115,14 @@ class BaseEngine:
             self.params.fftshiftFlag = 0
 
     def _initializeErrors(self):
+        """BaseEngine._initializeErrors (BaseEngine.py:199-225): with
+        saveMemory the detector error is the scalar 0, otherwise a
+        (numFrames, Nd, Nd) array (kept from an earlier run)."""
         r = self.reconstruction
+        if self.params.saveMemory:
+            r.detectorError = 0
+        elif not hasattr(r, "detectorError"):
+            r.detectorError = np.zeros((self.experimentalData.numFrames, r.Nd, r.Nd))
         if not hasattr(r, "errorAtPos"):
             r.errorAtPos = np.zeros((self.experimentalData.numFrames, 1), dtype=np.float32)
         if not hasattr(r, "error"):
@@ -238,6 +245,11 @@ class BaseEngine:
         T["order"] = torch.as_tensor(orders, device=dev)
         T["mask"] = torch.as_tensor(masks, device=dev)
         F = d.numFrames
+        if not self.params.saveMemory:
+            # reconstruction.detectorError of the last iteration (getRMSD,
+            # BaseEngine.py:851-868), written by the kernels in natural
+            # detector order
+            T["detector_error"] = torch.zeros((F, int(r.Np), int(r.Np)), dtype=torch.float32, device=dev)
         T["err_at_pos"] = torch.zeros((n_iter, F), dtype=torch.float64, device=dev)
         T["err_iter"] = torch.zeros((n_iter,), dtype=torch.float64, device=dev)
         tf = self._transfer_function()
@@ -278,10 +290,16 @@ class BaseEngine:
         inst.energy, inst.tf = ptr("energy"), ptr("tf")
         inst.err_at_pos, inst.err_iter = ptr("err_at_pos"), ptr("err_iter")
         inst.pc_direction = ptr("pc_direction")
+        inst.detector_error = ptr("detector_error")
         # positionCorrection runs once len(error) > startAtIteration = 1
         inst.pc_start = max(0, 2 - int(getattr(self, "_n_err0", 0)))
         inst.workgroups = int(self.workgroups)
+        inst.flags = self._instance_flags()
         return inst
+
+    def _instance_flags(self):
+        """PTY_FLAG_* of this engine's instance (mqNewton's momentum method)."""
+        return 0
 
     def _momentum_enabled(self):
         return False
@@ -320,10 +338,14 @@ class BaseEngine:
         that ran this engine has been checked."""
         T = self._T
         r = self.reconstruction
-        errs = T["err_iter"][:n_done].cpu().numpy()
-        r.error = np.append(np.asarray(r.error, dtype=np.float64), errs)
-        if n_done:
-            r.errorAtPos = T["err_at_pos"][n_done - 1].cpu().numpy()
+        if self.params.saveMemory:
+            self._collect_save_memory_errors(n_done)
+        else:
+            errs = T["err_iter"][:n_done].cpu().numpy()
+            r.error = np.append(np.asarray(r.error, dtype=np.float64), errs)
+            if n_done:
+                r.errorAtPos = T["err_at_pos"][n_done - 1].cpu().numpy()
+                r.detectorError = T["detector_error"].cpu().numpy().astype(np.float64)
         r._move_data_to_cpu()
         if "pc_direction" in T:
             self.D = T["pc_direction"].cpu().numpy()  # the search directions, as the reference's self.D
@@ -331,6 +353,25 @@ class BaseEngine:
             if k in T:
                 setattr(r, k, T[k].cpu().numpy())
         self._T = None
+
+    def _collect_save_memory_errors(self, n_done):
+        """Errors as the reference reports them with saveMemory
+        (BaseEngine.py:819-841, 856-866): errorAtPos stays the (numFrames, 1)
+        array that getRMSD fills with sum |I_k - Iest|, and dividing it by the
+        (numFrames,) energies broadcasts to a (numFrames, numFrames) array,
+        so every iteration's error is sum_k e_k * sum_j 1 / (E_j + 1e-20) and
+        errorAtPos ends as that matrix."""
+        T, r = self._T, self.reconstruction
+        if not n_done:
+            return
+        en = np.asarray(self.experimentalData.energyAtPos, dtype=np.float32) + np.float32(1e-20)
+        # the kernels store e_k / (E_k + 1e-20); undo the division in FP64
+        e = T["err_at_pos"][:n_done].cpu().numpy() * en.astype(np.float64)[None, :]
+        inv = float(np.sum(1.0 / en.astype(np.float64)))
+        r.error = np.append(np.asarray(r.error, dtype=np.float64), e.sum(axis=1) * inv)
+        last = e[-1].astype(np.float32)[:, None]
+        r.errorAtPos = last / en
+        r.detectorError = 0
 
     def _end(self, n_done):
         """Check the device flag, then errors and state back to NumPy."""
@@ -386,6 +427,13 @@ class BaseEngine:
         if pos.min() < 0 or pos.max() + r.Np > r.No:
             raise ValueError("position correction moved a patch outside the object")
         T["positions"].copy_(torch.as_tensor(np.ascontiguousarray(pos, dtype=np.int32)))
+        if self.params.positionOrder == "NA" and loop + 1 < T["order"].shape[0]:
+            # setPositionOrder "NA" (BaseEngine.py:655-663) sorts by distance
+            # from the centre of the positions at the start of every
+            # iteration: the next iteration's order follows the moved positions
+            nxt = self._position_order(self._n_err0 + loop + 1)
+            self._orders[loop + 1] = nxt
+            T["order"][loop + 1].copy_(torch.as_tensor(np.ascontiguousarray(nxt, dtype=np.int32)))
 
     def _probe_window(self):
         """_probeWindow (BaseEngine.py:237-266) in float64, cast to float32:

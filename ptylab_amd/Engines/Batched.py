@@ -46,7 +46,7 @@ class Batched:
         staged = [e._stage() for e in self.engines]
         insts = [s[0] for s in staged]
         for inst in insts:
-            inst.flags = _native.PTY_FLAG_BATCHED
+            inst.flags |= _native.PTY_FLAG_BATCHED
         if self.workgroups is not None:
             G = [int(g) for g in self.workgroups]
             if len(G) != len(insts):

@@ -1,13 +1,14 @@
-"""mqNewton (Engines/mqNewton.py): qNewton with adaptive momentum after every
-scan position.  The position step is qNewton's (object step, Omax over the
-updated object, probe step; qNewton.py:106-140 = mqNewton.py:197-235); then
-the object and the probe each take an ADAM step on the gradient buffer -
-current with the squared 2-norm of the whole gradient as the second moment
-(mqNewton.py:112-113, 160-174, 176-195).  Only the default momentum_method
-"ADAM" is implemented."""
+"""mqNewton (Engines/mqNewton.py): qNewton with momentum after every scan
+position.  The position step is qNewton's (object step, Omax over the updated
+object, probe step; qNewton.py:106-140 = mqNewton.py:226-251); then the object
+and the probe each take a step of the selected momentum engine on the
+gradient buffer - current (mqNewton.py:121-122, 184-224): "ADAM"
+(:142-153, the default) and "NADAM" (:155-174) with the squared 2-norm of the
+whole gradient as the second moment, or plain "momentum" (:176-182)."""
 
 import numpy as np
 
+from .. import _native
 from .BaseEngine import BaseEngine
 
 
@@ -19,7 +20,7 @@ class mqNewton(BaseEngine):
         super().__init__(reconstruction, experimentalData, params, monitor)
         self.initializeReconstructionParams()
         r = self.reconstruction
-        # mqNewton.py:42-48
+        # mqNewton.py:43-48
         r.initializeObjectMomentum()
         r.initializeProbeMomentum()
         r.objectBuffer = np.array(r.object, copy=True)
@@ -46,24 +47,39 @@ class mqNewton(BaseEngine):
         return True
 
     def _engine_coefficients(self):
-        # the C ABI carries the regularisers in the alpha slots and the ADAM
-        # coefficients in the momentum slots (include/pty_hip.h, PTY_MQNEWTON)
+        # the C ABI carries the regularisers in the alpha slots and the
+        # momentum coefficients in the momentum slots (include/pty_hip.h,
+        # PTY_MQNEWTON): beta1 / beta2, or frictionM for plain momentum
+        plain = self.momentum_method == "momentum"
         return dict(betaObject=self.betaObject, betaProbe=self.betaProbe,
                     alphaObject=self.regObject, alphaProbe=self.regProbe,
-                    momFriction=self.beta1, momFeedback=self.beta2,
+                    momFriction=self.frictionM if plain else self.beta1, momFeedback=self.beta2,
                     probeWeight=self.betaObject_m, sliceBeta=self.betaProbe_m)
 
+    def _instance_flags(self):
+        return {"ADAM": 0, "NADAM": _native.PTY_FLAG_MQN_NADAM,
+                "momentum": _native.PTY_FLAG_MQN_MOMENTUM}[self.momentum_method]
+
     def _prepareReconstruction(self):
-        if self.momentum_method != "ADAM":
-            raise NotImplementedError(f"mqNewton momentum_method {self.momentum_method!r} is not implemented "
-                                      "by the MI355X engines (ADAM is)")
+        if self.momentum_method not in ("ADAM", "NADAM", "momentum"):
+            # getattr(mqNewton, momentum_method) (mqNewton.py:69) fails the same way
+            raise AttributeError(f"type object 'mqNewton' has no attribute {self.momentum_method!r}")
         r = self.reconstruction
-        # initializeAdaptiveMomentum (mqNewton.py:70-84) seeds the second
-        # moments with a copy of the first; the GPU keeps them as one number
-        # each, which is exact when the first moments start at zero
-        for name in ("objectMomentum", "probeMomentum"):
-            m = getattr(r, name, None)
-            if m is not None and np.any(np.asarray(m) != 0):
-                raise NotImplementedError("mqNewton resumed with a nonzero momentum is not implemented by the MI355X "
-                                          "engines")
+        if self.momentum_method == "momentum":
+            # initializeAdaptiveMomentum (mqNewton.py:68-79) creates the second
+            # moments for ADAM and NADAM only, and the momentum updates read them
+            # (mqNewton.py:190-198), so on a reconstruction without them the
+            # reference stops with this error at the first position
+            for name in ("objectMomentum_v", "probeMomentum_v"):
+                if not hasattr(r, name):
+                    raise AttributeError(f"'Reconstruction' object has no attribute {name!r}")
+        else:
+            # initializeAdaptiveMomentum seeds the second moments with a copy of
+            # the first; the GPU keeps them as one number each, which is exact
+            # when the first moments start at zero
+            for name in ("objectMomentum", "probeMomentum"):
+                m = getattr(r, name, None)
+                if m is not None and np.any(np.asarray(m) != 0):
+                    raise NotImplementedError("mqNewton resumed with a nonzero momentum is not implemented by the "
+                                              "MI355X engines")
         super()._prepareReconstruction()

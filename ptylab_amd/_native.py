@@ -16,8 +16,10 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libptyhip.so"
 
-PTY_ABI_VERSION = 3
+PTY_ABI_VERSION = 4
 PTY_FLAG_BATCHED = 1
+PTY_FLAG_MQN_NADAM = 2
+PTY_FLAG_MQN_MOMENTUM = 4
 # error codes (include/pty_hip.h)
 PTY_OK, PTY_EINVAL, PTY_ENOMEM, PTY_EDEVICE, PTY_ETIMEDOUT = 0, -22, -12, -5, -110
 PTY_EPIE, PTY_MPIE, PTY_MULTIPIE, PTY_E3PIE, PTY_QNEWTON, PTY_MQNEWTON = 0, 1, 2, 3, 4, 5
@@ -91,6 +93,8 @@ class PtyInstance(ctypes.Structure):
         ("flags", ctypes.c_int32),
         ("pc_direction", ctypes.c_void_p),
         ("pc_start", ctypes.c_int32),
+        ("pc_radius", ctypes.c_int32),
+        ("detector_error", ctypes.c_void_p),
     ]
 
 

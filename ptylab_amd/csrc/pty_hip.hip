@@ -43,7 +43,7 @@
 
 using namespace pty;
 
-static_assert(sizeof(pty_instance) == 208, "pty_instance layout (ABI 3) changed: update the bindings");
+static_assert(sizeof(pty_instance) == 216, "pty_instance layout (ABI 4) changed: update the bindings");
 
 #define NTHREADS 256
 #define NWAVE (NTHREADS / 64)
@@ -722,7 +722,7 @@ constexpr int wave_scratch() { return fft_scratch<NP, Col<NP, CWR>::CW>(); }
 // Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
 // BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
 template <int NP, int CWR>
-__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs) {
+__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de) {
   constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -752,8 +752,14 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
     }
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
-      const float im = ldg(Ik + (size_t)(l * LPC + rho) * NP + p);
-      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im - iest[l]);
+      const int n = l * LPC + rho;
+      const float im = ldg(Ik + (size_t)n * NP + p);
+      if (ITEM_LEAD(act, frw)) {
+        err += (double)fabsf(im - iest[l]);
+        // element (n, p) is detector pixel (brev(n), brev(p)) (the frames'
+        // plan-time permutation)
+        if (de) de[(size_t)brev(n, Line<NP>::LOG2) * NP + brev(p, Line<NP>::LOG2)] = fabsf(im - iest[l]);
+      }
       v[l] = cscale(v[l], sqrtf(im / (iest[l] + 1e-10f)));
     }
     fft_inv_colg<NP, CWR>(R, v, rho);
@@ -821,7 +827,7 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
 // intensity constraint is applied in natural order, then the forward row
 // transform starts the back-propagation.
 template <int NP>
-__device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep) {
+__device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep, float* de) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -852,7 +858,10 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     }
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-      if (ITEM_LEAD(act, frw)) err += (double)fabsf(im[l] - iest[l]);
+      if (ITEM_LEAD(act, frw)) {
+        err += (double)fabsf(im[l] - iest[l]);
+        if (de) de[(size_t)r * NP + l * LPF + R.t] = fabsf(im[l] - iest[l]);
+      }
       v[l] = cscale(v[l], sqrtf(im[l] / (iest[l] + 1e-10f)));
     }
     fft_fwd<NP>(v, R.tw, R.t, R.fx);
@@ -1230,15 +1239,21 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
   }
 }
 
-// mqNewton's adaptive momentum after every position (mqNewton.py:112-113,
-// 160-174 ADAM, 195-235): gradient g = buffer - current for the whole object
-// and the whole probe, m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) |g|^2 (the
-// squared 2-norm of the whole gradient, one number), current -= beta_m *
-// (m / (1 - b1^itr)) / (sqrt(v / (1 - b2^itr)) + 1e-8), buffer = current.
-// Two passes: FP64 partials of |g|^2 per workgroup, then (after a barrier)
-// every workgroup sums them in workgroup order and updates its own chunk.
-// The instance fields carry b1 (momFriction), b2 (momFeedback) and the
-// object and probe steps beta_m (probeWeight, sliceBeta).
+// mqNewton's momentum step after every position (mqNewton.py:121-122,
+// 184-224), on the gradient g = buffer - current of the whole object and the
+// whole probe, with itr = iteration + 1:
+//   ADAM (142-153): m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) |g|^2 (the
+//     squared 2-norm of the whole gradient, one number), step =
+//     (m / (1 - b1^itr)) / (sqrt(v / (1 - b2^itr)) + 1e-8);
+//   NADAM (155-174): the same m and v, step = (b1 m / (1 - b1^itr) +
+//     g (1 - b1) / (1 - b1^itr)) / (sqrt(v / (1 - b2^itr)) + 1e-8);
+//   momentum (176-182): m = g + friction m, step = m (v untouched);
+// then current -= beta_m step, buffer = current.  ADAM and NADAM take two
+// passes: FP64 partials of |g|^2 per workgroup, then (after a barrier) every
+// workgroup sums them in workgroup order and updates its own chunk; the
+// momentum method needs only the second.  The instance fields carry b1 (or
+// the friction) in momFriction, b2 in momFeedback and the object and probe
+// steps beta_m in probeWeight and sliceBeta; the method is PTY_FLAG_MQN_*.
 __device__ __forceinline__ void adam_norm_pass(Run& R) {
   const DevInst& D = *R.D;
   for (int which = 0; which < 2; ++which) {
@@ -1261,17 +1276,25 @@ __device__ __forceinline__ void adam_norm_pass(Run& R) {
 
 __device__ __forceinline__ void adam_update_pass(Run& R, int itr, unsigned long long p) {
   const DevInst& D = *R.D;
+  const bool nadam = (D.a.flags & PTY_FLAG_MQN_NADAM) != 0;
+  const bool plain = (D.a.flags & PTY_FLAG_MQN_MOMENTUM) != 0;
   const float b1 = D.a.momFriction, b2 = D.a.momFeedback;
   const float omb1 = (float)(1.0 - (double)b1), omb2 = (float)(1.0 - (double)b2);
   const float s1 = (float)(1.0 - pow((double)b1, (double)itr));
   const float s2 = (float)(1.0 - pow((double)b2, (double)itr));
   for (int which = 0; which < 2; ++which) {
-    double sq = 0.0;
-    for (int g = 0; g < R.G; ++g) sq += cloadd(R, D.adampart + (size_t)which * R.G + g);
-    const float nrm = sqrtf((float)sq);
-    const float vt = b2 * (float)cloadd(R, D.adamv + (p & 1ull) * 2 + which) + omb2 * (nrm * nrm);
-    if (R.g == 0 && threadIdx.x == 0) cstored(R, D.adamv + ((p + 1ull) & 1ull) * 2 + which, (double)vt);
-    const float den = sqrtf(vt / s2) + 1e-8f;
+    float den = 1.f;
+    if (!plain) {
+      double sq = 0.0;
+      for (int g = 0; g < R.G; ++g) sq += cloadd(R, D.adampart + (size_t)which * R.G + g);
+      const float nrm = sqrtf((float)sq);
+      const float vt = b2 * (float)cloadd(R, D.adamv + (p & 1ull) * 2 + which) + omb2 * (nrm * nrm);
+      if (R.g == 0 && threadIdx.x == 0) cstored(R, D.adamv + ((p + 1ull) & 1ull) * 2 + which, (double)vt);
+      den = sqrtf(vt / s2) + 1e-8f;
+    } else if (R.g == 0 && threadIdx.x == 0) {
+      // the second moment passes through unchanged
+      cstored(R, D.adamv + ((p + 1ull) & 1ull) * 2 + which, cloadd(R, D.adamv + (p & 1ull) * 2 + which));
+    }
     const float bm = which == 0 ? D.a.probeWeight : D.a.sliceBeta;
     const size_t n = which == 0 ? (size_t)D.nO * D.a.No * D.a.No : (size_t)D.nPr * D.a.Np * D.a.Np;
     float2* X = (float2*)(which == 0 ? D.a.object : D.a.probe);
@@ -1283,8 +1306,20 @@ __device__ __forceinline__ void adam_update_pass(Run& R, int itr, unsigned long 
     for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
       const float2 x = cload(R, X + i), bb = cload(R, B + i), m = cload(R, M + i);
       const float2 gr = make_float2(bb.x - x.x, bb.y - x.y);
-      const float2 mn = make_float2(b1 * m.x + omb1 * gr.x, b1 * m.y + omb1 * gr.y);
-      const float2 u = make_float2((mn.x / s1) / den, (mn.y / s1) / den);
+      float2 mn, u;
+      if (plain) {
+        mn = make_float2(gr.x + b1 * m.x, gr.y + b1 * m.y);
+        u = mn;
+      } else {
+        mn = make_float2(b1 * m.x + omb1 * gr.x, b1 * m.y + omb1 * gr.y);
+        const float2 mh = make_float2(mn.x / s1, mn.y / s1);
+        if (nadam) {
+          const float2 gs = make_float2((gr.x * omb1) / s1, (gr.y * omb1) / s1);
+          u = make_float2((b1 * mh.x + gs.x) / den, (b1 * mh.y + gs.y) / den);
+        } else {
+          u = make_float2(mh.x / den, mh.y / den);
+        }
+      }
       const float2 xn = make_float2(x.x - bm * u.x, x.y - bm * u.y);
       cstore(R, M + i, mn);
       cstore(R, X + i, xn);
@@ -1589,7 +1624,7 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
 // ---------------------------------------------------------------------------
 template <int NP, bool ASP, int CWR, bool QN, bool QPH, bool PC, bool BAT>
 __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
-                                             bool sync_after, int it) {
+                                             bool sync_after, int it, float* de) {
   const DevInst& D = *R.D;
   // only the angular-spectrum families take the field-loop forms: with them
   // the Fraunhofer family spilled more registers and lost 6 % on one-mode
@@ -1603,12 +1638,12 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if constexpr (ASP) {
     tf_cols<NP, CWR>(R, false);
     if (!cluster_sync(R)) return false;
-    asp_rows_constraint<NP>(R, k, j, ep);
+    asp_rows_constraint<NP>(R, k, j, ep, de);
     if (!cluster_sync(R)) return false;
     tf_cols<NP, CWR>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
-    detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs);
+    detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs, de);
     if (!cluster_sync(R)) return false;
   }
   if (FLK && R.fl)
@@ -1629,8 +1664,10 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     qn_probe_rows<NP>(R, slot);
     if (!cluster_sync(R)) return false;
     if (D.a.engine == PTY_MQNEWTON) {
-      adam_norm_pass(R);
-      if (!cluster_sync(R)) return false;
+      if (!(D.a.flags & PTY_FLAG_MQN_MOMENTUM)) {
+        adam_norm_pass(R);
+        if (!cluster_sync(R)) return false;
+      }
       adam_update_pass(R, it + 1, (unsigned long long)it * D.a.F + j);
       if (!cluster_sync(R)) return false;
     }
@@ -1639,7 +1676,8 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
 }
 
 template <int NP, int CWR>
-__device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep) {
+__device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
+                                               float* de) {
   const DevInst& D = *R.D;
   const int ns = D.a.nslice;
   e3_rows_first<NP>(R, k, slot, zslot);
@@ -1650,7 +1688,7 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
     e3_rows_forward<NP>(R, k, s, slot);
     if (!cluster_sync(R)) return false;
   }
-  detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, 1);
+  detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, 1, de);
   if (!cluster_sync(R)) return false;
   e3_rows_update<NP>(R, k, ns - 1, slot, true);
   if (!cluster_sync(R)) return false;
@@ -1718,13 +1756,17 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
           sync_after = !(dr >= NP || -dr >= NP || dc >= NP || -dc >= NP);
         }
       }
+      // the detector error of the instance's last iteration, when asked for
+      float* de = D.a.detector_error && it == D.a.n_iter - 1
+                      ? D.a.detector_error + (size_t)k * NP * NP
+                      : nullptr;
       bool ok;
       if constexpr (BF == FAM_E3PIE)
-        ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep);
+        ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep, de);
       else
         ok = pie_position<NP, BF == FAM_PIE_ASP || BF == FAM_QN_ASP || BF == FAM_PIE_SASP, CWR,
                           BF == FAM_QN_FRAUNHOFER || BF == FAM_QN_ASP, BF == FAM_PIE_SASP, PC, BAT>(
-            R, k, j, slot, zslot, ep, sync_after, it);
+            R, k, j, slot, zslot, ep, sync_after, it, de);
       if (!ok) return;
       if (mom) {
         momentum_pass<NP>(R);
@@ -2231,10 +2273,17 @@ int auto_workgroups(const pty_instance& a, int ninst, int nt) {
 }
 
 int validate(const pty_instance& a, int idx, int nt) {
-  if (a.flags & ~PTY_FLAG_BATCHED) return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
+  if (a.flags & ~(PTY_FLAG_BATCHED | PTY_FLAG_MQN_NADAM | PTY_FLAG_MQN_MOMENTUM))
+    return set_err(PTY_EINVAL, "instance %d: unknown flags 0x%x", idx, a.flags);
+  if ((a.flags & (PTY_FLAG_MQN_NADAM | PTY_FLAG_MQN_MOMENTUM)) && a.engine != PTY_MQNEWTON)
+    return set_err(PTY_EINVAL, "instance %d: PTY_FLAG_MQN_* needs engine PTY_MQNEWTON", idx);
+  if ((a.flags & PTY_FLAG_MQN_NADAM) && (a.flags & PTY_FLAG_MQN_MOMENTUM))
+    return set_err(PTY_EINVAL, "instance %d: PTY_FLAG_MQN_NADAM and PTY_FLAG_MQN_MOMENTUM exclude each other", idx);
   if (a.engine < PTY_EPIE || a.engine > PTY_MQNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
   if (a.pc_direction && (a.engine == PTY_E3PIE || a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON))
     return set_err(PTY_EINVAL, "instance %d: position correction runs with the ePIE, mPIE and multiPIE engines", idx);
+  if (a.pc_direction && a.pc_radius >= 2)
+    return set_err(PTY_EINVAL, "instance %d: pc_radius >= 2 (Fourier position correction) is not implemented", idx);
   if (a.pc_direction && (a.nlambda != 1 || a.nosm != 1 || a.nslice != 1))
     return set_err(PTY_EINVAL, "instance %d: position correction needs nlambda = nosm = nslice = 1", idx);
   if (a.engine == PTY_MQNEWTON && (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
@@ -2471,7 +2520,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
       if ((rc = dalloc(p, &mem, (size_t)2 * G * sizeof(double)))) goto fail;
       d.adampart = (double*)mem;
       // the second moments start at zero (a fresh engine's momentum copy,
-      // mqNewton.py:76-84) and persist across the plan's launches
+      // mqNewton.py:68-79) and persist across the plan's launches
       if ((rc = dalloc(p, &mem, 4 * sizeof(double)))) goto fail;
       PLAN_TRY(hipMemsetAsync(mem, 0, 4 * sizeof(double), st));
       d.adamv = (double*)mem;

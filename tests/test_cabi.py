@@ -35,9 +35,10 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layout_matches_header():
     from ptylab_amd import _native
 
-    # 10 int32 + 8 float + 14 pointers + 2 int32 + pointer + int32, naturally
-    # aligned (4 bytes of tail padding); the library asserts the same size
-    assert ctypes.sizeof(_native.PtyInstance) == 10 * 4 + 8 * 4 + 14 * 8 + 2 * 4 + 8 + 4 + 4 == 208
+    # 10 int32 + 8 float + 14 pointers + 2 int32 + pointer + 2 int32 + pointer,
+    # naturally aligned (no padding); the library asserts the same size
+    assert ctypes.sizeof(_native.PtyInstance) == 10 * 4 + 8 * 4 + 14 * 8 + 2 * 4 + 8 + 2 * 4 + 8 == 216
+    assert _native.PtyInstance.detector_error.offset == 208 and _native.PtyInstance.pc_radius.offset == 204
 
 
 def _declared_net():

@@ -278,3 +278,33 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_mqnewton_momentum_methods_host_side():
+    """mqNewton's momentum_method (mqNewton.py:65-79): ADAM and NADAM run; the
+    plain "momentum" engine reads second moments that only ADAM / NADAM
+    create, so on a fresh reconstruction it raises the reference's
+    AttributeError, and runs once they exist; an unknown name fails the
+    reference's getattr lookup.  Each method maps to its instance flag."""
+    d = regression_dataset()
+    data = make_data(d)
+    params = P.Params()
+    rec = P.Reconstruction(data, params)
+    np.random.seed(SEED)
+    rec.initializeObjectProbe()
+    eng = P.Engines.mqNewton(rec, data, params, P.DummyMonitor())
+    assert eng._instance_flags() == 0
+    eng.momentum_method = "NADAM"
+    eng._prepareReconstruction()
+    assert eng._instance_flags() == P._native.PTY_FLAG_MQN_NADAM
+    eng.momentum_method = "momentum"
+    assert eng._instance_flags() == P._native.PTY_FLAG_MQN_MOMENTUM
+    assert eng._engine_coefficients()["momFriction"] == eng.frictionM
+    with pytest.raises(AttributeError, match="objectMomentum_v"):
+        eng._prepareReconstruction()
+    rec.objectMomentum_v = np.zeros_like(rec.object)
+    rec.probeMomentum_v = np.zeros_like(rec.probe)
+    eng._prepareReconstruction()
+    eng.momentum_method = "Adagrad"
+    with pytest.raises(AttributeError):
+        eng._prepareReconstruction()

@@ -357,3 +357,78 @@ def test_fftshift_switch_is_the_same_reconstruction():
     np.random.seed(5)
     bad.reconstruct()
     assert _rel(bad.object, runs[0].object) > 1e-2
+
+
+def test_nadam_and_momentum_closed_forms():
+    """NADAM (mqNewton.py:155-174): from zero moments the first step is
+    (1 + beta1) g / (|g| + 1e-8), i.e. (1 + beta1) times ADAM's first step.
+    Plain momentum (mqNewton.py:176-182): m1 = g1, m2 = g2 + frictionM g1,
+    and the step is m."""
+    run, geo = _c0_run()
+    rng = np.random.default_rng(1)
+    shape = (1, 1, 1, 1, 16, 16)
+    g = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    g2 = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    z = np.zeros_like(g)
+    upd, m, v = run._nadam(g, z, z.copy(), 1)
+    gd = g.astype(np.complex128)
+    want = (1 + run.beta1) * gd / (np.linalg.norm(gd.ravel()) + 1e-8)
+    np.testing.assert_allclose(upd, want, rtol=2e-6, atol=1e-9)
+    adam, _, _ = run._adam(g, z, z.copy(), 1)
+    np.testing.assert_allclose(upd, (1 + run.beta1) * adam, rtol=2e-6, atol=1e-9)
+    u1, m1, v1 = run._plain_momentum(g, z, z.copy(), 1)
+    np.testing.assert_array_equal(u1, g)
+    np.testing.assert_array_equal(v1, z)
+    u2, m2, _ = run._plain_momentum(g2, m1, v1, 2)
+    np.testing.assert_allclose(u2, g2.astype(np.complex128) + run.frictionM * gd, rtol=1e-6, atol=1e-7)
+    np.testing.assert_array_equal(u2, m2)
+
+
+@pytest.mark.parametrize("method", ["NADAM", "momentum"])
+def test_mqnewton_methods_without_steps_are_qnewton(method):
+    """With betaObject_m = betaProbe_m = 0 every momentum engine leaves the
+    object and probe alone, so mqNewton reproduces qNewton bit for bit."""
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=64, No=256, F=30, seed=0)
+    geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+    runs = []
+    for eng, kw in (("qNewton", {}), ("mqNewton", dict(betaObject_m=0.0, betaProbe_m=0.0, momentum_method=method))):
+        np.random.seed(1)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        np.random.seed(5)
+        runs.append(O.OracleRun(geo, obj, pr, engine=eng, numIterations=2, **kw).reconstruct())
+    a, b = runs
+    np.testing.assert_array_equal(a.object, b.object)
+    np.testing.assert_array_equal(a.probe, b.probe)
+
+
+def test_detector_error_and_save_memory_errors():
+    """getRMSD / getErrorMetrics (BaseEngine.py:819-868).  Without saveMemory
+    the per-pixel detector error of the last iteration sums, position by
+    position, to the error curve's last entry (pinned by the reference's
+    golden error vectors); with saveMemory the reference's (F, 1) / (F,)
+    division broadcasts, so every iteration's error is the sum of the
+    per-position errors times the sum of the inverse energies, and the run
+    is otherwise the same."""
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=32, No=96, F=12, seed=3, step_frac=0.3)
+    geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+    runs = []
+    for sm in (False, True):
+        np.random.seed(4)
+        obj = O.initial_guess(geo.shape_O, "ones", geo)
+        pr = O.initial_guess(geo.shape_P, "circ", geo)
+        r = O.OracleRun(geo, obj, pr, engine="mPIE", numIterations=3, saveMemory=sm)
+        np.random.seed(5)
+        runs.append(r.reconstruct())
+    a, b = runs
+    E = O.energy_at_pos(w.ptychogram)
+    per_pos = np.sum(a.detectorError, axis=(-1, -2))
+    np.testing.assert_allclose(np.sum(per_pos / (E + 1e-20)), a.error[-1], rtol=1e-12)
+    assert a.detectorError.shape == (12, 32, 32) and a.detectorError.dtype == np.float64
+    np.testing.assert_array_equal(a.object, b.object)
+    assert b.detectorError == 0 and b.errorAtPos.shape == (12, 12)
+    np.testing.assert_allclose(b.error[-1], np.sum(per_pos) * np.sum(1 / (E + 1e-20)), rtol=1e-5)
