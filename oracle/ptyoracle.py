@@ -418,6 +418,7 @@ class OracleRun:
     absorbingProbeBoundary: bool = False
     absorbingProbeBoundaryAleph: float = 5e-2
     positionCorrectionSwitch: bool = False
+    positionCorrectionSwitch_radius: int = 1
     saveMemory: bool = False
     precision: str = "c64"
     error: List[float] = field(default_factory=list)
@@ -678,25 +679,41 @@ class OracleRun:
     _PC_COL = np.array([-1, 0, 1, -1, 0, 1, -1, 0, 1])
 
     def _position_correction(self, k, patch):
-        """positionCorrection (BaseEngine.py:1241-1323), direct branch
-        (radius < 2), CPM, after the update of position k."""
+        """positionCorrection (BaseEngine.py:1241-1323), CPM, after the update
+        of position k: the direct branch (radius < 2, 9 one-pixel shifts) or
+        the Fourier branch (radius >= 2, :1277-1302)."""
         if len(self.error) <= 1:  # startAtIteration = 1
             return
         Np = int(self.geo.Np)
         row, col = self.positions[k]
         sy, sx = slice(row, row + Np), slice(col, col + Np)
         O = self.object
-        cc = np.zeros((len(self._PC_ROW), 1))
-        for s in range(len(self._PC_ROW)):
-            t = np.roll(patch, self._PC_ROW[s], axis=-2)
-            sh = np.roll(t, self._PC_COL[s], axis=-1)
-            # a complex sum stored into a float array keeps its real part
-            cc[s] = np.squeeze(np.sum(sh.conj() * O[..., sy, sx], axis=(-2, -1))).real
+        radius = int(self.positionCorrectionSwitch_radius)
+        if radius < 2:
+            rowShifts, colShifts = self._PC_ROW, self._PC_COL
+            cc = np.zeros((len(rowShifts), 1))
+            for s in range(len(rowShifts)):
+                t = np.roll(patch, rowShifts[s], axis=-2)
+                sh = np.roll(t, colShifts[s], axis=-1)
+                # a complex sum stored into a float array keeps its real part
+                cc[s] = np.squeeze(np.sum(sh.conj() * O[..., sy, sx], axis=(-2, -1))).real
+            betaGrad, r = 1000, 3
+        else:
+            ss = slice(-radius, radius + 1)
+            rowShifts, colShifts = np.mgrid[ss, ss]
+            rowShifts, colShifts = rowShifts.flatten(), colShifts.flatten()
+            FT_O = np.fft.fft2(O[..., sy, sx] - O[..., sy, sx].mean())
+            FT_Op = np.fft.fft2(patch - O.mean())
+            xcor = np.fft.ifft2(FT_O * FT_Op.conj())
+            xcor = abs(np.fft.fftshift(xcor))
+            N = xcor.shape[-1]
+            sw = slice(N // 2 - radius, N // 2 + radius + 1)
+            cc = xcor[..., sw, sw].flatten()
+            betaGrad, r = 5, 10
         cc = abs(cc)
         normFactor = np.sum(patch.conj() * patch, axis=(-2, -1)).real
-        grad_x = 1000 * np.sum((cc.T - np.mean(cc)) / normFactor * self._PC_COL)
-        grad_y = 1000 * np.sum((cc.T - np.mean(cc)) / normFactor * self._PC_ROW)
-        r = 3
+        grad_x = betaGrad * np.sum((cc.T - np.mean(cc)) / normFactor * colShifts)
+        grad_y = betaGrad * np.sum((cc.T - np.mean(cc)) / normFactor * rowShifts)
         if abs(grad_x) > r:
             grad_x = r * grad_x / abs(grad_x)
         if abs(grad_y) > r:

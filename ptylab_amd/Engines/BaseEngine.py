@@ -97,9 +97,6 @@ class BaseEngine:
             raise NotImplementedError("only CPM is implemented")
         if p.positionCorrectionSwitch and self._POSITION_CORRECTION:
             r = self.reconstruction
-            if getattr(p, "positionCorrectionSwitch_radius", 1) >= 2:
-                raise NotImplementedError("positionCorrectionSwitch_radius >= 2 (Fourier cross-correlation) is not "
-                                          "implemented by the MI355X engines")
             if r.nlambda != 1 or r.nosm != 1 or r.nslice != 1:
                 raise NotImplementedError("position correction needs nlambda = nosm = nslice = 1")
 
@@ -293,6 +290,7 @@ class BaseEngine:
         inst.detector_error = ptr("detector_error")
         # positionCorrection runs once len(error) > startAtIteration = 1
         inst.pc_start = max(0, 2 - int(getattr(self, "_n_err0", 0)))
+        inst.pc_radius = int(getattr(self.params, "positionCorrectionSwitch_radius", 1))
         inst.workgroups = int(self.workgroups)
         inst.flags = self._instance_flags()
         return inst

@@ -130,6 +130,8 @@ struct DevInst {
   double* adampart;  // mqNewton: per-workgroup FP64 partials of |gradient|^2, (2, G)
   double* adamv;     // mqNewton: ADAM second moments of object and probe, (2 parities, 2)
   double* pcpart;    // position correction: per-workgroup FP64 partials, (G, PC_TERMS)
+  float2* PCW;       // Fourier position correction: both patches' transforms, (2, Np, Np)
+  double* pcc;       // Fourier position correction: |xcor| on the (2r + 1)^2 shifts
   unsigned osel[32]; // fields whose object terms the owner of field f's object sums (bit g)
   unsigned psel[32]; // fields whose probe terms the owner of field f's probe sums (bit g)
   int fl;            // the row passes may loop over the fields of a line in one wave: one
@@ -1408,6 +1410,133 @@ __device__ __forceinline__ void pc_direction_update(Run& R, int k) {
   Dk[1] = 0.5 * gx + 0.9 * Dk[1];
 }
 
+// Fourier branch of the position correction (BaseEngine.py:1277-1302,
+// positionCorrectionSwitch_radius r >= 2): cc = |fftshift(ifft2(fft2(O[sy, sx]
+// - mean) conj(fft2(Opatch - O.mean()))))| on the (2r + 1)^2 shifts around
+// the centre, Opatch the patch before this position's update.  For the
+// circular correlation the mean of the whole object drops out (the mean-free
+// updated patch sums to zero), so the product's zero-frequency term is
+// removed instead of subtracting the means.  Three passes over the two-field
+// workspace PCW: forward row transforms of both patches (with the FP64
+// partials of normFactor = sum |Opatch|^2), forward column transforms /
+// product / inverse column transform keeping the 2r + 1 rows of the window,
+// inverse row transforms of those rows giving |xcor| (numpy's ifft2 divides
+// by Np^2); then workgroup 0 forms the search direction (pcf_direction_update).
+template <int NP>
+__device__ __forceinline__ void pcf_rows(Run& R, int k) {
+  constexpr int L = Line<NP>::L;
+  const DevInst& D = *R.D;
+  const int No = D.a.No;
+  const int row0 = D.a.positions[2 * k], col0 = D.a.positions[2 * k + 1];
+  const float2* O = (const float2*)D.a.object;
+  const size_t Np2 = (size_t)NP * NP;
+  double nrm = 0.0;
+  const int per = NP / R.G;
+  const int line0 = R.g * per;
+  for (int li = R.wave * Line<NP>::SUB + R.sub; li < per; li += R.nwave * Line<NP>::SUB) {
+    const int i = line0 + li;
+    float2 a[L], b[L];
+    load_row<NP>(R, a, O + (size_t)(row0 + i) * No + col0, R.t);
+    load_row<NP>(R, b, D.OP + (size_t)i * NP, R.t);
+#pragma unroll
+    for (int l = 0; l < L; ++l) nrm += (double)b[l].x * b[l].x + (double)b[l].y * b[l].y;
+    fft_fwd<NP>(a, R.tw, R.t, R.fx);
+    fft_fwd<NP>(b, R.tw, R.t, R.fx);
+    store_row<NP>(R, D.PCW + (size_t)i * NP, a, R.t);
+    store_row<NP>(R, D.PCW + Np2 + (size_t)i * NP, b, R.t);
+  }
+  const double s = block_sum(R, nrm);
+  if (threadIdx.x == 0) cstored(R, D.pcpart + (size_t)R.g * PC_TERMS + 18, s);
+}
+
+template <int NP, int CWR>
+__device__ __forceinline__ void pcf_cols(Run& R) {
+  constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC, CW = Col<NP, CWR>::CW, NG = Col<NP, CWR>::NG;
+  const DevInst& D = *R.D;
+  const size_t Np2 = (size_t)NP * NP;
+  const int r = D.a.pc_radius;
+  const int rho = col_rho<NP, CWR>(R);
+  const int cpg = (NG + R.G - 1) / R.G;
+  const int cg0 = R.g * cpg;
+  const int cgn = cg0 < NG ? (NG - cg0 < cpg ? NG - cg0 : cpg) : 0;
+  for (int gi = R.wave; gi < cgn; gi += R.nwave) {
+    const int p = (cg0 + gi) * CW + R.lane % CW;
+    float2 a[CL], b[CL];
+    load_colg<NP, CWR>(R, a, D.PCW, p, rho);
+    load_colg<NP, CWR>(R, b, D.PCW + Np2, p, rho);
+    fft_fwd_colg<NP, CWR>(R, a, rho);
+    fft_fwd_colg<NP, CWR>(R, b, rho);
+#pragma unroll
+    for (int l = 0; l < CL; ++l) a[l] = cmul(a[l], conjf2(b[l]));
+    // element (n, p) holds frequency (brev(n), brev(p)): zero frequency at n = p = 0
+    if (p == 0 && rho == 0) a[0] = make_float2(0.f, 0.f);
+    fft_inv_colg<NP, CWR>(R, a, rho);
+#pragma unroll
+    for (int l = 0; l < CL; ++l) {
+      const int n = l * LPC + rho;  // natural row order after the inverse transform
+      if (n <= r || n >= NP - r) cstore(R, D.PCW + (size_t)n * NP + p, a[l]);
+    }
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void pcf_window_rows(Run& R) {
+  constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF, SUB = Line<NP>::SUB;
+  const DevInst& D = *R.D;
+  const int r = D.a.pc_radius, nw = 2 * r + 1;
+  const float inv = 1.0f / ((float)NP * (float)NP);
+  const int step = R.G * R.nwave * SUB;
+  for (int w = (R.g * R.nwave + R.wave) * SUB + R.sub; w < nw; w += step) {
+    const int y = (w - r) & (NP - 1);  // row shift w - r
+    float2 v[L];
+    load_row<NP>(R, v, D.PCW + (size_t)y * NP, R.t);
+    fft_inv<NP>(v, R.tw, R.t, R.fx);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const int x = l * LPF + R.t;
+      const int dx = x <= r ? x : x - NP;  // column shift
+      if (dx >= -r && dx <= r) cstored(R, D.pcc + (size_t)w * nw + (dx + r), (double)(sqrtf(cabs2(v[l])) * inv));
+    }
+  }
+}
+
+// D[k] = daleth clip(grad) + beth D[k] with grad = betaGrad sum_s (cc_s -
+// mean cc) / normFactor (colShift_s, rowShift_s), betaGrad = 5, clipped at
+// r = 10 pixels (BaseEngine.py:1295-1323); rowShifts / colShifts in np.mgrid
+// order (row shift slowest).  Every thread of workgroup 0 takes a share of
+// the shifts.
+__device__ __forceinline__ void pcf_direction_update(Run& R, int k) {
+  const DevInst& D = *R.D;
+  if (R.g != 0) return;
+  const int r = D.a.pc_radius, nw = 2 * r + 1, n = nw * nw;
+  double sc = 0.0;
+  for (int s = threadIdx.x; s < n; s += R.nt) sc += cloadd(R, D.pcc + s);
+  sc = block_sum(R, sc);
+  if (threadIdx.x == 0) R.red[0] = sc;
+  __syncthreads();
+  const double mean = R.red[0] / (double)n;
+  __syncthreads();
+  double nf = 0.0;
+  for (int g = 0; g < R.G; ++g) nf += cloadd(R, D.pcpart + (size_t)g * PC_TERMS + 18);
+  double px = 0.0, py = 0.0;
+  for (int s = threadIdx.x; s < n; s += R.nt) {
+    const double w = (cloadd(R, D.pcc + s) - mean) / nf;
+    px += w * (double)(s % nw - r);
+    py += w * (double)(s / nw - r);
+  }
+  double gx = block_sum(R, px);
+  double gy = block_sum(R, py);
+  if (threadIdx.x == 0) {
+    gx *= 5.0;
+    gy *= 5.0;
+    if (fabs(gx) > 10.0) gx = 10.0 * gx / fabs(gx);
+    if (fabs(gy) > 10.0) gy = 10.0 * gy / fabs(gy);
+    double* Dk = D.a.pc_direction + 2 * (size_t)k;
+    Dk[0] = 0.5 * gy + 0.9 * Dk[0];
+    Dk[1] = 0.5 * gx + 0.9 * Dk[1];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // e3PIE slice chain (e3PIE.py:89-173, updates :192-225).  W holds the
 // (lambda, psm) fields of the slice being propagated; ESW keeps the exit
@@ -1653,9 +1782,19 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
-      pc_corr_pass<NP>(R, k);
-      if (!cluster_sync(R)) return false;
-      pc_direction_update(R, k);
+      if (D.a.pc_radius >= 2) {
+        pcf_rows<NP>(R, k);
+        if (!cluster_sync(R)) return false;
+        pcf_cols<NP, CWR>(R);
+        if (!cluster_sync(R)) return false;
+        pcf_window_rows<NP>(R);
+        if (!cluster_sync(R)) return false;
+        pcf_direction_update(R, k);
+      } else {
+        pc_corr_pass<NP>(R, k);
+        if (!cluster_sync(R)) return false;
+        pc_direction_update(R, k);
+      }
     }
   }
   if constexpr (QN) {
@@ -2282,8 +2421,8 @@ int validate(const pty_instance& a, int idx, int nt) {
   if (a.engine < PTY_EPIE || a.engine > PTY_MQNEWTON) return set_err(PTY_EINVAL, "instance %d: unknown engine %d", idx, a.engine);
   if (a.pc_direction && (a.engine == PTY_E3PIE || a.engine == PTY_QNEWTON || a.engine == PTY_MQNEWTON))
     return set_err(PTY_EINVAL, "instance %d: position correction runs with the ePIE, mPIE and multiPIE engines", idx);
-  if (a.pc_direction && a.pc_radius >= 2)
-    return set_err(PTY_EINVAL, "instance %d: pc_radius >= 2 (Fourier position correction) is not implemented", idx);
+  if (a.pc_direction && a.pc_radius >= 2 && 2 * a.pc_radius + 1 > a.Np)
+    return set_err(PTY_EINVAL, "instance %d: pc_radius %d needs 2 pc_radius + 1 <= Np", idx, a.pc_radius);
   if (a.pc_direction && (a.nlambda != 1 || a.nosm != 1 || a.nslice != 1))
     return set_err(PTY_EINVAL, "instance %d: position correction needs nlambda = nosm = nslice = 1", idx);
   if (a.engine == PTY_MQNEWTON && (!a.objectBuffer || !a.objectMomentum || !a.probeBuffer || !a.probeMomentum))
@@ -2515,6 +2654,13 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
     if (a.pc_direction) {
       if ((rc = dalloc(p, &mem, (size_t)G * PC_TERMS * sizeof(double)))) goto fail;
       d.pcpart = (double*)mem;
+      if (a.pc_radius >= 2) {
+        if ((rc = dalloc(p, &mem, (size_t)2 * np2 * sizeof(float2)))) goto fail;
+        d.PCW = (float2*)mem;
+        const size_t nw = (size_t)(2 * a.pc_radius + 1);
+        if ((rc = dalloc(p, &mem, nw * nw * sizeof(double)))) goto fail;
+        d.pcc = (double*)mem;
+      }
     }
     if (a.engine == PTY_MQNEWTON) {
       if ((rc = dalloc(p, &mem, (size_t)2 * G * sizeof(double)))) goto fail;

@@ -218,8 +218,8 @@ def test_mqnewton_on_photon_scaled_inputs(c0):
         assert eo <= bar and ep <= bar and ee <= 1e-3
 
 
-@pytest.mark.parametrize("engine", ["mPIE", "pcPIE"])
-def test_position_correction_against_oracle(c0, engine):
+@pytest.mark.parametrize("engine,radius", [("mPIE", 1), ("pcPIE", 1), ("mPIE", 4), ("pcPIE", 10)])
+def test_position_correction_against_oracle(c0, engine, radius):
     """Position correction (BaseEngine.py:1241-1323, 1400-1438) on the GPU:
     per position the 9 shifted correlations of the pre-update with the updated
     patch (FP64 sums), the search directions D, and between launches the
@@ -228,7 +228,7 @@ def test_position_correction_against_oracle(c0, engine):
     from ptyhelpers import build_package
 
     d = c0.as_dict()
-    sw = dict(positionCorrectionSwitch=True)
+    sw = dict(positionCorrectionSwitch=True, positionCorrectionSwitch_radius=radius)
     eng = build_package(d, engine, "Fraunhofer", (1, 1, 1, 1), 4, 3, order="random", No=c0.No, switches=sw)
     np.random.seed(3)
     eng.reconstruct()
@@ -236,7 +236,8 @@ def test_position_correction_against_oracle(c0, engine):
     ref = run_oracle(d, engine, "Fraunhofer", (1, 1, 1, 1), 4, 3, order="random", No=c0.No, switches=sw)
     dD = np.max(np.abs(eng.D - ref.D))
     moved = int(np.count_nonzero(ref.D))
-    print(f"{engine} position correction: max |D_gpu - D_oracle| {dD:.2e} px, nonzero directions {moved}, "
+    print(f"{engine} position correction radius {radius}: max |D_gpu - D_oracle| {dD:.2e} px, "
+          f"max |D| {np.abs(ref.D).max():.2f} px, nonzero directions {moved}, "
           f"positions equal {np.array_equal(gpu.positions, ref.positions)}", flush=True)
     assert moved > 0
     assert dD < 5e-2

@@ -432,3 +432,70 @@ def test_detector_error_and_save_memory_errors():
     np.testing.assert_array_equal(a.object, b.object)
     assert b.detectorError == 0 and b.errorAtPos.shape == (12, 12)
     np.testing.assert_allclose(b.error[-1], np.sum(per_pos) * np.sum(1 / (E + 1e-20)), rtol=1e-5)
+
+
+def _circular_xcor_brute(a, b, radius):
+    """|sum_x (a - mean a)[x + d] conj(b - mean b)[x]| over the shifts
+    d in [-radius, radius]^2 (row shift slowest), by explicit loops."""
+    n = a.shape[-1]
+    am, bm = a - a.mean(), b - b.mean()
+    out = []
+    for dy in range(-radius, radius + 1):
+        for dx in range(-radius, radius + 1):
+            acc = 0j
+            for y in range(n):
+                for x in range(n):
+                    acc += am[(y + dy) % n, (x + dx) % n] * np.conj(bm[y, x])
+            out.append(abs(acc))
+    return np.array(out)
+
+
+def test_fourier_position_correction_is_the_circular_cross_correlation():
+    """The Fourier branch (BaseEngine.py:1277-1302): |fftshift(ifft2(fft2(O -
+    mean) conj(fft2(Opatch - O.mean()))))| cropped to +-radius is the circular
+    cross-correlation of the mean-free patches (brute force, float64), with
+    the row shift slowest as np.mgrid orders rowShifts / colShifts; the
+    whole-object mean drops out (the mean-free updated patch sums to zero),
+    so removing the product's zero-frequency term instead of subtracting the
+    means gives the same correlation -- the form the GPU computes."""
+    rng = np.random.default_rng(3)
+    n, radius = 16, 3
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    b = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    want = _circular_xcor_brute(a, b, radius)
+    other_mean = 0.37 - 0.21j  # any offset of the pre-update patch
+    xc = np.fft.ifft2(np.fft.fft2(a - a.mean()) * np.fft.fft2(b - other_mean).conj())
+    sw = slice(n // 2 - radius, n // 2 + radius + 1)
+    got = abs(np.fft.fftshift(xc))[sw, sw].flatten()
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    P = np.fft.fft2(a) * np.fft.fft2(b).conj()
+    P[0, 0] = 0
+    dc_free = abs(np.fft.fftshift(np.fft.ifft2(P)))[sw, sw].flatten()
+    np.testing.assert_allclose(dc_free, want, rtol=1e-12, atol=1e-12)
+
+
+def test_fourier_position_correction_points_along_a_known_shift():
+    """With the updated patch the pre-update one rolled by (2, 0) the
+    correlation peaks at that shift: the row direction is positive, clipped
+    at 10 pixels and damped by daleth = 0.5 (BaseEngine.py:1304-1323), the
+    column direction vanishes for a symmetric column profile; the transposed
+    case gives a negative column direction for a roll left."""
+    run, geo = _c0_pair("pcPIE", positionCorrectionSwitch=True, positionCorrectionSwitch_radius=4)
+    run.error = [1.0, 1.0]
+    k = 0
+    row, col = run.positions[k]
+    Np = int(geo.Np)
+    t = 2 * np.pi * np.arange(Np) / Np
+    gen = 2 + np.cos(t) + 0.3 * np.sin(2 * t)
+    sym = 2 + np.cos(t)
+    patch = np.outer(gen, sym).astype(np.complex64)[None, None, None, None]
+    run.object[..., row:row + Np, col:col + Np] = np.roll(patch, 2, axis=-2)
+    run._position_correction(k, patch)
+    dy, dx = run.D[k]
+    assert dy > 0 and abs(dx) < 1e-3 * abs(dy)
+    run.D[k] = 0.0
+    patch = np.outer(sym, gen).astype(np.complex64)[None, None, None, None]
+    run.object[..., row:row + Np, col:col + Np] = np.roll(patch, -2, axis=-1)
+    run._position_correction(k, patch)
+    dy, dx = run.D[k]
+    assert dx < 0 and abs(dy) < 1e-3 * abs(dx)
