@@ -25,7 +25,18 @@ Parity pins (what checks each part; the tests live in ``tests/``):
   ``epie_asp``, ``epie_mixed_state``, ``mpie_single``, ``mpie_mixed_state``.
 * e3PIE slice chain: golden ``e3pie_multislice``.
 * multiPIE: no golden exists; pinned by the identity multiPIE == mPIE when
-  nlambda == nosm == 1 (the two probe updates then sum over the same axes).
+  nlambda == nosm == 1 (the two probe updates then sum over the same axes) and
+  by duplicated wavelengths / object modes / slices reducing to one channel
+  (which mPIE's probe sum fails).
+* ``fftshiftSwitch``: the shifted-data, unshifted-transform run equals the
+  centred one to rounding.
+* mqNewton ADAM / NADAM / momentum: the beta_m = 0 reduction to qNewton and
+  each engine's closed-form first steps.
+* position correction: pcPIE == mPIE without it; known rolls give the
+  expected directions (direct and Fourier branches); the Fourier correlation
+  equals a brute-force circular cross-correlation of the mean-free patches.
+* detectorError / saveMemory: the per-pixel errors sum to the error curve; the
+  saveMemory curve is the reference's broadcast sum.
 * ``positionOrder='random'``: no golden; pinned by the RNG-consumption identity
   with ``np.random.shuffle`` (tested) -- otherwise "parity unpinned".
 * ``orthogonalize_modes``: pinned by orthogonality, descending eigenvalues and
