@@ -202,21 +202,33 @@ struct TPad {
   int s, msk, cc, size, stride;  // padding n + cc * ((n >> s) & msk); elements of one line; line stride
 };
 
-// Padding of the transpose between layouts k1 and k2 (symmetric).
+// Padding of the transpose between layouts k1 and k2 (symmetric), for a
+// line of N points on P lanes, LS lines per wave.
+struct TPadEntry {
+  int N, P, LS, a, b;  // configuration and layout pair (a < b)
+  TPad pad;
+};
+constexpr TPadEntry kTPad[] = {
+    {64, 16, 4, 0, 2, {2, 7, 5, 99, 100}},      {64, 16, 4, 0, 4, {2, 7, 5, 99, 100}},
+    {64, 16, 4, 2, 4, {0, 0, 0, 64, 68}},       {64, 8, 8, 0, 3, {3, 7, 1, 71, 74}},
+    {128, 32, 2, 0, 1, {2, 31, 1, 159, 168}},   {128, 32, 2, 0, 5, {3, 15, 1, 143, 152}},
+    {128, 32, 2, 1, 3, {3, 15, 2, 158, 168}},   {128, 32, 2, 3, 5, {0, 0, 0, 128, 136}},
+    {128, 16, 4, 0, 1, {3, 15, 1, 143, 148}},   {128, 16, 4, 0, 4, {3, 15, 1, 143, 148}},
+    {128, 16, 4, 1, 4, {3, 15, 1, 143, 148}},   {128, 8, 8, 0, 3, {4, 7, 1, 135, 138}},
+    {256, 64, 1, 0, 2, {2, 63, 1, 319, 319}},   {256, 64, 1, 0, 6, {4, 15, 1, 271, 271}},
+    {256, 64, 1, 2, 4, {4, 15, 4, 316, 316}},   {256, 64, 1, 4, 6, {0, 0, 0, 256, 256}},
+    {256, 32, 2, 0, 2, {3, 31, 1, 287, 296}},   {256, 32, 2, 0, 5, {3, 31, 1, 287, 296}},
+    {256, 32, 2, 2, 5, {3, 31, 1, 287, 296}},   {256, 16, 4, 0, 4, {4, 15, 1, 271, 276}},
+    {256, 8, 8, 0, 3, {5, 7, 1, 263, 266}},     {512, 64, 1, 0, 3, {3, 63, 1, 575, 575}},
+    {512, 64, 1, 0, 6, {4, 31, 1, 543, 543}},   {512, 64, 1, 3, 6, {4, 31, 2, 574, 574}},
+    {512, 32, 2, 0, 1, {4, 31, 1, 543, 552}},   {512, 32, 2, 0, 5, {4, 31, 1, 543, 552}},
+    {512, 32, 2, 1, 5, {4, 31, 1, 543, 552}},   {512, 16, 4, 0, 4, {5, 15, 1, 527, 532}},
+    {512, 8, 8, 0, 3, {6, 7, 1, 519, 522}},
+};
 constexpr TPad tpad(int N, int P, int LS, int k1, int k2) {
   const int a = k1 < k2 ? k1 : k2, b = k1 < k2 ? k2 : k1;
-  if (N == 256 && P == 64 && LS == 1) {
-    if (a == 4 && b == 6) return {0, 0, 0, 256, 256};
-    if (a == 2 && b == 4) return {4, 15, 4, 316, 316};
-    if (a == 0 && b == 2) return {2, 63, 1, 319, 319};
-    if (a == 0 && b == 6) return {4, 15, 1, 271, 271};
-  }
-  if (N == 512 && P == 64 && LS == 1) {
-    if (a == 3 && b == 6) return {4, 31, 2, 574, 574};
-    if (a == 0 && b == 3) return {3, 63, 1, 575, 575};
-    if (a == 0 && b == 6) return {4, 31, 1, 543, 543};
-  }
-  if (N == 512 && P == 32 && LS == 2) return {4, 31, 1, 543, 552};
+  for (const TPadEntry& e : kTPad)
+    if (e.N == N && e.P == P && e.LS == LS && e.a == a && e.b == b) return e.pad;
   return {-1, 0, 0, 0, 0};
 }
 

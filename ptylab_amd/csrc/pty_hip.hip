@@ -58,7 +58,8 @@ static_assert(sizeof(pty_instance) == 216, "pty_instance layout (ABI 4) changed:
 #define CTRL_GROUP 32    // counter barrier: 8 group arrival counters, 32 words apart
 #define CTRL_RELEASE 288 // counter barrier: published generation
 #define CTRL_WORDS 320
-#define SLOT_WORDS 32  // words of per-position maxima in each of the 4 ring slots
+#define SLOT_WORDS 32   // words of per-position maxima in each of the 4 ring slots
+#define SLOT_QN_OMAX 2   // qNewton's object maximum (words 0, 1: the first row pass's maxima)
 
 #define N_XCD 8  // XCDs of an MI355X (gfx950), each with its own L2
 #define PC_TERMS 19  // position correction: 9 complex correlations and the patch norm
@@ -122,6 +123,8 @@ struct DevInst {
   const float2* HT;  // transfer function(s), bit-reversed layout, / Np^2; one per wavelength
   const float2* Q;   // Fresnel quadratic phase (Np, Np), natural order, or null
   const unsigned char* live;  // ASP: per transfer function, 1 where column p of HT has a nonzero entry; or null
+  const short* lgrp;  // ASP: per transfer function, the column groups (of the kernel's width) with a live column
+  const int* nlive;   // [0]: most live groups of any transfer function, [1 + t]: those of function t; or null
   int fpl;           // W fields per transfer function (fields of one wavelength)
   float2* OP;        // qNewton: the object patch before the position's update (nO, Np, Np)
   unsigned* ctrl;    // control block, CTRL_* layout
@@ -179,6 +182,7 @@ struct Run {
   float2* xb;
   float2* fx;  // this wave's LDS scratch of the transposed line transforms
   int* fail;
+  unsigned* smx;  // LDS: the workgroup's running maxima and arrival count of publish_max2
   bool loc;  // every workgroup of the cluster runs on one XCD: exchange through its L2
   bool fl;   // field-loop row passes (DevInst::fl, and at least one line per item slot)
 };
@@ -232,6 +236,44 @@ __device__ __forceinline__ float load_slot(const Run& R, const unsigned* slot, i
 __device__ __forceinline__ void clear_slot(const Run& R, unsigned* slot) {
   if (threadIdx.x < SLOT_WORDS) __hip_atomic_store(slot + threadIdx.x * R.G + R.g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// clears this workgroup's partial of one word of a ring slot
+__device__ __forceinline__ void clear_slot_word(const Run& R, unsigned* slot, int w) {
+  if (threadIdx.x == 0) __hip_atomic_store(slot + w * R.G + R.g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The first row pass's maxima (probe word 0, object word 1): the waves of a
+// workgroup combine theirs with LDS atomics, and the last wave to finish
+// publishes the workgroup's partials with plain stores (device-coherent ones
+// across XCDs).  Every workgroup writes its partials at every position, so
+// these words need no clearing, and no global atomic read-modify-write is
+// issued: its round trip past the L2 is what the barrier's store drain would
+// otherwise wait for.  The LDS words are reset by the publishing wave; the
+// next position's pass is several workgroup barriers later.
+__device__ __forceinline__ void store_slot(const Run& R, unsigned* p, unsigned v) {
+  if (R.loc)
+    *(__attribute__((address_space(1))) unsigned*)p = v;
+  else
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish_max2(const Run& R, unsigned* slot, float a, float b) {
+  a = wave_max(a);
+  b = wave_max(b);
+  if (R.lane == 0) {
+    unsigned* m = R.smx;
+    __hip_atomic_fetch_max(m, __float_as_uint(a), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_max(m + 1, __float_as_uint(b), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const unsigned n = __hip_atomic_fetch_add(m + 2, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (n + 1u == (unsigned)R.nwave) {
+      const unsigned pa = __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const unsigned pb = __hip_atomic_load(m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(m, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(m + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(m + 2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      store_slot(R, slot + R.g, pa);
+      store_slot(R, slot + R.G + R.g, pb);
+    }
+  }
+}
 
 // Cross-workgroup data of the persistent kernel (exit waves, object, probe,
 // momenta, maxima, error partials) is only ever read and written with
@@ -281,6 +323,22 @@ __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
     v = o < v ? o : v;
   }
   return v;
+}
+
+// called by every lane of a wave; returns the maximum of the G partials of
+// word w that publish_max2 wrote (XCD-local or device-coherent loads)
+__device__ __forceinline__ float load_max_published(const Run& R, const unsigned* slot, int w) {
+  const unsigned* p = slot + w * R.G;
+  float m = 0.f;
+  if (R.loc) {
+    const LocalRef L = local_ref(p);
+    for (int i = R.lane; i < R.G; i += 64)
+      m = fmaxf(m, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(L.r, L.off + (uint32_t)i * 4u, 0, 1)));
+  } else {
+    for (int i = R.lane; i < R.G; i += 64)
+      m = fmaxf(m, __uint_as_float(__hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+  }
+  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(m))));
 }
 
 // Arrival is two-level: a workgroup increments the counter of its XCD group
@@ -507,6 +565,13 @@ struct Col {
   static constexpr int NG = NP / CW;                          // column groups
 };
 
+// Col<NP, CWR>::CW of a run-time probe size
+template <int CWR>
+constexpr int col_width(int np) {
+  return np == 16 ? Col<16, CWR>::CW : np == 32 ? Col<32, CWR>::CW : np == 64 ? Col<64, CWR>::CW
+       : np == 128 ? Col<128, CWR>::CW : np == 256 ? Col<256, CWR>::CW : Col<512, CWR>::CW;
+}
+
 template <int NP>
 struct Cfg {
   static constexpr int NT = NP >= 128 ? 512 : 1024;  // threads per workgroup
@@ -517,7 +582,9 @@ struct Cfg {
   // passes one (NW, values per lane, 64) float buffer
   static constexpr int XROW = 2 * IPR * NP;
   static constexpr int XCOL = NW * NP * Col<NP>::CW / 2;
-  static constexpr int XBUF = XROW > XCOL ? XROW : XCOL;  // float2 elements
+  static constexpr int XSTG = NW * Col<NP>::CW * (2 * NP + 32) * (NP <= 256);  // staged column rounds
+  static constexpr int XRC = XROW > XCOL ? XROW : XCOL;
+  static constexpr int XBUF = XRC > XSTG ? XRC : XSTG;  // float2 elements
 };
 
 #define ITEMS_BEGIN(R, NP, line, f, act, islot, lslot, frw)                     \
@@ -592,7 +659,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
   const bool cb = prop_checkerboard(D.a.propagator);
   const float2* __restrict__ Qp = D.Q;
   constexpr bool qn = QN;
-  clear_slot(R, zslot);
+  if constexpr (QN) clear_slot_word(R, zslot, SLOT_QN_OMAX);
   float pmax = 0.f, omax = 0.f;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     const FieldIdx x = field_idx(f, no, npm, ns);
@@ -644,12 +711,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
     }
     ITEM_ROUND_SYNC();
   ITEMS_END
-  pmax = wave_max(pmax);
-  omax = wave_max(omax);
-  if (R.lane == 0) {
-    atomic_fmax_slot(R, slot, 0, pmax);
-    atomic_fmax_slot(R, slot, 1, omax);
-  }
+  publish_max2(R, slot, pmax, omax);
 }
 
 // Column passes: a wave owns a group of CW adjacent columns of one field
@@ -724,7 +786,7 @@ constexpr int wave_scratch() { return fft_scratch<NP, Col<NP, CWR>::CW>(); }
 // Fields f with f % fs == fs - 1 form the modelled intensity (the last slice,
 // BaseEngine.py:889-891).  Writes this workgroup's FP64 error partial.
 template <int NP, int CWR>
-__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de) {
+__device__ __forceinline__ void detector_cols_fraunhofer_direct(Run& R, int k, int j, double* ep, int fs, float* de) {
   constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -772,6 +834,133 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
 
+// The same pass with each round's block staged through LDS.  A round covers
+// C = (groups per round) * CW adjacent columns of every field; the workgroup
+// loads that block of W (fields x NP rows x C columns) and the C columns of
+// the frame with row-contiguous accesses (a load instruction of the direct
+// form touches 64 rows, one segment each, here 64 / C rows of C * 8 bytes),
+// each wave takes its column group from the tile, and the block goes back
+// with row-contiguous stores.  The transforms, sums and constraint are the
+// direct form's, so results are bit-identical.  Tiles are column-major with
+// a column stride of NP + 16 / C float2 (NP + 32 / C floats for the frame),
+// which spreads the rows of one store instruction over distinct LDS banks
+// while a wave's column read stays contiguous.  LDS: tiles of at most
+// NW * CW * (NP + 16) float2 and NW * CW * (NP + 32) floats plus the
+// NW * CL * 64 floats of the field sums: NW * CW * (2 NP + 32) float2 in all
+// (XSTG of the kernel configurations), too much for NP = 512.
+template <int NP>
+constexpr bool col_staged() { return NP <= 256; }
+template <int NP, int CWR>
+__device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, int j, double* ep, int fs, float* de) {
+  constexpr int CW = Col<NP, CWR>::CW, CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
+  const DevInst& D = *R.D;
+  const size_t Np2 = (size_t)NP * NP;
+  float2* __restrict__ W = D.W;
+  const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
+  const float invN = 1.0f / (float)NP;
+  // the tile geometry is recomputed in every pass (opaque inputs): hoisted
+  // out of the position loop it would hold registers through every other pass
+  int nF = D.nF, nFp = R.nFp;
+  __asm__ volatile("" : "+s"(nF), "+s"(nFp));
+  const int rho = col_rho<NP, CWR>(R);
+  const int cl = R.lane % CW;
+  const int C = (R.nwave / nFp) * CW;  // columns per round, a power of two
+  const int lgC = __builtin_ctz((unsigned)C);
+  const int lgNC = lgC + Line<NP>::LOG2;
+  const int cs = NP + (C == 1 ? 0 : C <= 16 ? 16 / C : 1);
+  const int csi = NP + (C == 1 ? 0 : C <= 32 ? 32 / C : 1);
+  const int TS = C * cs;
+  float2* tile = R.xa;
+  float* ikt = (float*)(R.xa + nFp * TS);
+  float* xs = ikt + C * csi;
+  double err = 0.0;
+  CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
+    const int pc0 = (cg0_ + base_) * CW;                               // first column of the round
+    const int Cv = (cgn_ - base_ < gpr_ ? cgn_ - base_ : gpr_) * CW;  // its columns
+    {
+      float2 w[CL];
+#pragma unroll
+      for (int i = 0; i < CL; ++i) {
+        const int e = i * R.nt + (int)threadIdx.x;
+        const int s = e >> lgNC, n = (e >> lgC) & (NP - 1), c = e & (C - 1);
+        const int sc = s < nF ? s : nF - 1, cc = c < Cv ? c : Cv - 1;
+        w[i] = cload(R, W + sc * Np2 + (size_t)n * NP + pc0 + cc);
+      }
+#pragma unroll
+      for (int i = 0; i < CL; ++i) {
+        const int e = i * R.nt + (int)threadIdx.x;
+        const int s = e >> lgNC, n = (e >> lgC) & (NP - 1), c = e & (C - 1);
+        if (s < nF && c < Cv) tile[s * TS + c * cs + n] = w[i];
+      }
+      for (int e = threadIdx.x; e < (NP << lgC); e += R.nt) {
+        const int n = e >> lgC, c = e & (C - 1);
+        ikt[c * csi + n] = ldg(Ik + (size_t)n * NP + pc0 + (c < Cv ? c : Cv - 1));
+      }
+    }
+    __syncthreads();
+    // this lane's tile column; a wave past the round's last group repeats
+    // that group (and stores nothing), as in the direct form
+    const int q = (lg_ * CW < Cv ? lg_ * CW : Cv - CW) + cl;
+    float2 v[CL];
+#pragma unroll
+    for (int l = 0; l < CL; ++l) v[l] = tile[f * TS + q * cs + l * LPC + rho];
+    fft_fwd_colg<NP, CWR>(R, v, rho);
+    float iest[CL];
+#pragma unroll
+    for (int l = 0; l < CL; ++l) v[l] = cscale(v[l], invN);
+    if (nF == 1) {
+      // one field: the modelled intensity is its own (0 + x = x)
+#pragma unroll
+      for (int l = 0; l < CL; ++l) iest[l] = cabs2(v[l]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < CL; ++l) xs[(islot * CL + l) * 64 + R.lane] = act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f;
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < CL; ++l) iest[l] = 0.f;
+      for (int g = 0; g < nF; ++g) {
+#pragma unroll
+        for (int l = 0; l < CL; ++l) iest[l] += xs[((lslot + g) * CL + l) * 64 + R.lane];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < CL; ++l) {
+      const int n = l * LPC + rho;
+      const float im = ikt[q * csi + n];
+      if (ITEM_LEAD(act, frw)) {
+        err += (double)fabsf(im - iest[l]);
+        // element (n, p) is detector pixel (brev(n), brev(p)) (the frames'
+        // plan-time permutation)
+        if (de) de[(size_t)brev(n, Line<NP>::LOG2) * NP + brev(p, Line<NP>::LOG2)] = fabsf(im - iest[l]);
+      }
+      v[l] = cscale(v[l], sqrtf(im / (iest[l] + 1e-10f)));
+    }
+    fft_inv_colg<NP, CWR>(R, v, rho);
+    if (act) {
+#pragma unroll
+      for (int l = 0; l < CL; ++l) tile[frw * TS + q * cs + l * LPC + rho] = v[l];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < CL; ++i) {
+      const int e = i * R.nt + (int)threadIdx.x;
+      const int s = e >> lgNC, n = (e >> lgC) & (NP - 1), c = e & (C - 1);
+      if (s < nF && c < Cv) cstore(R, W + s * Np2 + (size_t)n * NP + pc0 + c, tile[s * TS + c * cs + n]);
+    }
+    CITEM_ROUND_SYNC();
+  CITEMS_END
+  const double s = block_sum(R, err);
+  if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
+}
+
+template <int NP, int CWR>
+__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de) {
+  if constexpr (col_staged<NP>())
+    detector_cols_fraunhofer_staged<NP, CWR>(R, k, j, ep, fs, de);
+  else
+    detector_cols_fraunhofer_direct<NP, CWR>(R, k, j, ep, fs, de);
+}
+
 // Row elements in columns that the column pass skipped (outside the band
 // limit of field f's transfer function) are zero.  live_bits reads the
 // column flags of this lane's elements (bit l for element l * LPF + t); it is
@@ -794,23 +983,37 @@ __device__ __forceinline__ void zero_dead_columns(unsigned bits, float2 (&v)[Lin
 
 // Column pass of an angular-spectrum propagation: forward column transform,
 // multiply by the (bit-reversed, 1/Np^2-scaled) transfer function or its
-// conjugate, inverse column transform.  Fields are independent.
+// conjugate, inverse column transform.  Fields are independent, so the
+// items are (field, column group) pairs and one wave takes one item.  A
+// column group outside the transfer function's band limit is zero after the
+// product; it is neither read nor written (the next row pass zeroes it), so
+// only the live groups of each field are items (DevInst::lgrp), split into G
+// contiguous ranges whose sizes differ by at most one: the live groups of a
+// band-limited function lie scattered in the bit-reversed column order, and
+// an equal share of them per workgroup keeps the slowest workgroup, which
+// the barrier after the pass waits for, from carrying several.
 template <int NP, int CWR>
 __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
-  constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
+  constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC, NG = Col<NP, CWR>::NG;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
   const int rho = col_rho<NP, CWR>(R);
-  CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
-    (void)islot;
-    (void)lslot;
-    (void)frw;
+  const int* __restrict__ nl = D.nlive;
+  const int ngl = nl ? nl[0] : NG;  // items per field
+  const int T = R.nF * ngl;
+  const int i0 = (int)(((long long)R.g * T) / R.G), i1 = (int)(((long long)(R.g + 1) * T) / R.G);
+  for (int i = i0 + R.wave; i < i1; i += R.nwave) {
+    const int f = i / ngl, m = i - f * ngl;
+    const int tf = f / D.fpl;
+    int cg = m;
+    if (nl) {
+      if (m >= nl[1 + tf]) continue;
+      cg = D.lgrp[tf * NG + m];
+    }
+    const int p = cg * Col<NP, CWR>::CW + R.lane % Col<NP, CWR>::CW;
     // the transfer function of field f's wavelength
-    const float2* __restrict__ HT = D.HT + (size_t)(f / D.fpl) * Np2;
-    // a column outside the transfer function's band limit is zero after the
-    // product; it is neither read nor written (the next row pass zeroes it)
-    if (D.live && __builtin_amdgcn_ballot_w64(D.live[(f / D.fpl) * NP + p] != 0) == 0) continue;
+    const float2* __restrict__ HT = D.HT + (size_t)tf * Np2;
     float2 v[CL];
     load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
     fft_fwd_colg<NP, CWR>(R, v, rho);
@@ -821,8 +1024,8 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
       v[l] = cmul(v[l], h);
     }
     fft_inv_colg<NP, CWR>(R, v, rho);
-    if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
-  CITEMS_END
+    store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
+  }
 }
 
 // ASP detector row pass: inverse row transform gives the detector field, the
@@ -897,7 +1100,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   const float2* __restrict__ Qp = D.Q;
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
-  const float Pmax = load_slot(R, slot, 0), Omax = load_slot(R, slot, 1);
+  const float Pmax = load_max_published(R, slot, 0), Omax = load_max_published(R, slot, 1);
   const float bO = D.a.betaObject;
   const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
   // ePIE.py:128-160 weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
@@ -1023,7 +1226,7 @@ __device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot,
   float2* __restrict__ W = D.W;
   const bool cb = prop_checkerboard(D.a.propagator);
   const float2* __restrict__ Qp = D.Q;
-  clear_slot(R, zslot);
+  (void)zslot;
   float pmax = 0.f, omax = 0.f;
   const int per = NP / R.G, line0 = R.g * per;
   const int islot = R.wave * Line<NP>::SUB + R.sub;
@@ -1061,12 +1264,7 @@ __device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot,
       }
     }
   }
-  pmax = wave_max(pmax);
-  omax = wave_max(omax);
-  if (R.lane == 0) {
-    atomic_fmax_slot(R, slot, 0, pmax);
-    atomic_fmax_slot(R, slot, 1, omax);
-  }
+  publish_max2(R, slot, pmax, omax);
 }
 
 template <int NP, bool ASPF, bool QPH>
@@ -1083,7 +1281,7 @@ __device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned
   const float2* __restrict__ Qp = D.Q;
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
-  const float Pmax = load_slot(R, slot, 0), Omax = load_slot(R, slot, 1);
+  const float Pmax = load_max_published(R, slot, 0), Omax = load_max_published(R, slot, 1);
   const float bO = D.a.betaObject;
   const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
   const float bP = engine == PTY_EPIE ? D.a.betaProbe : stepP;
@@ -1162,7 +1360,7 @@ __device__ __forceinline__ void qn_omax_pass(Run& R, unsigned* slot) {
     m = fmaxf(m, s);
   }
   m = wave_max(m);
-  if (R.lane == 0) atomic_fmax_slot(R, slot, 2, m);
+  if (R.lane == 0) atomic_fmax_slot(R, slot, SLOT_QN_OMAX, m);
 }
 
 // qNewton probe step (qNewton.py:129-140): probe += betaProbe sum over
@@ -1176,7 +1374,7 @@ __device__ __forceinline__ void qn_probe_rows(Run& R, const unsigned* slot) {
   const int nF = D.nF;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ P = (float2*)D.a.probe;
-  const float Omax = load_slot(R, slot, 2);
+  const float Omax = load_slot(R, slot, SLOT_QN_OMAX);
   const float reg = D.a.alphaProbe, bP = D.a.betaProbe;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     (void)frw;
@@ -1935,7 +2133,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
 // lane roles, LDS views, the twiddles of length NP.
 template <int NP, int NT, int CWR>
 __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2* s_tw, double* s_red, float2* s_x,
-                                         float2* s_fx, int* s_fail) {
+                                         float2* s_fx, int* s_fail, unsigned* s_mx) {
   R.D = D;
   R.g = g;
   R.G = D->G;
@@ -1955,6 +2153,8 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   R.xb = s_x + R.ipr * NP;
   R.fx = s_fx + R.wave * wave_scratch<NP, CWR>();
   R.fail = s_fail;
+  R.smx = s_mx;
+  if (threadIdx.x < 3) s_mx[threadIdx.x] = 0u;
   R.loc = false;
   R.fl = false;
   for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
@@ -1973,10 +2173,11 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   __shared__ float2 s_x[Cfg<NP>::XBUF];
   __shared__ float2 s_fx[Cfg<NP>::NW * wave_scratch<NP, 0>()];
   __shared__ int s_fail;
+  __shared__ unsigned s_mx[3];
   const int2 bm = bmap[blockIdx.x];
   if (bm.x < 0) return;
   Run R;
-  init_run<NP, Cfg<NP>::NT, 0>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
+  init_run<NP, Cfg<NP>::NT, 0>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
   run_iterations<NP, FAM, 0>(R, it0, it1, jb, je);
 }
 
@@ -1987,7 +2188,9 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
 // property of the launch size alone (nblocks <= co-resident capacity).
 #define BATCH_NT 512
 #define BATCH_MAX_NP 256
+#ifndef BATCH_CW
 #define BATCH_CW 1  // column width of the batched kernel (fewest registers)
+#endif
 
 template <int NP>
 struct BatchCfg {
@@ -1995,7 +2198,9 @@ struct BatchCfg {
   static constexpr int IPR = NW * Line<NP>::SUB;
   static constexpr int XROW = 2 * IPR * NP;
   static constexpr int XCOL = NW * NP * Col<NP, BATCH_CW>::CW / 2;
-  static constexpr int XBUF = XROW > XCOL ? XROW : XCOL;
+  static constexpr int XSTG = NW * Col<NP, BATCH_CW>::CW * (2 * NP + 32);  // staged column rounds
+  static constexpr int XRC = XROW > XCOL ? XROW : XCOL;
+  static constexpr int XBUF = XRC > XSTG ? XRC : XSTG;
 };
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int BATCH_XBUF =
@@ -2014,6 +2219,7 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ float2 s_x[BATCH_XBUF];
   __shared__ float2 s_fx[(BATCH_NT / 64) * BATCH_FX];
   __shared__ int s_fail;
+  __shared__ unsigned s_mx[3];
   __shared__ int s_blk;
   // The plan packs every cluster onto one XCD: bmap entry N_XCD * s + x is
   // slot s of XCD x.  A workgroup reads the XCD it runs on (XCC_ID hardware
@@ -2044,31 +2250,31 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   Run R;
   switch (D->a.Np) {
     case 16:
-      init_run<16, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
+      init_run<16, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
       R.loc = true;
       R.fl = D->fl && 16 / R.G >= R.ipr;
       run_iterations<16, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 32:
-      init_run<32, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
+      init_run<32, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
       R.loc = true;
       R.fl = D->fl && 32 / R.G >= R.ipr;
       run_iterations<32, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 64:
-      init_run<64, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
+      init_run<64, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
       R.loc = true;
       R.fl = D->fl && 64 / R.G >= R.ipr;
       run_iterations<64, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 128:
-      init_run<128, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
+      init_run<128, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
       R.loc = true;
       R.fl = D->fl && 128 / R.G >= R.ipr;
       run_iterations<128, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 256:
-      init_run<256, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail);
+      init_run<256, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
       R.loc = true;
       R.fl = D->fl && 256 / R.G >= R.ipr;
       run_iterations<256, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
@@ -2122,6 +2328,27 @@ __global__ void pty_live_columns_kernel(const float2* __restrict__ ht, unsigned 
     any |= (v.x != 0.f || v.y != 0.f);
   }
   live[i] = any;
+}
+
+// lgrp[t][m] = the m-th column group (cw adjacent columns) of transfer function
+// t holding a live column, nlive[1 + t] their number, nlive[0] the largest
+// such number; one thread per transfer function
+__global__ void pty_live_groups_kernel(const unsigned char* __restrict__ live, short* __restrict__ lgrp,
+                                       int* __restrict__ nlive, int np, int cw, int ntf) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int ng = np / cw;
+  int most = 0;
+  for (int t = 0; t < ntf; ++t) {
+    int n = 0;
+    for (int gi = 0; gi < ng; ++gi) {
+      int any = 0;
+      for (int c = 0; c < cw; ++c) any |= live[(size_t)t * np + gi * cw + c];
+      if (any) lgrp[(size_t)t * ng + n++] = (short)gi;
+    }
+    nlive[1 + t] = n;
+    most = n > most ? n : most;
+  }
+  nlive[0] = most;
 }
 
 __global__ void pty_permute_tf_kernel(const float2* __restrict__ tf, float2* __restrict__ ht, int np, int lg) {
@@ -2707,6 +2934,14 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
         pty_live_columns_kernel<<<(ntf * np + 255) / 256, 256, 0, st>>>(d.HT, (unsigned char*)mem, np, ntf);
         PLAN_TRY(hipGetLastError());
         d.live = (const unsigned char*)mem;
+        // column width of the kernel class the instance runs in
+        const int cw = batched ? col_width<BATCH_CW>(np) : col_width<0>(np);
+        if ((rc = dalloc(p, &mem, (size_t)ntf * (np / cw) * sizeof(short)))) goto fail;
+        d.lgrp = (const short*)mem;
+        if ((rc = dalloc(p, &mem, (size_t)(ntf + 1) * sizeof(int)))) goto fail;
+        d.nlive = (const int*)mem;
+        pty_live_groups_kernel<<<1, 64, 0, st>>>(d.live, (short*)d.lgrp, (int*)d.nlive, np, cw, ntf);
+        PLAN_TRY(hipGetLastError());
       }
     }
     if (a.n_iter > p->max_iter) p->max_iter = a.n_iter;
