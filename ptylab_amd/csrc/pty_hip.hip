@@ -278,10 +278,9 @@ __device__ __forceinline__ void publish_max2(const Run& R, unsigned* slot, float
 // Cross-workgroup data of the persistent kernel (exit waves, object, probe,
 // momenta, maxima, error partials) is only ever read and written with
 // coherent accesses: agent-scope relaxed atomics (global_load/store ... sc1)
-// in general, or, for a cluster on one XCD, plain stores and sc0 loads that
-// the XCD's L2 serves once the barrier has invalidated the CU's L1 (below);
-// no cache the readers use holds a stale copy.  A barrier then needs no L2
-// maintenance: every wave drains its
+// in general, or, for a cluster on one XCD, plain stores and sc1 loads that
+// the XCD's L2 serves (below); no cache the readers use holds a stale copy.
+// A barrier then needs no cache maintenance: every wave drains its
 // outstanding stores (s_waitcnt vmcnt(0): the stores have been performed at
 // the coherence point), the workgroup barrier orders that before the
 // workgroup's arrival, and once every workgroup has arrived the cluster
@@ -294,15 +293,16 @@ __device__ __forceinline__ void drain_stores() { __builtin_amdgcn_s_waitcnt(0x0F
 // XCD-local accessors.  When every workgroup of a cluster runs on one XCD,
 // cross-workgroup data need only be coherent in that XCD's L2: stores are
 // plain (the CU's L1 is write-through, so a drained store is in L2) and
-// loads are sc0 buffer loads.  An sc0 (workgroup-scope) load may still hit
-// the CU's L1, so the flag barrier invalidates the L1 (buffer_inv sc1, the
-// compiler's own acquire step for gfx942/gfx950) before any wave reads
-// again; without it Np = 32 clusters of 4 and 8 workgroups read stale
-// exit-wave lines (buffer_inv sc0 did not cure it).  The agent-scope forms
-// above instead miss in L2 as well.  The buffer resource is built from the
-// first active lane's address minus 2^31, so the 32-bit offsets of one
-// wave's accesses reach 2 GiB either side of it (a wave's accesses of a pass
-// lie within one field).
+// loads are sc1 (device-scope) buffer loads, which the L1 never serves and
+// the L2 does (about 215 cycles on a hit, against 108 for an L1 hit and 560
+// for a line from memory).  An sc0 load may hit a line the CU's L1 cached
+// before another workgroup rewrote it (Np = 32 clusters of 4 and 8
+// workgroups read stale exit-wave lines that way); the sc1 form needs no L1
+// invalidate at the barrier, which cost 0.8-1.1 us per barrier.  The buffer
+// resource is built from the first active lane's address minus 2^31, so the
+// 32-bit offsets of one wave's accesses reach 2 GiB either side of it (a
+// wave's accesses of a pass lie within one field).
+#define LOC_AUX 16  // cache policy of the XCD-local loads: sc1
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 struct LocalRef {
   __amdgpu_buffer_rsrc_t r;
@@ -333,7 +333,7 @@ __device__ __forceinline__ float load_max_published(const Run& R, const unsigned
   if (R.loc) {
     const LocalRef L = local_ref(p);
     for (int i = R.lane; i < R.G; i += 64)
-      m = fmaxf(m, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(L.r, L.off + (uint32_t)i * 4u, 0, 1)));
+      m = fmaxf(m, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(L.r, L.off + (uint32_t)i * 4u, 0, LOC_AUX)));
   } else {
     for (int i = R.lane; i < R.G; i += 64)
       m = fmaxf(m, __uint_as_float(__hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
@@ -381,11 +381,12 @@ __device__ __forceinline__ void counter_sync(Run& R) {
 // Flag barrier: workgroup g publishes the generation it has reached in its
 // own word D.bar[g] (a store, no atomic); wave 0 of every workgroup polls
 // the G words until all of them reach the generation.  Used by clusters on
-// one XCD: stores are plain and the polls volatile buffer loads, which the
-// compiler issues at system scope (sc0 sc1), so they neither hit a stale L1
-// line nor leave the loop.  Measured against the counter barrier above it is the
-// faster one within an XCD and the slower one across XCDs, where every poll
-// of G words goes past the L2s.
+// one XCD: stores are plain and the polls device-scope atomic loads (sc1),
+// which the XCD's L2 serves without the L1 and the compiler keeps inside the
+// loop.  (System-scope polls, sc0 sc1, took 1.7 us for one round of loads
+// where a word just written by other CUs was read.)  Measured against the
+// counter barrier above it is the faster one within an XCD and the slower
+// one across XCDs, where every poll of G words goes past the L2s.
 __device__ __forceinline__ void flag_sync(Run& R) {
   if (R.wave == 0) {
     const DevInst& D = *R.D;
@@ -395,13 +396,12 @@ __device__ __forceinline__ void flag_sync(Run& R) {
     if (R.lane == 0) {
       *(__attribute__((address_space(1))) unsigned*)(bar + R.g) = gen;
     }
-    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(bar, (short)0, R.G * 4, 0x00020000);
     unsigned spins = 0;
     int fail = 0;
     for (;;) {
       unsigned mn = 0xffffffffu;
       for (int g = R.lane; g < R.G; g += 64) {
-        const unsigned v = __builtin_amdgcn_raw_buffer_load_b32(rb, (uint32_t)g * 4u, 0, 1 | (int)(1u << 31));
+        const unsigned v = __hip_atomic_load(bar + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         mn = v < mn ? v : mn;
       }
       if (wave_min_u32(mn) >= gen) break;
@@ -445,10 +445,6 @@ __device__ __forceinline__ bool cluster_sync(Run& R) {
   R.gen += 1u;
   if (R.loc) {
     flag_sync(R);
-    // lines this CU cached before the barrier may be stale: invalidate the
-    // L1 (it is shared by the CU's waves) before the workgroup barrier
-    // releases the other waves
-    if (R.wave == 0) __asm__ volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
   } else {
     counter_sync(R);
   }
@@ -458,7 +454,7 @@ __device__ __forceinline__ bool cluster_sync(Run& R) {
 
 // run-state accessors: XCD-local or device-coherent (local_ref above)
 __device__ __forceinline__ float2 lload(const LocalRef& L, uint32_t byte_off) {
-  const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off + byte_off, 0, 1);
+  const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off + byte_off, 0, LOC_AUX);
   return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
 }
 __device__ __forceinline__ float2 cload(const Run& R, const float2* p) {
@@ -474,7 +470,7 @@ __device__ __forceinline__ void cstore(const Run& R, float2* p, float2 v) {
 __device__ __forceinline__ double cloadd(const Run& R, const double* p) {
   if (R.loc) {
     const LocalRef L = local_ref(p);
-    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off, 0, 1);
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(L.r, L.off, 0, LOC_AUX);
     return __builtin_bit_cast(double, w);
   }
   return cloadd(p);
