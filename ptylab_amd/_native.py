@@ -263,8 +263,10 @@ class Plan:
             self._h = None
         _live_plans.discard(self)
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # module globals (sys included) may already be None during
+        # interpreter finalisation, so the check is bound at definition
+        if not getattr(self, "_h", None) or _finalizing():
             return
         try:
             self.close()
