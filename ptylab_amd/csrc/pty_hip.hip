@@ -57,7 +57,11 @@ static_assert(sizeof(pty_instance) == 216, "pty_instance layout (ABI 4) changed:
 #define CTRL_BADPOS 3    // set when a position (rewritten by the caller between launches) left the object
 #define CTRL_GROUP 32    // counter barrier: 8 group arrival counters, 32 words apart
 #define CTRL_RELEASE 288 // counter barrier: published generation
-#define CTRL_WORDS 320
+#define CTRL_XPOP 320    // XCD barrier: workgroups of the cluster on each XCD (8 words)
+#define CTRL_XCTR 352    // XCD barrier: arrivals on each XCD, 32 words apart, counted in that XCD's L2
+#define CTRL_XTOP 608    // XCD barrier: arrivals of complete XCDs
+#define CTRL_XREL 640    // XCD barrier: published generation
+#define CTRL_WORDS 672
 #define SLOT_WORDS 32   // words of per-position maxima in each of the 4 ring slots
 #define SLOT_QN_OMAX 2   // qNewton's object maximum (words 0, 1: the first row pass's maxima)
 
@@ -185,6 +189,9 @@ struct Run {
   unsigned* smx;  // LDS: the workgroup's running maxima and arrival count of publish_max2
   bool loc;  // every workgroup of the cluster runs on one XCD: exchange through its L2
   bool fl;   // field-loop row passes (DevInst::fl, and at least one line per item slot)
+  bool xbar;  // XCD barrier (cluster spread over the XCDs; xcd_barrier_setup done)
+  unsigned xgen;  // XCD barrier: generation
+  unsigned xcd, xpop, nx;  // thread 0: its XCD, the cluster's workgroups there, non-empty XCDs
 };
 
 // ---------------------------------------------------------------------------
@@ -378,6 +385,47 @@ __device__ __forceinline__ void counter_sync(Run& R) {
   }
 }
 
+// XCD barrier, for a cluster spread over the XCDs: a workgroup arrives on
+// the counter of the XCD it runs on with an atomic performed in that XCD's L2
+// (workgroup scope: no round trip past the L2), the last arriver of an XCD
+// increments the device-scope top counter, and the last XCD to complete
+// publishes the generation on a release line that every workgroup polls.
+// Against the counter barrier above, one of the two serial device-scope
+// atomic round trips on the arrival path becomes an L2 atomic.  The
+// exchange data themselves are written and read with device-scope accesses
+// and drained before the arrival, so they are visible across XCDs once the
+// release is seen.  Each workgroup reads its XCD from the XCC_ID register;
+// the counts per XCD come from xcd_barrier_setup, so nothing depends on how
+// the dispatcher spreads the workgroups.
+__device__ __forceinline__ unsigned xcc_id() {
+  return (unsigned)__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u;
+}
+__device__ __forceinline__ void xcd_sync(Run& R) {
+  if (threadIdx.x == 0) {
+    unsigned* ctrl = R.D->ctrl;
+    const unsigned gen = R.xgen;
+    const unsigned old =
+        __hip_atomic_fetch_add(ctrl + CTRL_XCTR + 32 * R.xcd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old + 1u == gen * R.xpop) {
+      const unsigned t = __hip_atomic_fetch_add(ctrl + CTRL_XTOP, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (t + 1u == gen * R.nx) __hip_atomic_store(ctrl + CTRL_XREL, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    unsigned spins = 0;
+    int fail = 0;
+    while (__hip_atomic_load(ctrl + CTRL_XREL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
+      ++spins;
+      if ((spins & 255u) == 0u &&
+          (spins > SPIN_LIMIT || __hip_atomic_load(ctrl + CTRL_FAIL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(ctrl + CTRL_FAIL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *R.fail = fail;
+  }
+}
+
 // Flag barrier: workgroup g publishes the generation it has reached in its
 // own word D.bar[g] (a store, no atomic); wave 0 of every workgroup polls
 // the G words until all of them reach the generation.  Used by clusters on
@@ -445,6 +493,9 @@ __device__ __forceinline__ bool cluster_sync(Run& R) {
   R.gen += 1u;
   if (R.loc) {
     flag_sync(R);
+  } else if (R.xbar) {
+    R.xgen += 1u;
+    xcd_sync(R);
   } else {
     counter_sync(R);
   }
@@ -2125,6 +2176,30 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
   }
 }
 
+// Counts the cluster's workgroups on each XCD for the XCD barrier: every
+// workgroup adds itself to the count of its XCD, one counter barrier makes
+// the counts final, then thread 0 keeps its XCD's count and the number of
+// XCDs the cluster occupies.  Returns false when that barrier timed out.
+__device__ __forceinline__ bool xcd_barrier_setup(Run& R) {
+  unsigned* ctrl = R.D->ctrl;
+  if (threadIdx.x == 0) {
+    R.xcd = xcc_id();
+    __hip_atomic_fetch_add(ctrl + CTRL_XPOP + R.xcd, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (!cluster_sync(R)) return false;
+  if (threadIdx.x == 0) {
+    unsigned nx = 0u;
+    for (int x = 0; x < N_XCD; ++x) {
+      const unsigned c = __hip_atomic_load(ctrl + CTRL_XPOP + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      nx += c > 0u ? 1u : 0u;
+      if ((unsigned)x == R.xcd) R.xpop = c;
+    }
+    R.nx = nx;
+  }
+  R.xbar = true;
+  return true;
+}
+
 // Per-workgroup set-up for probe size NP on a workgroup of NT threads:
 // lane roles, LDS views, the twiddles of length NP.
 template <int NP, int NT, int CWR>
@@ -2153,6 +2228,9 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   if (threadIdx.x < 3) s_mx[threadIdx.x] = 0u;
   R.loc = false;
   R.fl = false;
+  R.xbar = false;
+  R.xgen = 0u;
+  R.xcd = R.xpop = R.nx = 0u;
   for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
 }
@@ -2174,6 +2252,7 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   if (bm.x < 0) return;
   Run R;
   init_run<NP, Cfg<NP>::NT, 0>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+  if (R.G > 1 && !xcd_barrier_setup(R)) return;
   run_iterations<NP, FAM, 0>(R, it0, it1, jb, je);
 }
 
