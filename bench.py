@@ -62,6 +62,9 @@ def parse_args():
     ap.add_argument("--li-stride", type=int, default=4,
                     help="learned_init: predict and stitch the patches of every n-th position (the fallback check "
                          "uses all of them)")
+    ap.add_argument("--li-known-probe", action="store_true",
+                    help="learned_init: both sides start mPIE from the known probe (the one the network's features "
+                         "use) instead of the reference's default 'circ' guess")
     ap.add_argument("--engine", choices=("mPIE", "ePIE"), default="mPIE", help="engine of the batched workload")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--gen-workers", type=int, default=16,
@@ -662,7 +665,7 @@ def main_learned_init(args):
     true = torch.as_tensor(w.object_true[0, 0, 0, 0], device="cuda")
     roi = torch.as_tensor(LI.roi_mask(w.positions, probe, w.No), device="cuda")
     max_iter = 60
-    thresholds = (1e-2, 0.12, 0.10, 0.09, 0.08, 0.07)
+    thresholds = (1e-2, 0.12, 0.10, 0.09, 0.08, 0.075, 0.07)
 
     def race(learned):
         data = P.ExperimentalData()
@@ -676,6 +679,8 @@ def main_learned_init(args):
         rec.No = w.No
         np.random.seed(1)
         rec.initializeObjectProbe()
+        if args.li_known_probe:
+            rec.probe = np.broadcast_to(probe.astype(np.complex64), rec.shape_P).copy()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         info = LI.initialize_object(rec, data, net, probe, stride=args.li_stride) if learned else None
@@ -724,6 +729,7 @@ def main_learned_init(args):
         "config": {
             "workload": "learned initialiser + mPIE: Np=128, No=512, 500 positions, Fraunhofer, 1e4 photons, "
                         "held-out object; network trained on %d simulated patterns" % (steps * net.B),
+            "initial_probe": "known (the network's)" if args.li_known_probe else "circ (reference default)",
             "train_seconds": t_train,
             "train_final_loss": float(np.mean(losses[-50:])),
             "init": info,
