@@ -50,10 +50,13 @@ def parse_args():
                          "larger than its position count)")
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--F", type=int, default=1000)
-    ap.add_argument("--workload", choices=("c1", "c0", "batched", "learned_init"), default="c1",
+    ap.add_argument("--workload", choices=("c1", "c0", "c2", "batched", "learned_init"), default="c1",
                     help="c1: single mPIE ASP reconstruction (default); c0: single plain ePIE Fraunhofer "
-                         "reconstruction (Np=64, 100 positions; microseconds per position); batched: 64 independent "
+                         "reconstruction (Np=64, 100 positions; microseconds per position); c2: single multislice "
+                         "reconstruction (four slices, Np=128, 400 positions); batched: 64 independent "
                          "reconstructions; learned_init: time to object NRMSE, learned-init vs plain mPIE")
+    ap.add_argument("--c2-engine", choices=("e3PIE", "multiPIE"), default="e3PIE",
+                    help="c2: the multislice engine (e3PIE: the slice chain; multiPIE: slices as independent axes)")
     ap.add_argument("--no-orthogonalization", action="store_true",
                     help="c1: skip the probe-mode orthogonalisation every 10 iterations")
     ap.add_argument("--cpu-procs", type=int, default=16,
@@ -171,6 +174,8 @@ def main():
         return main_learned_init(args)
     if args.workload == "c0":
         return main_c0(args)
+    if args.workload == "c2":
+        return main_c2(args)
     import torch
 
     import ptylab_amd as P
@@ -527,32 +532,35 @@ def main_batched(args):
     print(json.dumps(out))
 
 
-def main_c0(args):
-    """Plain ePIE, Fraunhofer, one reconstruction (the CPU-runnable
-    configuration: Np = 64, No = 256, 100 positions, 1e4 photons): a step is
-    one iteration (one launch); reported as microseconds per position and
-    as the (small) HBM fraction of a latency-bound single reconstruction."""
+def _single_run(args, w, engine, propagator, nslice, dz, bpos_fn, fpos, workload, oracle_iters, oracle_kw):
+    """One reconstruction of workload ``w`` with the package's engine: W
+    warm-up iterations, then K timed ones (one launch each, HIP events on the
+    launch stream); ``bpos_fn(momentum_fraction)`` gives the algorithmic
+    bytes of one position update.  The CPU baseline times the NumPy oracle
+    on the same inputs for ``oracle_iters`` iterations."""
     import torch
 
     import ptylab_amd as P
-    from ptylab_amd import synthetic as S
 
     if not torch.cuda.is_available():
         print("no HIP GPU available", file=sys.stderr)
         sys.exit(1)
-    w = S.cpm_fraunhofer(Np=64, No=256, F=100, seed=0)
     data = P.ExperimentalData()
     for k, v in w.as_dict().items():
         setattr(data, k, v)
     data._setData()
     params = P.Params()
-    params.propagatorType = "Fraunhofer"
+    params.propagatorType = propagator
     params.positionOrder = "random"
     rec = P.Reconstruction(data, params)
     rec.No = w.No
+    rec.nslice = nslice
+    if nslice > 1:
+        rec.dz = dz
+        rec.refrIndex = 1.0
     np.random.seed(3)
     rec.initializeObjectProbe()
-    eng = P.Engines.ePIE(rec, data, params, P.DummyMonitor())
+    eng = getattr(P.Engines, engine)(rec, data, params, P.DummyMonitor())
     eng.workgroups = args.workgroups
     K, W = args.steps, args.warmup
     eng.numIterations = W + K
@@ -576,11 +584,12 @@ def main_c0(args):
     plan.check()
     launch_ms = [a.elapsed_time(b) for a, b in ev]
     n_wg = plan.workgroups()
+    masks = getattr(eng, "_masks", None)
+    frac = float(masks[W:W + K].sum()) / (K * F) if masks is not None and engine in ("mPIE", "multiPIE") else 0.0
     eng._end(W + K)
-    bpos = bytes_per_position(64, w.No, 1, 1, 1, 1, 0.0)
+    bpos = bpos_fn(frac)
     mean_launch_s = float(np.mean(launch_ms)) / 1e3
     achieved = bpos * F / mean_launch_s / 1e9
-    fpos = fft_flops_per_position(64, 2)
     out = {
         "metric": METRIC,
         "value": K * F / elapsed,
@@ -595,7 +604,7 @@ def main_c0(args):
         "dtype": "c64",
         "data": "synthetic",
         "config": {
-            "workload": "ePIE, Fraunhofer, Np=64, No=256, F=100 positions per step (configuration 0)",
+            "workload": workload,
             "positions_per_step": F,
             "engine_workgroups": n_wg,
             "us_per_position": mean_launch_s / F * 1e6,
@@ -623,18 +632,58 @@ def main_c0(args):
     if not args.no_cpu_baseline:
         import oracle as O
 
-        geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No)
+        geo = O.Geometry(w.ptychogram, w.encoder, w.wavelength, w.dxd, w.zo, w.entrancePupilDiameter, No=w.No,
+                         nslice=nslice)
         np.random.seed(3)
         obj = O.initial_guess(geo.shape_O, "ones", geo)
         pr = O.initial_guess(geo.shape_P, "circ", geo)
-        run = O.OracleRun(geo, obj, pr, engine="ePIE", propagator="Fraunhofer", numIterations=20)
+        run = O.OracleRun(geo, obj, pr, engine=engine, propagator=propagator, numIterations=oracle_iters,
+                          dz=dz, **oracle_kw)
         t0 = time.perf_counter()
         run.reconstruct()
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": 20 * F / dt, "unit": "position updates/s", "cores": 1, "kind": "port",
-                               "sample": f"NumPy oracle, ePIE Fraunhofer Np=64 No=256: 20 iterations of {F} positions "
-                                         f"({20 * F} updates, {dt:.1f} s)", **host_info()}
+        out["cpu_baseline"] = {"value": oracle_iters * F / dt, "unit": "position updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"NumPy oracle, {engine} {propagator} Np={w.Np} No={w.No} nslice={nslice}: "
+                                         f"{oracle_iters} iteration(s) of {F} positions ({oracle_iters * F} updates, "
+                                         f"{dt:.1f} s)", **host_info()}
     print(json.dumps(out))
+
+
+def main_c0(args):
+    """Plain ePIE, Fraunhofer, one reconstruction (the CPU-runnable
+    configuration: Np = 64, No = 256, 100 positions, 1e4 photons): a step is
+    one iteration (one launch); reported as microseconds per position and
+    as the (small) HBM fraction of a latency-bound single reconstruction."""
+    from ptylab_amd import synthetic as S
+
+    w = S.cpm_fraunhofer(Np=64, No=256, F=100, seed=0)
+    _single_run(args, w, "ePIE", "Fraunhofer", 1, 0.0, lambda frac: bytes_per_position(64, w.No, 1, 1, 1, 1, 0.0),
+                fft_flops_per_position(64, 2),
+                "ePIE, Fraunhofer, Np=64, No=256, F=100 positions per step (configuration 0)", 20, {})
+
+
+def main_c2(args):
+    """Multislice configuration: four weak-phase slices 5 um apart (ASP
+    between slices), Np = 128, No = 512, 400 positions, Fraunhofer to the
+    detector, 1e4 photons with Poisson noise.  e3PIE runs the slice chain
+    (3 forward and 3 backward slice propagations plus the detector pair per
+    position: 14 transforms); multiPIE treats the slices as independent
+    broadcast axes (8 transforms, momentum at the mPIE rate).  A step is one
+    iteration of the 400 positions."""
+    from ptylab_amd import synthetic as S
+
+    Np, ns = 128, 4
+    w = S.cpm_multislice(Np=Np, No=512, F=400, nslice=ns, dz=5e-6, seed=20)
+    if args.c2_engine == "e3PIE":
+        n_fft = 2 + 4 * (ns - 1)
+        bfn = lambda frac: bytes_per_position(Np, w.No, 1, 1, 1, ns, 0.0)  # noqa: E731
+    else:
+        n_fft = 2 * ns
+        bfn = lambda frac: bytes_per_position(Np, w.No, 1, 1, 1, ns, frac)  # noqa: E731
+    _single_run(args, w, args.c2_engine, "Fraunhofer", ns, 5e-6, bfn, fft_flops_per_position(Np, n_fft),
+                f"{args.c2_engine}, multislice nslice=4 dz=5um, Fraunhofer, Np=128, No=512, F=400 positions per step "
+                "(configuration 2)", 2, {})
 
 
 def main_learned_init(args):
