@@ -129,6 +129,9 @@ struct DevInst {
   const unsigned char* live;  // ASP: per transfer function, 1 where column p of HT has a nonzero entry; or null
   const short* lgrp;  // ASP: per transfer function, the column groups (of the kernel's width) with a live column
   const int* nlive;   // [0]: most live groups of any transfer function, [1 + t]: those of function t; or null
+  const short* cix;   // ASP families without qNewton: per transfer function and column, where the column sits in
+                      // the packed exchange rows (see load_row_live), or null when W keeps full rows
+  int wst;            // row stride of the packed exchange rows (live groups x column width), when cix is set
   int fpl;           // W fields per transfer function (fields of one wavelength)
   float2* OP;        // qNewton: the object patch before the position's update (nO, Np, Np)
   unsigned* ctrl;    // control block, CTRL_* layout
@@ -584,6 +587,76 @@ __device__ __forceinline__ float2 ldg2(const float2* p) {
   return __builtin_bit_cast(float2, *(const GAS unsigned long long*)p);
 }
 
+// Row elements in columns that the column pass skipped (outside the band
+// limit of field f's transfer function) are zero.  live_bits reads the
+// column flags of this lane's elements (bit l for element l * LPF + t); it is
+// issued before the row loads so that its latency hides under theirs.
+template <int NP>
+__device__ __forceinline__ unsigned live_bits(const DevInst& D, int f, int t) {
+  if (!D.live) return ~0u;
+  const unsigned char* lv = D.live + (size_t)(f / D.fpl) * NP;
+  unsigned b = 0u;
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) b |= (lv[l * Line<NP>::LPF + t] ? 1u : 0u) << l;
+  return b;
+}
+template <int NP>
+__device__ __forceinline__ void zero_dead_columns(unsigned bits, float2 (&v)[Line<NP>::L]) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l)
+    if (!((bits >> l) & 1u)) v[l] = make_float2(0.f, 0.f);
+}
+
+// Packed exchange rows.  Between a row pass and a column pass of an
+// angular-spectrum propagation only the live column groups (those the
+// column passes transform, DevInst::lgrp) carry data; with DevInst::cix set
+// W holds just those, packed: row y of field f starts at W + f * NP * ws +
+// y * ws (ws = DevInst::wst), and the m-th live group of field f's transfer
+// function sits at columns [m * CW, (m + 1) * CW).  cix[p], per column p of
+// the row layout: the packed column when column p is live; -1 - the packed
+// column when p is dead inside a live group (stored, for the column pass,
+// but read as zero by the row passes, as zero_dead_columns does); PACK_DEAD
+// when p's group is dead (neither stored nor read).  A row pass thus moves
+// the live groups only (33 of 256 columns at the benchmark's band limit)
+// instead of full rows; the values the passes compute are unchanged.
+#define PACK_DEAD (-32768)
+__device__ __forceinline__ const short* packed_ix(const DevInst& D, int f) {
+  return D.cix ? D.cix + (size_t)(f / D.fpl) * D.a.Np : nullptr;
+}
+// row stride of the frequency-domain exchange rows of W
+template <int NP>
+__device__ __forceinline__ int xrow_stride(const DevInst& D) { return D.cix ? D.wst : NP; }
+// loads a frequency-domain row: full row with dead columns zeroed when ix is
+// null (lb from live_bits), else the live columns of the packed row
+template <int NP>
+__device__ __forceinline__ void load_row_live(const Run& R, float2 (&v)[Line<NP>::L], const float2* row,
+                                              const short* ix, unsigned lb, int t) {
+  if (!ix) {
+    load_row<NP>(R, v, row, t);
+    zero_dead_columns<NP>(lb, v);
+    return;
+  }
+  short c[Line<NP>::L];
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) c[l] = ix[l * Line<NP>::LPF + t];
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = c[l] >= 0 ? cload(R, row + c[l]) : make_float2(0.f, 0.f);
+}
+template <int NP>
+__device__ __forceinline__ void store_row_live(const Run& R, float2* row, const float2 (&v)[Line<NP>::L],
+                                               const short* ix, int t) {
+  if (!ix) {
+    store_row<NP>(R, row, v, t);
+    return;
+  }
+  short c[Line<NP>::L];
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) c[l] = ix[l * Line<NP>::LPF + t];
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l)
+    if (c[l] != PACK_DEAD) cstore(R, row + (c[l] >= 0 ? c[l] : -1 - c[l]), v[l]);
+}
+
 // ---------------------------------------------------------------------------
 // work decomposition
 // ---------------------------------------------------------------------------
@@ -692,7 +765,7 @@ __device__ __forceinline__ void lds_put(float2* buf, int islot, const float2 (&v
 // probe row, (checkerboard for the centred Fraunhofer transform), forward row
 // transform into W; per pixel sums of |P|^2 over the probe fields and of |O|^2
 // over the object fields give the maxima the updates divide by.
-template <int NP, bool QN, bool ASPF, bool QPH>
+template <int NP, bool QN, bool ASPF, bool QPH, bool PK>
 __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -734,7 +807,12 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       v[l] = e;
     }
     fft_fwd<NP>(v, R.tw, R.t, R.fx);
-    if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+    if constexpr (ASPF) {
+      const int ws = PK ? xrow_stride<NP>(D) : NP;
+      if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, (PK ? packed_ix(D, f) : nullptr), R.t);
+    } else {
+      if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+    }
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
       // fields outer, elements inner: the L reads of one field are in
@@ -799,20 +877,22 @@ template <int NP, int CWR>
 using ColVals = float2[Col<NP, CWR>::CL];
 
 template <int NP, int CWR>
-__device__ __forceinline__ void load_colg(const Run& R, ColVals<NP, CWR>& v, const float2* f, int p, int rho) {
+__device__ __forceinline__ void load_colg(const Run& R, ColVals<NP, CWR>& v, const float2* f, int p, int rho,
+                                          int rs = NP) {
   if (R.loc) {
-    const LocalRef L = local_ref(f + (size_t)rho * NP + p);
+    const LocalRef L = local_ref(f + (size_t)rho * rs + p);
 #pragma unroll
-    for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = lload(L, (uint32_t)(l * Col<NP, CWR>::LPC * NP * 8));
+    for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = lload(L, (uint32_t)(l * Col<NP, CWR>::LPC * rs * 8));
     return;
   }
 #pragma unroll
-  for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = cload(f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p);
+  for (int l = 0; l < Col<NP, CWR>::CL; ++l) v[l] = cload(f + (size_t)(l * Col<NP, CWR>::LPC + rho) * rs + p);
 }
 template <int NP, int CWR>
-__device__ __forceinline__ void store_colg(const Run& R, float2* f, const ColVals<NP, CWR>& v, int p, int rho) {
+__device__ __forceinline__ void store_colg(const Run& R, float2* f, const ColVals<NP, CWR>& v, int p, int rho,
+                                           int rs = NP) {
 #pragma unroll
-  for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(R, f + (size_t)(l * Col<NP, CWR>::LPC + rho) * NP + p, v[l]);
+  for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(R, f + (size_t)(l * Col<NP, CWR>::LPC + rho) * rs + p, v[l]);
 }
 template <int NP, int CWR>
 __device__ __forceinline__ void fft_fwd_colg(const Run& R, ColVals<NP, CWR>& v, int rho) {
@@ -1008,25 +1088,6 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
     detector_cols_fraunhofer_direct<NP, CWR>(R, k, j, ep, fs, de);
 }
 
-// Row elements in columns that the column pass skipped (outside the band
-// limit of field f's transfer function) are zero.  live_bits reads the
-// column flags of this lane's elements (bit l for element l * LPF + t); it is
-// issued before the row loads so that its latency hides under theirs.
-template <int NP>
-__device__ __forceinline__ unsigned live_bits(const DevInst& D, int f, int t) {
-  if (!D.live) return ~0u;
-  const unsigned char* lv = D.live + (size_t)(f / D.fpl) * NP;
-  unsigned b = 0u;
-#pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) b |= (lv[l * Line<NP>::LPF + t] ? 1u : 0u) << l;
-  return b;
-}
-template <int NP>
-__device__ __forceinline__ void zero_dead_columns(unsigned bits, float2 (&v)[Line<NP>::L]) {
-#pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l)
-    if (!((bits >> l) & 1u)) v[l] = make_float2(0.f, 0.f);
-}
 
 // Column pass of an angular-spectrum propagation: forward column transform,
 // multiply by the (bit-reversed, 1/Np^2-scaled) transfer function or its
@@ -1039,7 +1100,7 @@ __device__ __forceinline__ void zero_dead_columns(unsigned bits, float2 (&v)[Lin
 // band-limited function lie scattered in the bit-reversed column order, and
 // an equal share of them per workgroup keeps the slowest workgroup, which
 // the barrier after the pass waits for, from carrying several.
-template <int NP, int CWR>
+template <int NP, int CWR, bool PK>
 __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
   constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC, NG = Col<NP, CWR>::NG;
   const DevInst& D = *R.D;
@@ -1061,8 +1122,12 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
     const int p = cg * Col<NP, CWR>::CW + R.lane % Col<NP, CWR>::CW;
     // the transfer function of field f's wavelength
     const float2* __restrict__ HT = D.HT + (size_t)tf * Np2;
+    // packed exchange rows: the m-th live group sits at columns m * CW
+    const int ws = PK ? xrow_stride<NP>(D) : NP;
+    const int pw = PK && D.cix ? m * Col<NP, CWR>::CW + R.lane % Col<NP, CWR>::CW : p;
+    float2* const Wf = W + (size_t)f * NP * ws;
     float2 v[CL];
-    load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
+    load_colg<NP, CWR>(R, v, Wf, pw, rho, ws);
     fft_fwd_colg<NP, CWR>(R, v, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
@@ -1071,14 +1136,14 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
       v[l] = cmul(v[l], h);
     }
     fft_inv_colg<NP, CWR>(R, v, rho);
-    store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
+    store_colg<NP, CWR>(R, Wf, v, pw, rho, ws);
   }
 }
 
 // ASP detector row pass: inverse row transform gives the detector field, the
 // intensity constraint is applied in natural order, then the forward row
 // transform starts the back-propagation.
-template <int NP>
+template <int NP, bool PK>
 __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep, float* de) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -1086,13 +1151,15 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
   float2* __restrict__ W = D.W;
   const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
   const int nF = D.nF, fs = D.fs;
+  const int ws = PK ? xrow_stride<NP>(D) : NP;
   double err = 0.0;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     float2 v[L];
     float im[L];
-    const unsigned lb = live_bits<NP>(D, f, R.t);
-    load_row<NP>(R, v, W + f * Np2 + (size_t)r * NP, R.t);
-    zero_dead_columns<NP>(lb, v);
+    const short* ix = PK ? packed_ix(D, f) : nullptr;
+    const unsigned lb = ix ? ~0u : live_bits<NP>(D, f, R.t);
+    float2* const Wr = W + (size_t)f * NP * ws + (size_t)r * ws;
+    load_row_live<NP>(R, v, Wr, ix, lb, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
     fft_inv<NP>(v, R.tw, R.t, R.fx);
@@ -1117,7 +1184,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
       v[l] = cscale(v[l], sqrtf(im[l] / (iest[l] + 1e-10f)));
     }
     fft_fwd<NP>(v, R.tw, R.t, R.fx);
-    if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+    if (act) store_row_live<NP>(R, Wr, v, ix, R.t);
     ITEM_ROUND_SYNC();
   ITEMS_END
   const double s = block_sum(R, err);
@@ -1131,7 +1198,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
 // and each object / probe field is written by one owner slot (the field with
 // psm index 0, resp. osm index 0).  Each object pixel of the patch has one
 // owner.
-template <int NP, bool QN, bool ASPF, bool QPH>
+template <int NP, bool QN, bool ASPF, bool QPH, bool PK>
 __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot, bool sync_tail) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -1163,12 +1230,26 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], of[L], pf[L];
-    unsigned lb = ~0u;
-    if constexpr (ASPF) lb = live_bits<NP>(D, f, R.t);
-    load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
-    load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
-    load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
-    if constexpr (ASPF) zero_dead_columns<NP>(lb, d);
+    const short* ix = ASPF && PK ? packed_ix(D, f) : nullptr;
+    if (ix) {
+      // the packed frequency-domain row after the patch and probe rows,
+      // whose loads need no column table
+      const int ws = PK ? xrow_stride<NP>(D) : NP;
+      load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      load_row_live<NP>(R, d, W + (size_t)f * NP * ws + (size_t)r * ws, ix, ~0u, R.t);
+    } else if constexpr (ASPF) {
+      // the column flags are read before the row loads, whose latency hides theirs
+      const unsigned lb = live_bits<NP>(D, f, R.t);
+      load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+      load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      zero_dead_columns<NP>(lb, d);
+    } else {
+      load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+      load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+    }
     fft_inv<NP>(d, R.tw, R.t, R.fx);
     float2 tO[L], tP[L];
 #pragma unroll
@@ -1301,7 +1382,12 @@ __device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot,
         v[l] = e;
       }
       fft_fwd<NP>(v, R.tw, R.t, R.fx);
-      if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+      if constexpr (ASPF) {
+        const int ws = xrow_stride<NP>(D);
+        if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, packed_ix(D, f), R.t);
+      } else {
+        if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+      }
     }
     if (act) {
 #pragma unroll
@@ -1349,11 +1435,20 @@ __device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned
     for (int l = 0; l < L; ++l) accO[l] = make_float2(0.f, 0.f);
     for (int f = 0; f < nF; ++f) {
       float2 d[L], pf[L], nv[L];
-      unsigned lb = ~0u;
-      if constexpr (ASPF) lb = live_bits<NP>(D, f, R.t);
-      load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
-      load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
-      if constexpr (ASPF) zero_dead_columns<NP>(lb, d);
+      const short* ix = ASPF ? packed_ix(D, f) : nullptr;
+      if (ix) {
+        const int ws = xrow_stride<NP>(D);
+        load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
+        load_row_live<NP>(R, d, W + (size_t)f * NP * ws + (size_t)r * ws, ix, ~0u, R.t);
+      } else if constexpr (ASPF) {
+        const unsigned lb = live_bits<NP>(D, f, R.t);
+        load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+        load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
+        zero_dead_columns<NP>(lb, d);
+      } else {
+        load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+        load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
+      }
       fft_inv<NP>(d, R.tw, R.t, R.fx);
 #pragma unroll
       for (int l = 0; l < L; ++l) {
@@ -2007,14 +2102,14 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (FLK && R.fl)
     pie_rows_first_fl<NP, ASP, QPH>(R, k, slot, zslot);
   else
-    pie_rows_first<NP, QN, ASP, QPH>(R, k, slot, zslot);
+    pie_rows_first<NP, QN, ASP, QPH, BAT>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
-    tf_cols<NP, CWR>(R, false);
+    tf_cols<NP, CWR, BAT>(R, false);
     if (!cluster_sync(R)) return false;
-    asp_rows_constraint<NP>(R, k, j, ep, de);
+    asp_rows_constraint<NP, BAT>(R, k, j, ep, de);
     if (!cluster_sync(R)) return false;
-    tf_cols<NP, CWR>(R, true);
+    tf_cols<NP, CWR, BAT>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
     detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs, de);
@@ -2023,7 +2118,7 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (FLK && R.fl)
     pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
   else
-    pie_rows_update<NP, QN, ASP, QPH>(R, k, slot, !sync_after);
+    pie_rows_update<NP, QN, ASP, QPH, BAT>(R, k, slot, !sync_after);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
@@ -2067,7 +2162,7 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
   e3_rows_first<NP>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   for (int s = 1; s < ns; ++s) {
-    tf_cols<NP, CWR>(R, false);
+    tf_cols<NP, CWR, false>(R, false);
     if (!cluster_sync(R)) return false;
     e3_rows_forward<NP>(R, k, s, slot);
     if (!cluster_sync(R)) return false;
@@ -2077,7 +2172,7 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
   e3_rows_update<NP>(R, k, ns - 1, slot, true);
   if (!cluster_sync(R)) return false;
   for (int s = ns - 1; s >= 1; --s) {
-    tf_cols<NP, CWR>(R, true);
+    tf_cols<NP, CWR, false>(R, true);
     if (!cluster_sync(R)) return false;
     e3_rows_update<NP>(R, k, s - 1, slot, false);
     if (!cluster_sync(R)) return false;
@@ -2407,9 +2502,12 @@ __global__ void pty_live_columns_kernel(const float2* __restrict__ ht, unsigned 
 
 // lgrp[t][m] = the m-th column group (cw adjacent columns) of transfer function
 // t holding a live column, nlive[1 + t] their number, nlive[0] the largest
-// such number; one thread per transfer function
+// such number; with cix, the packed-row code of every column (load_row_live):
+// m * cw + c for column c of the m-th live group when that column is live,
+// -1 - (m * cw + c) when it is dead, PACK_DEAD for the columns of dead
+// groups; one thread per transfer function
 __global__ void pty_live_groups_kernel(const unsigned char* __restrict__ live, short* __restrict__ lgrp,
-                                       int* __restrict__ nlive, int np, int cw, int ntf) {
+                                       int* __restrict__ nlive, short* __restrict__ cix, int np, int cw, int ntf) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const int ng = np / cw;
   int most = 0;
@@ -2418,6 +2516,12 @@ __global__ void pty_live_groups_kernel(const unsigned char* __restrict__ live, s
     for (int gi = 0; gi < ng; ++gi) {
       int any = 0;
       for (int c = 0; c < cw; ++c) any |= live[(size_t)t * np + gi * cw + c];
+      if (cix) {
+        for (int c = 0; c < cw; ++c) {
+          const int pc = gi * cw + c, j = n * cw + c;
+          cix[(size_t)t * np + pc] = (short)(!any ? PACK_DEAD : live[(size_t)t * np + pc] ? j : -1 - j);
+        }
+      }
       if (any) lgrp[(size_t)t * ng + n++] = (short)gi;
     }
     nlive[1 + t] = n;
@@ -3019,8 +3123,24 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
         d.lgrp = (const short*)mem;
         if ((rc = dalloc(p, &mem, (size_t)(ntf + 1) * sizeof(int)))) goto fail;
         d.nlive = (const int*)mem;
-        pty_live_groups_kernel<<<1, 64, 0, st>>>(d.live, (short*)d.lgrp, (int*)d.nlive, np, cw, ntf);
+        // packed exchange rows for the PIE families in batched launches;
+        // qNewton's update pass writes full real-space rows into W that its
+        // probe pass reads.  (The per-size kernels keep full rows: there the
+        // column table's load before every row access lengthened the
+        // position, C1 29.9 -> 30.6 us, while the batched launch gained 22 %.)
+        const bool packed = batched && a.engine != PTY_QNEWTON && a.engine != PTY_MQNEWTON;
+        if (packed) {
+          if ((rc = dalloc(p, &mem, (size_t)ntf * np * sizeof(short)))) goto fail;
+          d.cix = (const short*)mem;
+        }
+        pty_live_groups_kernel<<<1, 64, 0, st>>>(d.live, (short*)d.lgrp, (int*)d.nlive, (short*)d.cix, np, cw, ntf);
         PLAN_TRY(hipGetLastError());
+        if (packed) {
+          int most = 0;
+          PLAN_TRY(hipStreamSynchronize(st));
+          PLAN_TRY(hipMemcpy(&most, d.nlive, sizeof(int), hipMemcpyDeviceToHost));
+          d.wst = most * cw;
+        }
       }
     }
     if (a.n_iter > p->max_iter) p->max_iter = a.n_iter;
