@@ -195,6 +195,7 @@ struct Run {
   bool xbar;  // XCD barrier (cluster spread over the XCDs; xcd_barrier_setup done)
   unsigned xgen;  // XCD barrier: generation
   unsigned xcd, xpop, nx;  // thread 0: its XCD, the cluster's workgroups there, non-empty XCDs
+  const short* lcix;  // LDS copy of DevInst::cix (per-size kernels), or null
 };
 
 // ---------------------------------------------------------------------------
@@ -620,7 +621,9 @@ __device__ __forceinline__ void zero_dead_columns(unsigned bits, float2 (&v)[Lin
 // the live groups only (33 of 256 columns at the benchmark's band limit)
 // instead of full rows; the values the passes compute are unchanged.
 #define PACK_DEAD (-32768)
-__device__ __forceinline__ const short* packed_ix(const DevInst& D, int f) {
+#define PACK_LDS 2048  // column codes a per-size workgroup keeps in LDS (transfer functions x Np)
+__device__ __forceinline__ const short* packed_ix(const Run& R, const DevInst& D, int f) {
+  if (R.lcix) return R.lcix + (size_t)(f / D.fpl) * D.a.Np;
   return D.cix ? D.cix + (size_t)(f / D.fpl) * D.a.Np : nullptr;
 }
 // row stride of the frequency-domain exchange rows of W
@@ -809,7 +812,7 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
     fft_fwd<NP>(v, R.tw, R.t, R.fx);
     if constexpr (ASPF) {
       const int ws = PK ? xrow_stride<NP>(D) : NP;
-      if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, (PK ? packed_ix(D, f) : nullptr), R.t);
+      if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, (PK ? packed_ix(R, D, f) : nullptr), R.t);
     } else {
       if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     }
@@ -1156,7 +1159,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     float2 v[L];
     float im[L];
-    const short* ix = PK ? packed_ix(D, f) : nullptr;
+    const short* ix = PK ? packed_ix(R, D, f) : nullptr;
     const unsigned lb = ix ? ~0u : live_bits<NP>(D, f, R.t);
     float2* const Wr = W + (size_t)f * NP * ws + (size_t)r * ws;
     load_row_live<NP>(R, v, Wr, ix, lb, R.t);
@@ -1230,7 +1233,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     const size_t orow_off = (size_t)(row0 + r) * No + col0;
     const size_t prow_off = (size_t)r * NP;
     float2 d[L], of[L], pf[L];
-    const short* ix = ASPF && PK ? packed_ix(D, f) : nullptr;
+    const short* ix = ASPF && PK ? packed_ix(R, D, f) : nullptr;
     if (ix) {
       // the packed frequency-domain row after the patch and probe rows,
       // whose loads need no column table
@@ -1384,7 +1387,7 @@ __device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot,
       fft_fwd<NP>(v, R.tw, R.t, R.fx);
       if constexpr (ASPF) {
         const int ws = xrow_stride<NP>(D);
-        if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, packed_ix(D, f), R.t);
+        if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, packed_ix(R, D, f), R.t);
       } else {
         if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
       }
@@ -1435,7 +1438,7 @@ __device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned
     for (int l = 0; l < L; ++l) accO[l] = make_float2(0.f, 0.f);
     for (int f = 0; f < nF; ++f) {
       float2 d[L], pf[L], nv[L];
-      const short* ix = ASPF ? packed_ix(D, f) : nullptr;
+      const short* ix = ASPF ? packed_ix(R, D, f) : nullptr;
       if (ix) {
         const int ws = xrow_stride<NP>(D);
         load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
@@ -2102,14 +2105,14 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (FLK && R.fl)
     pie_rows_first_fl<NP, ASP, QPH>(R, k, slot, zslot);
   else
-    pie_rows_first<NP, QN, ASP, QPH, BAT>(R, k, slot, zslot);
+    pie_rows_first<NP, QN, ASP, QPH, true>(R, k, slot, zslot);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
-    tf_cols<NP, CWR, BAT>(R, false);
+    tf_cols<NP, CWR, true>(R, false);
     if (!cluster_sync(R)) return false;
-    asp_rows_constraint<NP, BAT>(R, k, j, ep, de);
+    asp_rows_constraint<NP, true>(R, k, j, ep, de);
     if (!cluster_sync(R)) return false;
-    tf_cols<NP, CWR, BAT>(R, true);
+    tf_cols<NP, CWR, true>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
     detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs, de);
@@ -2118,7 +2121,7 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (FLK && R.fl)
     pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
   else
-    pie_rows_update<NP, QN, ASP, QPH, BAT>(R, k, slot, !sync_after);
+    pie_rows_update<NP, QN, ASP, QPH, true>(R, k, slot, !sync_after);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
@@ -2326,6 +2329,7 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   R.xbar = false;
   R.xgen = 0u;
   R.xcd = R.xpop = R.nx = 0u;
+  R.lcix = nullptr;
   for (int i = threadIdx.x; i < NP / 2; i += NT) s_tw[i] = g_twiddle[Line<NP>::LOG2 - 4][i];
   __syncthreads();
 }
@@ -2343,10 +2347,22 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   __shared__ float2 s_fx[Cfg<NP>::NW * wave_scratch<NP, 0>()];
   __shared__ int s_fail;
   __shared__ unsigned s_mx[3];
+  __shared__ short s_cix[PACK_LDS];
   const int2 bm = bmap[blockIdx.x];
   if (bm.x < 0) return;
   Run R;
   init_run<NP, Cfg<NP>::NT, 0>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+  {
+    // the packed-row column codes in LDS: the row passes look them up
+    // before every exchange-row access
+    const DevInst& D = *R.D;
+    const int n = D.cix ? (D.nF / D.fpl) * NP : 0;
+    if (n > 0 && n <= PACK_LDS) {
+      for (int i = threadIdx.x; i < n; i += Cfg<NP>::NT) s_cix[i] = D.cix[i];
+      __syncthreads();
+      R.lcix = s_cix;
+    }
+  }
   if (R.G > 1 && !xcd_barrier_setup(R)) return;
   run_iterations<NP, FAM, 0>(R, it0, it1, jb, je);
 }
@@ -3128,7 +3144,7 @@ int pty_plan_create(pty_plan** out, const pty_instance* insts, int32_t ninst, vo
         // probe pass reads.  (The per-size kernels keep full rows: there the
         // column table's load before every row access lengthened the
         // position, C1 29.9 -> 30.6 us, while the batched launch gained 22 %.)
-        const bool packed = batched && a.engine != PTY_QNEWTON && a.engine != PTY_MQNEWTON;
+        const bool packed = a.engine != PTY_QNEWTON && a.engine != PTY_MQNEWTON;
         if (packed) {
           if ((rc = dalloc(p, &mem, (size_t)ntf * np * sizeof(short)))) goto fail;
           d.cix = (const short*)mem;
