@@ -356,9 +356,13 @@ def _oracle_instance_sample(args):
 def cpu_baseline_all_cores(workloads, engine, propagator, npsm, procs, chunk=100, budget=8.0):
     """All-cores CPU throughput of the batch: `procs` forked processes, one
     instance each (the first `procs` instances), each running oracle
-    positions for about `budget` seconds; the aggregate is the sum of the
-    per-process rates (measured inside the workers, so process start-up is
-    excluded).  Runs before this process initialises the GPU (fork)."""
+    positions for about `budget` seconds, which gives every sampled
+    instance's seconds per position c_i with all cores busy (measured inside
+    the workers, so process start-up is excluded).  One iteration of the
+    sampled instances, spread evenly over the cores, takes sum F_i c_i /
+    procs, so the aggregate is procs * sum F_i / sum F_i c_i (a sum of the
+    per-process rates would overweight the cheap small-Np instances).  Runs
+    before this process initialises the GPU (fork)."""
     import multiprocessing as mp
 
     procs = max(1, min(procs, len(workloads)))
@@ -368,14 +372,17 @@ def cpu_baseline_all_cores(workloads, engine, propagator, npsm, procs, chunk=100
         res = pool.map(_oracle_instance_sample, jobs, chunksize=1)
     wall = time.perf_counter() - t0
     n = int(sum(r[0] for r in res))
+    Fs = [w.ptychogram.shape[0] for w in workloads[:procs]]
+    work = sum(F * r[1] / r[0] for F, r in zip(Fs, res))  # seconds of one iteration on one core
     return {
-        "value": float(sum(r[0] / r[1] for r in res)),
+        "value": float(procs * sum(Fs) / work),
         "unit": "position updates/s",
         "cores": procs,
         "kind": "port",
         "sample": f"NumPy oracle, {engine} {propagator} npsm={npsm}: {procs} concurrent processes, one batch instance "
                   f"each (the first {procs}), positions in chunks of at most {chunk} for about {budget:.0f} s each "
-                  f"({n} updates, {wall:.1f} s wall); value = sum of the per-process rates",
+                  f"({n} updates, {wall:.1f} s wall); value = cores x positions of one iteration of the sampled "
+                  f"instances / its single-core seconds at the measured per-position costs",
         **host_info(),
     }
 
