@@ -191,7 +191,18 @@ def measure(shape: Shape, G: int, positions: int = 48, seed: int = 0) -> float:
     T["energy"] = torch.full((F,), float(Np * Np), device=dev)
     T["err_at_pos"] = torch.zeros((n_iter, F), dtype=torch.float64, device=dev)
     T["err_iter"] = torch.zeros((n_iter,), dtype=torch.float64, device=dev)
-    if shape.propagator == _native.PTY_ASP or e3:
+    asp_like = shape.propagator in (_native.PTY_ASP, _native.PTY_POLYCHROME_ASP)
+    scaled = shape.propagator in (_native.PTY_SCALED_ASP, _native.PTY_SCALED_POLYCHROME_ASP)
+    if shape.propagator == _native.PTY_FRESNEL:
+        # Fresnel: the quadratic phase multiplied into the row passes
+        ph = torch.rand(Np, Np, generator=gen) * 6.283
+        T["tf"] = torch.polar(torch.ones(Np, Np), ph).to(torch.complex64).to(dev)
+    if scaled:
+        # scaled ASP: the transfer function Q2 and the phase c Q1 (one
+        # wavelength here: the fields count as probe modes)
+        ph = torch.rand(2, Np, Np, generator=gen) * 6.283
+        T["tf"] = torch.polar(torch.ones(2, Np, Np), ph).to(torch.complex64).to(dev)
+    if asp_like or e3:
         ph = torch.rand(Np, Np, generator=gen) * 6.283
         mag = torch.ones(Np, Np)
         if shape.band and not e3:

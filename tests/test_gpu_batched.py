@@ -206,3 +206,47 @@ def test_batched_field_loop_engines_equal_separate_runs(engine):
         ra, rb = a.reconstruction, b.reconstruction
         assert np.array_equal(np.asarray(ra.object), np.asarray(rb.object)), c
         assert np.array_equal(np.asarray(ra.probe), np.asarray(rb.probe)), c
+
+
+def test_batched_polychrome_and_scaled_asp_equal_separate_runs():
+    """Packed exchange rows with several transfer functions: polychromeASP
+    (three wavelengths, each with its own band limit and live columns),
+    ASP and scaledASP instances on the reference's regression inputs
+    (Np = 32, a band limit of a few columns) in one batched launch leave
+    object, probe and error bit-identical to separate (per-size) runs."""
+    import ptylab_amd as P
+    from regression_inputs import SEED, regression_dataset
+
+    d = regression_dataset()
+    cases = [("ePIE", "polychromeASP", (3, 1, 1, 1), 2), ("ePIE", "ASP", (1, 1, 1, 1), 3),
+             ("mPIE", "scaledASP", (1, 1, 1, 1), 3)]
+
+    def engines():
+        return [build_package(d, e, prop, modes, iters, SEED) for e, prop, modes, iters in cases]
+
+    sep = engines()
+    np.random.seed(80)
+    for e in sep:
+        e.reconstruct()
+    bat = engines()
+    np.random.seed(80)
+    P.Engines.Batched(bat).reconstruct()
+    for a, b, c in zip(sep, bat, cases):
+        ra, rb = a.reconstruction, b.reconstruction
+        print(f"batched {c[1]}: max |dO| {np.abs(np.asarray(ra.object) - np.asarray(rb.object)).max():.2e}", flush=True)
+        assert np.array_equal(np.asarray(ra.object), np.asarray(rb.object)), c
+        assert np.array_equal(np.asarray(ra.probe), np.asarray(rb.probe)), c
+        np.testing.assert_allclose(np.asarray(ra.error), np.asarray(rb.error), rtol=1e-12)
+
+
+@pytest.mark.parametrize("prop", ["Fresnel", "polychromeASP", "scaledASP", "scaledPolychromeASP"])
+def test_autotuner_measures_every_propagator(prop):
+    """The autotuner times a batched instance of every propagator family on
+    synthetic data (each with the transfer function or phase its plan needs)."""
+    import ptylab_amd as P
+    from ptylab_amd import autotune
+
+    pid = {"Fresnel": P._native.PTY_FRESNEL, "polychromeASP": P._native.PTY_POLYCHROME_ASP,
+           "scaledASP": P._native.PTY_SCALED_ASP, "scaledPolychromeASP": P._native.PTY_SCALED_POLYCHROME_ASP}[prop]
+    t = autotune.measure(autotune.Shape(P._native.PTY_EPIE, pid, 32, 128, 2), 2, positions=8)
+    assert np.isfinite(t) and t > 0
