@@ -316,9 +316,46 @@ __device__ __forceinline__ int tpart(int t, int k) {
 // instructions of one wave are performed in order, so the reads see the
 // other lanes' writes; the compiler barriers keep the accesses in program
 // order.
+// Swap of lane bit 5 (m = 32) or lane bit 4 (m = 16) with the register bit
+// that tells a from b: the gfx950 permlane swaps exchange the upper half of a
+// with the lower half of b (m = 32), or the odd rows of 16 lanes of a with the
+// even rows of b (m = 16), so afterwards a holds the values whose lane bit was
+// 0, with that lane bit now carrying the register bit, and b those whose lane
+// bit was 1.  Exact data movement.
+template <int M>
+__device__ __forceinline__ void lane_reg_swap(float2& a, float2& b) {
+  static_assert(M == 16 || M == 32, "lane bit 4 or 5");
+  const int ax = __float_as_int(a.x), ay = __float_as_int(a.y), bx = __float_as_int(b.x), by = __float_as_int(b.y);
+  if constexpr (M == 32) {
+    const auto px = __builtin_amdgcn_permlane32_swap(ax, bx, false, false);
+    const auto py = __builtin_amdgcn_permlane32_swap(ay, by, false, false);
+    a = make_float2(__int_as_float((int)px[0]), __int_as_float((int)py[0]));
+    b = make_float2(__int_as_float((int)px[1]), __int_as_float((int)py[1]));
+  } else {
+    const auto px = __builtin_amdgcn_permlane16_swap(ax, bx, false, false);
+    const auto py = __builtin_amdgcn_permlane16_swap(ay, by, false, false);
+    a = make_float2(__int_as_float((int)px[0]), __int_as_float((int)py[0]));
+    b = make_float2(__int_as_float((int)px[1]), __int_as_float((int)py[1]));
+  }
+}
+
 template <int N, int P, int LS, int K1, int K2>
 __device__ __forceinline__ void ttranspose(float2 (&v)[N / P], float2* __restrict__ sx, int t, int c) {
   using T = TPlan<N, P, LS>;
+  if constexpr (N == 256 && P == 64 && LS == 1 && ((K1 == 6 && K2 == 4) || (K1 == 4 && K2 == 6))) {
+    // Layouts 6 and 4 of a 256-point line on 64 lanes differ only in which of
+    // the element index bits 4, 5 and 6, 7 sit in the lane (bits 4, 5) and
+    // which in the register index (bits 0, 1): swap lane bit 5 with register
+    // bit 1 and lane bit 4 with register bit 0, in registers (no LDS).
+    lane_reg_swap<32>(v[0], v[2]);
+    lane_reg_swap<32>(v[1], v[3]);
+    lane_reg_swap<16>(v[0], v[1]);
+    lane_reg_swap<16>(v[2], v[3]);
+    (void)sx;
+    (void)t;
+    (void)c;
+    return;
+  }
   constexpr TPad d = tpad(N, P, LS, K1, K2);
   static_assert(d.s >= 0, "no transpose padding for this layout pair");
   float2* base = sx + c * d.stride;
