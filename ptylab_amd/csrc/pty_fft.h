@@ -418,7 +418,7 @@ __device__ __forceinline__ void tstages_inv(float2 (&v)[N / P], const float2* __
   }
 }
 
-template <int N, int P, int LS, int J>
+template <int N, int P, int LS, int J, bool Z>
 __device__ __forceinline__ void tfwd_from(float2 (&v)[N / P], const float2* __restrict__ tw, float2* __restrict__ sx,
                                           int t, int c) {
   using T = TPlan<N, P, LS>;
@@ -427,8 +427,8 @@ __device__ __forceinline__ void tfwd_from(float2 (&v)[N / P], const float2* __re
   tstages_fwd<N, P, LS, K>(v, tw, t, HI, K);
   if constexpr (J + 1 < T::NK) {
     ttranspose<N, P, LS, K, T::kseq(J + 1)>(v, sx, t, c);
-    tfwd_from<N, P, LS, J + 1>(v, tw, sx, t, c);
-  } else {
+    tfwd_from<N, P, LS, J + 1, Z>(v, tw, sx, t, c);
+  } else if constexpr (!Z) {
     ttranspose<N, P, LS, K, T::K0>(v, sx, t, c);
   }
 }
@@ -450,20 +450,34 @@ __device__ __forceinline__ void tinv_from(float2 (&v)[N / P], const float2* __re
 // fft_fwd / fft_inv for a line on P lanes (lane stride LS): the transposed
 // form where it is enabled (L >= 4 values per lane; `sx` then points to the
 // calling wave's TPlan::SCRATCH elements of LDS, `c` = lane % LS), the
-// cross-lane form otherwise.
-template <int N, int LPF, int LS>
+// cross-lane form otherwise.  With Z the frequency side (the forward
+// output, the inverse input) stays in the transposed form's last layout,
+// where register l of lane t holds slot zslot(l, t) of the line, and the
+// transpose between that layout and the caller's is skipped; without Z
+// both sides use the caller's layout, slot l * P + t.
+template <int N, int P, int LS>
+__device__ __forceinline__ int zslot(int l, int t) {
+  using T = TPlan<N, P, LS>;
+  if constexpr (T::ENABLED) {
+    static_assert(T::kseq(T::NK - 1) == 0, "the last layout of the transposed form is layout 0");
+    return (t << T::LG) | l;
+  } else {
+    return l * P + t;
+  }
+}
+template <int N, int LPF, int LS, bool Z = false>
 __device__ __forceinline__ void fft_fwd_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t,
                                           float2* __restrict__ sx, int c) {
   if constexpr (TPlan<N, LPF, LS>::ENABLED) {
     static_assert(tplan_ok<N, LPF, LS>(), "transpose paddings are not conflict-free");
     unsigned off = 0;
     __asm__ volatile("" : "+s"(off));
-    tfwd_from<N, LPF, LS, 0>(v, tw_ + off, sx, t, c);
+    tfwd_from<N, LPF, LS, 0, Z>(v, tw_ + off, sx, t, c);
   } else {
     fft_fwd_x<N, LPF, LS>(v, tw_, t);
   }
 }
-template <int N, int LPF, int LS>
+template <int N, int LPF, int LS, bool Z = false>
 __device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __restrict__ tw_, int t,
                                           float2* __restrict__ sx, int c) {
   if constexpr (TPlan<N, LPF, LS>::ENABLED) {
@@ -471,8 +485,8 @@ __device__ __forceinline__ void fft_inv_g(float2 (&v)[N / LPF], const float2* __
     using T = TPlan<N, LPF, LS>;
     unsigned off = 0;
     __asm__ volatile("" : "+s"(off));
-    // the caller's layout is K0; the inverse starts in the last layout
-    ttranspose<N, LPF, LS, T::K0, T::kseq(T::NK - 1)>(v, sx, t, c);
+    // the inverse starts in the last layout
+    if constexpr (!Z) ttranspose<N, LPF, LS, T::K0, T::kseq(T::NK - 1)>(v, sx, t, c);
     tinv_from<N, LPF, LS, T::NK - 1>(v, tw_ + off, sx, t, c);
   } else {
     fft_inv_x<N, LPF, LS>(v, tw_, t);
@@ -491,15 +505,18 @@ constexpr int fft_scratch() {
   return m > 0 ? m : 1;
 }
 
-// row layout: a line on LPF adjacent lanes
-template <int N>
+// row layout: a line on LPF adjacent lanes (Z: frequency side in slots
+// rslot(l, t), see zslot)
+template <int N, bool Z = false>
 __device__ __forceinline__ void fft_fwd(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t, float2* sx) {
-  fft_fwd_g<N, Line<N>::LPF, 1>(v, tw, t, sx, 0);
+  fft_fwd_g<N, Line<N>::LPF, 1, Z>(v, tw, t, sx, 0);
+}
+template <int N, bool Z = false>
+__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t, float2* sx) {
+  fft_inv_g<N, Line<N>::LPF, 1, Z>(v, tw, t, sx, 0);
 }
 template <int N>
-__device__ __forceinline__ void fft_inv(float2 (&v)[Line<N>::L], const float2* __restrict__ tw, int t, float2* sx) {
-  fft_inv_g<N, Line<N>::LPF, 1>(v, tw, t, sx, 0);
-}
+__device__ __forceinline__ int rslot(int l, int t) { return zslot<N, Line<N>::LPF, 1>(l, t); }
 
 __device__ __forceinline__ int brev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }
 

@@ -582,6 +582,26 @@ __device__ __forceinline__ void store_col(float2* f, const float2 (&v)[Line<NP>:
 #pragma unroll
   for (int l = 0; l < Line<NP>::L; ++l) cstore(f + (size_t)(l * Line<NP>::LPF + t) * NP + p, v[l]);
 }
+// Frequency-side rows: the forward row transform leaves, and the inverse
+// takes, register l of lane t at slot rslot(l, t) of the row (the transposed
+// form's last layout; fft_fwd / fft_inv with Z); storage keeps slot order.
+template <int NP>
+__device__ __forceinline__ void load_row_f(const Run& R, float2 (&v)[Line<NP>::L], const float2* row, int t) {
+  if (R.loc) {
+    const int s0 = rslot<NP>(0, t);
+    const LocalRef L = local_ref(row + s0);
+#pragma unroll
+    for (int l = 0; l < Line<NP>::L; ++l) v[l] = lload(L, (uint32_t)((rslot<NP>(l, t) - s0) * 8));
+    return;
+  }
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) v[l] = cload(row + rslot<NP>(l, t));
+}
+template <int NP>
+__device__ __forceinline__ void store_row_f(const Run& R, float2* row, const float2 (&v)[Line<NP>::L], int t) {
+#pragma unroll
+  for (int l = 0; l < Line<NP>::L; ++l) cstore(R, row + rslot<NP>(l, t), v[l]);
+}
 // read-only inputs (measured frames, transfer function): ordinary cached loads
 __device__ __forceinline__ float ldg(const float* p) { return *(const GAS float*)p; }
 __device__ __forceinline__ float2 ldg2(const float2* p) {
@@ -590,15 +610,16 @@ __device__ __forceinline__ float2 ldg2(const float2* p) {
 
 // Row elements in columns that the column pass skipped (outside the band
 // limit of field f's transfer function) are zero.  live_bits reads the
-// column flags of this lane's elements (bit l for element l * LPF + t); it is
-// issued before the row loads so that its latency hides under theirs.
+// column flags of this lane's frequency-side elements (bit l for slot
+// rslot(l, t)); it is issued before the row loads so that its latency hides
+// under theirs.
 template <int NP>
 __device__ __forceinline__ unsigned live_bits(const DevInst& D, int f, int t) {
   if (!D.live) return ~0u;
   const unsigned char* lv = D.live + (size_t)(f / D.fpl) * NP;
   unsigned b = 0u;
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) b |= (lv[l * Line<NP>::LPF + t] ? 1u : 0u) << l;
+  for (int l = 0; l < Line<NP>::L; ++l) b |= (lv[rslot<NP>(l, t)] ? 1u : 0u) << l;
   return b;
 }
 template <int NP>
@@ -635,13 +656,13 @@ template <int NP>
 __device__ __forceinline__ void load_row_live(const Run& R, float2 (&v)[Line<NP>::L], const float2* row,
                                               const short* ix, unsigned lb, int t) {
   if (!ix) {
-    load_row<NP>(R, v, row, t);
+    load_row_f<NP>(R, v, row, t);
     zero_dead_columns<NP>(lb, v);
     return;
   }
   short c[Line<NP>::L];
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) c[l] = ix[l * Line<NP>::LPF + t];
+  for (int l = 0; l < Line<NP>::L; ++l) c[l] = ix[rslot<NP>(l, t)];
 #pragma unroll
   for (int l = 0; l < Line<NP>::L; ++l) v[l] = c[l] >= 0 ? cload(R, row + c[l]) : make_float2(0.f, 0.f);
 }
@@ -649,12 +670,12 @@ template <int NP>
 __device__ __forceinline__ void store_row_live(const Run& R, float2* row, const float2 (&v)[Line<NP>::L],
                                                const short* ix, int t) {
   if (!ix) {
-    store_row<NP>(R, row, v, t);
+    store_row_f<NP>(R, row, v, t);
     return;
   }
   short c[Line<NP>::L];
 #pragma unroll
-  for (int l = 0; l < Line<NP>::L; ++l) c[l] = ix[l * Line<NP>::LPF + t];
+  for (int l = 0; l < Line<NP>::L; ++l) c[l] = ix[rslot<NP>(l, t)];
 #pragma unroll
   for (int l = 0; l < Line<NP>::L; ++l)
     if (c[l] != PACK_DEAD) cstore(R, row + (c[l] >= 0 ? c[l] : -1 - c[l]), v[l]);
@@ -809,12 +830,12 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
       v[l] = e;
     }
-    fft_fwd<NP>(v, R.tw, R.t, R.fx);
+    fft_fwd<NP, true>(v, R.tw, R.t, R.fx);
     if constexpr (ASPF) {
       const int ws = PK ? xrow_stride<NP>(D) : NP;
       if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, (PK ? packed_ix(R, D, f) : nullptr), R.t);
     } else {
-      if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+      if (act) store_row_f<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
     }
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
@@ -897,13 +918,19 @@ __device__ __forceinline__ void store_colg(const Run& R, float2* f, const ColVal
 #pragma unroll
   for (int l = 0; l < Col<NP, CWR>::CL; ++l) cstore(R, f + (size_t)(l * Col<NP, CWR>::LPC + rho) * rs + p, v[l]);
 }
-template <int NP, int CWR>
+template <int NP, int CWR, bool Z = false>
 __device__ __forceinline__ void fft_fwd_colg(const Run& R, ColVals<NP, CWR>& v, int rho) {
-  fft_fwd_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, R.tw, rho, R.fx, R.lane % Col<NP, CWR>::CW);
+  fft_fwd_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW, Z>(v, R.tw, rho, R.fx, R.lane % Col<NP, CWR>::CW);
 }
-template <int NP, int CWR>
+template <int NP, int CWR, bool Z = false>
 __device__ __forceinline__ void fft_inv_colg(const Run& R, ColVals<NP, CWR>& v, int rho) {
-  fft_inv_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(v, R.tw, rho, R.fx, R.lane % Col<NP, CWR>::CW);
+  fft_inv_g<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW, Z>(v, R.tw, rho, R.fx, R.lane % Col<NP, CWR>::CW);
+}
+// row (of the column line) that register l of lane rho holds on the
+// frequency side of fft_fwd_colg / fft_inv_colg with Z
+template <int NP, int CWR>
+__device__ __forceinline__ int cslot(int l, int rho) {
+  return zslot<NP, Col<NP, CWR>::LPC, Col<NP, CWR>::CW>(l, rho);
 }
 
 // LDS scratch per wave of the line transforms of probe size NP (rows and
@@ -930,7 +957,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_direct(Run& R, int k, i
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     float2 v[CL];
     load_colg<NP, CWR>(R, v, W + f * Np2, p, rho);
-    fft_fwd_colg<NP, CWR>(R, v, rho);
+    fft_fwd_colg<NP, CWR, true>(R, v, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
       v[l] = cscale(v[l], invN);
@@ -946,7 +973,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_direct(Run& R, int k, i
     }
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
-      const int n = l * LPC + rho;
+      const int n = cslot<NP, CWR>(l, rho);
       const float im = ldg(Ik + (size_t)n * NP + p);
       if (ITEM_LEAD(act, frw)) {
         err += (double)fabsf(im - iest[l]);
@@ -956,7 +983,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_direct(Run& R, int k, i
       }
       v[l] = cscale(v[l], sqrtf(im / (iest[l] + 1e-10f)));
     }
-    fft_inv_colg<NP, CWR>(R, v, rho);
+    fft_inv_colg<NP, CWR, true>(R, v, rho);
     if (act) store_colg<NP, CWR>(R, W + f * Np2, v, p, rho);
     CITEM_ROUND_SYNC();
   CITEMS_END
@@ -1034,7 +1061,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
     float2 v[CL];
 #pragma unroll
     for (int l = 0; l < CL; ++l) v[l] = tile[f * TS + q * cs + l * LPC + rho];
-    fft_fwd_colg<NP, CWR>(R, v, rho);
+    fft_fwd_colg<NP, CWR, true>(R, v, rho);
     float iest[CL];
 #pragma unroll
     for (int l = 0; l < CL; ++l) v[l] = cscale(v[l], invN);
@@ -1055,7 +1082,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
     }
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
-      const int n = l * LPC + rho;
+      const int n = cslot<NP, CWR>(l, rho);
       const float im = ikt[q * csi + n];
       if (ITEM_LEAD(act, frw)) {
         err += (double)fabsf(im - iest[l]);
@@ -1065,7 +1092,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
       }
       v[l] = cscale(v[l], sqrtf(im / (iest[l] + 1e-10f)));
     }
-    fft_inv_colg<NP, CWR>(R, v, rho);
+    fft_inv_colg<NP, CWR, true>(R, v, rho);
     if (act) {
 #pragma unroll
       for (int l = 0; l < CL; ++l) tile[frw * TS + q * cs + l * LPC + rho] = v[l];
@@ -1131,14 +1158,14 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
     float2* const Wf = W + (size_t)f * NP * ws;
     float2 v[CL];
     load_colg<NP, CWR>(R, v, Wf, pw, rho, ws);
-    fft_fwd_colg<NP, CWR>(R, v, rho);
+    fft_fwd_colg<NP, CWR, true>(R, v, rho);
 #pragma unroll
     for (int l = 0; l < CL; ++l) {
-      float2 h = ldg2(HT + (size_t)(l * LPC + rho) * NP + p);
+      float2 h = ldg2(HT + (size_t)cslot<NP, CWR>(l, rho) * NP + p);
       if (conj) h = conjf2(h);
       v[l] = cmul(v[l], h);
     }
-    fft_inv_colg<NP, CWR>(R, v, rho);
+    fft_inv_colg<NP, CWR, true>(R, v, rho);
     store_colg<NP, CWR>(R, Wf, v, pw, rho, ws);
   }
 }
@@ -1165,7 +1192,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     load_row_live<NP>(R, v, Wr, ix, lb, R.t);
 #pragma unroll
     for (int l = 0; l < L; ++l) im[l] = ldg(Ik + (size_t)r * NP + l * LPF + R.t);
-    fft_inv<NP>(v, R.tw, R.t, R.fx);
+    fft_inv<NP, true>(v, R.tw, R.t, R.fx);
     float2 q[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) q[l] = make_float2(act && (f % fs == fs - 1) ? cabs2(v[l]) : 0.f, 0.f);
@@ -1186,7 +1213,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
       }
       v[l] = cscale(v[l], sqrtf(im[l] / (iest[l] + 1e-10f)));
     }
-    fft_fwd<NP>(v, R.tw, R.t, R.fx);
+    fft_fwd<NP, true>(v, R.tw, R.t, R.fx);
     if (act) store_row_live<NP>(R, Wr, v, ix, R.t);
     ITEM_ROUND_SYNC();
   ITEMS_END
@@ -1244,16 +1271,16 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
     } else if constexpr (ASPF) {
       // the column flags are read before the row loads, whose latency hides theirs
       const unsigned lb = live_bits<NP>(D, f, R.t);
-      load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+      load_row_f<NP>(R, d, W + f * Np2 + prow_off, R.t);
       load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
       load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
       zero_dead_columns<NP>(lb, d);
     } else {
-      load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+      load_row_f<NP>(R, d, W + f * Np2 + prow_off, R.t);
       load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
       load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
     }
-    fft_inv<NP>(d, R.tw, R.t, R.fx);
+    fft_inv<NP, true>(d, R.tw, R.t, R.fx);
     float2 tO[L], tP[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -1384,12 +1411,12 @@ __device__ __forceinline__ void pie_rows_first_fl(Run& R, int k, unsigned* slot,
         if (cb && ((r + l * LPF + R.t) & 1)) e = make_float2(-e.x, -e.y);
         v[l] = e;
       }
-      fft_fwd<NP>(v, R.tw, R.t, R.fx);
+      fft_fwd<NP, true>(v, R.tw, R.t, R.fx);
       if constexpr (ASPF) {
         const int ws = xrow_stride<NP>(D);
         if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, packed_ix(R, D, f), R.t);
       } else {
-        if (act) store_row<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+        if (act) store_row_f<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
       }
     }
     if (act) {
@@ -1445,14 +1472,14 @@ __device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned
         load_row_live<NP>(R, d, W + (size_t)f * NP * ws + (size_t)r * ws, ix, ~0u, R.t);
       } else if constexpr (ASPF) {
         const unsigned lb = live_bits<NP>(D, f, R.t);
-        load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+        load_row_f<NP>(R, d, W + f * Np2 + prow_off, R.t);
         load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
         zero_dead_columns<NP>(lb, d);
       } else {
-        load_row<NP>(R, d, W + f * Np2 + prow_off, R.t);
+        load_row_f<NP>(R, d, W + f * Np2 + prow_off, R.t);
         load_row<NP>(R, pf, P + f * Np2 + prow_off, R.t);
       }
-      fft_inv<NP>(d, R.tw, R.t, R.fx);
+      fft_inv<NP, true>(d, R.tw, R.t, R.fx);
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         float2 u = d[l];
