@@ -816,7 +816,9 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       const float pv = qn ? sqrtf(cabs2(pf[l])) : cabs2(pf[l]);
       q[l] = make_float2(act && x.io == 0 ? pv : 0.f, act && x.ip == 0 ? cabs2(of[l]) : 0.f);
     }
-    lds_put<NP>(R.xa, islot, q, R.t);
+    // one field: its terms are the field sums, no LDS exchange
+    const bool one = nF == 1;
+    if (!one) lds_put<NP>(R.xa, islot, q, R.t);
     float2 v[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -836,6 +838,16 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
       if (act) store_row_live<NP>(R, W + (size_t)f * NP * ws + (size_t)r * ws, v, (PK ? packed_ix(R, D, f) : nullptr), R.t);
     } else {
       if (act) store_row_f<NP>(R, W + f * Np2 + (size_t)r * NP, v, R.t);
+    }
+    if (one) {
+      if (act) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          pmax = fmaxf(pmax, 0.f + q[l].x);
+          omax = fmaxf(omax, 0.f + q[l].y);
+        }
+      }
+      continue;
     }
     __syncthreads();
     if (ITEM_LEAD(act, frw)) {
@@ -1318,9 +1330,25 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       tO[l] = cmul(make_float2(pf[l].x * wo, -pf[l].y * wo), dl);
       tP[l] = cmul(make_float2(of[l].x * wp, -of[l].y * wp), dl);
     }
+    constexpr bool qn = QN;
+    if (!qn && nF == 1 && (osel & psel & 1u)) {
+      // one field: its own terms are the sums of the LDS form (0 + x), no
+      // exchange and no workgroup barrier
+      if (act) {
+        float2 nv[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l) nv[l] = cadd(of[l], cscale(cadd(make_float2(0.f, 0.f), tO[l]), bO));
+        if (D.pcpart) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
+        store_row<NP>(R, O + x.fo * No2 + orow_off, nv, R.t);
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          nv[l] = cadd(pf[l], cscale(cadd(make_float2(0.f, 0.f), tP[l]), engine == PTY_EPIE ? D.a.betaProbe : stepP));
+        store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
+      }
+      continue;
+    }
     lds_put<NP>(R.xa, islot, tO, R.t);
     lds_put<NP>(R.xb, islot, tP, R.t);
-    constexpr bool qn = QN;
     // qNewton keeps the difference for its probe step; only this item
     // touches row r of field f in this pass
     if (qn && act) store_row<NP>(R, W + f * Np2 + prow_off, d, R.t);
