@@ -1348,7 +1348,20 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       continue;
     }
     lds_put<NP>(R.xa, islot, tO, R.t);
-    lds_put<NP>(R.xb, islot, tP, R.t);
+    // a probe field that takes its own term alone (mPIE with one object
+    // mode: the sum runs over osm) is written at once, without the exchange
+    const bool pown = !qn && psel == (1u << f);
+    if (pown) {
+      if (act && x.io == 0) {
+        float2 nv[L];
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          nv[l] = cadd(pf[l], cscale(cadd(make_float2(0.f, 0.f), tP[l]), engine == PTY_EPIE ? D.a.betaProbe : stepP));
+        store_row<NP>(R, P + x.fp * Np2 + prow_off, nv, R.t);
+      }
+    } else {
+      lds_put<NP>(R.xb, islot, tP, R.t);
+    }
     // qNewton keeps the difference for its probe step; only this item
     // touches row r of field f in this pass
     if (qn && act) store_row<NP>(R, W + f * Np2 + prow_off, d, R.t);
@@ -1371,7 +1384,7 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       if (qn || D.pcpart) store_row<NP>(R, D.OP + x.fo * Np2 + prow_off, of, R.t);
       store_row<NP>(R, O + x.fo * No2 + orow_off, nv, R.t);
     }
-    if (act && x.io == 0 && !qn) {
+    if (act && x.io == 0 && !qn && !pown) {
       // probe field x.fp: ePIE sums over (lambda, osm, slice), mPIE over osm,
       // multiPIE over (lambda, osm)
       float2 nv[L], acc[L];
