@@ -789,8 +789,9 @@ __device__ __forceinline__ void lds_put(float2* buf, int islot, const float2 (&v
 // probe row, (checkerboard for the centred Fraunhofer transform), forward row
 // transform into W; per pixel sums of |P|^2 over the probe fields and of |O|^2
 // over the object fields give the maxima the updates divide by.
-template <int NP, bool QN, bool ASPF, bool QPH, bool PK>
-__device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot) {
+template <int NP, bool QN, bool ASPF, bool QPH, bool PK, bool KEEP = false>
+__device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, unsigned* zslot,
+                                               float2 (&ko)[Line<NP>::L], float2 (&kp)[Line<NP>::L]) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
@@ -810,6 +811,15 @@ __device__ __forceinline__ void pie_rows_first(Run& R, int k, unsigned* slot, un
     float2 of[L], pf[L], q[L];
     load_row<NP>(R, of, O + x.fo * No2 + (size_t)(row0 + r) * No + col0, R.t);
     load_row<NP>(R, pf, P + x.fp * Np2 + (size_t)r * NP, R.t);
+    if constexpr (KEEP) {
+      // the last row pass of this position reuses this item's patch and
+      // probe rows (no pass in between writes them)
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        ko[l] = of[l];
+        kp[l] = pf[l];
+      }
+    }
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       // qNewton.py:114: Pmax = max(sum |P|); the PIE engines take sum |P|^2
@@ -1240,8 +1250,9 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
 // and each object / probe field is written by one owner slot (the field with
 // psm index 0, resp. osm index 0).  Each object pixel of the patch has one
 // owner.
-template <int NP, bool QN, bool ASPF, bool QPH, bool PK>
-__device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot, bool sync_tail) {
+template <int NP, bool QN, bool ASPF, bool QPH, bool PK, bool KEEP = false>
+__device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot, bool sync_tail,
+                                                const float2 (&ko)[Line<NP>::L], const float2 (&kp)[Line<NP>::L]) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const int no = D.a.nosm, npm = D.a.npsm, ns = D.a.nslice;
@@ -1277,20 +1288,44 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
       // the packed frequency-domain row after the patch and probe rows,
       // whose loads need no column table
       const int ws = PK ? xrow_stride<NP>(D) : NP;
-      load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
-      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      if constexpr (KEEP) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          of[l] = ko[l];
+          pf[l] = kp[l];
+        }
+      } else {
+        load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+        load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      }
       load_row_live<NP>(R, d, W + (size_t)f * NP * ws + (size_t)r * ws, ix, ~0u, R.t);
     } else if constexpr (ASPF) {
       // the column flags are read before the row loads, whose latency hides theirs
       const unsigned lb = live_bits<NP>(D, f, R.t);
       load_row_f<NP>(R, d, W + f * Np2 + prow_off, R.t);
-      load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
-      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      if constexpr (KEEP) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          of[l] = ko[l];
+          pf[l] = kp[l];
+        }
+      } else {
+        load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+        load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      }
       zero_dead_columns<NP>(lb, d);
     } else {
       load_row_f<NP>(R, d, W + f * Np2 + prow_off, R.t);
-      load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
-      load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      if constexpr (KEEP) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          of[l] = ko[l];
+          pf[l] = kp[l];
+        }
+      } else {
+        load_row<NP>(R, of, O + x.fo * No2 + orow_off, R.t);
+        load_row<NP>(R, pf, P + x.fp * Np2 + prow_off, R.t);
+      }
     }
     fft_inv<NP, true>(d, R.tw, R.t, R.fx);
     float2 tO[L], tP[L];
@@ -2170,10 +2205,17 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   // the Fraunhofer family spilled more registers and lost 6 % on one-mode
   // instances, which have no field sums to save
   constexpr bool FLK = BAT && ASP && !QN && !PC;
+  // per-size kernels whose waves own one (line, field) item per row pass
+  // keep the item's patch and probe rows in registers from the first row
+  // pass to the last (the batched kernel has no registers to spare)
+  float2 ko[Line<NP>::L], kp[Line<NP>::L];
+  const bool keep = !BAT && NP / R.G <= R.ipr / R.nFp;
   if (FLK && R.fl)
     pie_rows_first_fl<NP, ASP, QPH>(R, k, slot, zslot);
+  else if (!BAT && keep)
+    pie_rows_first<NP, QN, ASP, QPH, true, !BAT>(R, k, slot, zslot, ko, kp);
   else
-    pie_rows_first<NP, QN, ASP, QPH, true>(R, k, slot, zslot);
+    pie_rows_first<NP, QN, ASP, QPH, true>(R, k, slot, zslot, ko, kp);
   if (!cluster_sync(R)) return false;
   if constexpr (ASP) {
     tf_cols<NP, CWR, true>(R, false);
@@ -2189,7 +2231,10 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if (FLK && R.fl)
     pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
   else
-    pie_rows_update<NP, QN, ASP, QPH, true>(R, k, slot, !sync_after);
+    if (!BAT && keep)
+      pie_rows_update<NP, QN, ASP, QPH, true, !BAT>(R, k, slot, !sync_after, ko, kp);
+    else
+      pie_rows_update<NP, QN, ASP, QPH, true>(R, k, slot, !sync_after, ko, kp);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
