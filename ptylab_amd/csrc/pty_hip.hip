@@ -234,14 +234,52 @@ __device__ __forceinline__ double block_sum(Run& R, double v) {
 __device__ __forceinline__ void atomic_fmax_slot(const Run& R, unsigned* slot, int w, float v) {
   __hip_atomic_fetch_max(slot + w * R.G + R.g, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// called by every lane of a wave; returns the maximum to all of them
+// called by every lane of a wave; the maxima of words w0 and w1 over the G
+// partials, returned to all of them.  Every load of both words is issued
+// before the first is waited for, so the pair costs one round trip for
+// G <= 256 (a lane past the last partial loads the last one again: the
+// maximum is the same).
+__device__ __forceinline__ void load_slot2(const Run& R, const unsigned* slot, int w0, int w1, float& m0,
+                                           float& m1) {
+  const unsigned* p0 = slot + w0 * R.G;
+  const unsigned* p1 = slot + w1 * R.G;
+  const int last = R.G - 1;
+  float a = 0.f, b = 0.f;
+  for (int base = 0; base < R.G; base += 256) {
+    unsigned va[4], vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i0 = base + 64 * u + R.lane;
+      const int i = i0 < last ? i0 : last;
+      va[u] = __hip_atomic_load(p0 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      vb[u] = __hip_atomic_load(p1 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a = fmaxf(a, __uint_as_float(va[u]));
+      b = fmaxf(b, __uint_as_float(vb[u]));
+    }
+  }
+  // uniform: kept in scalar registers
+  m0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(a))));
+  m1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(b))));
+}
+// one word, in the same way
 __device__ __forceinline__ float load_slot(const Run& R, const unsigned* slot, int w) {
   const unsigned* p = slot + w * R.G;
-  float m = 0.f;
-  for (int g = R.lane; g < R.G; g += 64)
-    m = fmaxf(m, __uint_as_float(__hip_atomic_load(p + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-  // uniform: kept in a scalar register
-  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(m))));
+  const int last = R.G - 1;
+  float a = 0.f;
+  for (int base = 0; base < R.G; base += 256) {
+    unsigned v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i0 = base + 64 * u + R.lane;
+      v[u] = __hip_atomic_load(p + (i0 < last ? i0 : last), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a = fmaxf(a, __uint_as_float(v[u]));
+  }
+  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(a))));
 }
 // clears this workgroup's partials of a ring slot
 __device__ __forceinline__ void clear_slot(const Run& R, unsigned* slot) {
@@ -336,22 +374,48 @@ __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
   return v;
 }
 
-// called by every lane of a wave; returns the maximum of the G partials of
-// word w that publish_max2 wrote (XCD-local or device-coherent loads)
-__device__ __forceinline__ float load_max_published(const Run& R, const unsigned* slot, int w) {
-  const unsigned* p = slot + w * R.G;
-  float m = 0.f;
+// partial i of a maxima word (XCD-local or device-coherent load)
+__device__ __forceinline__ unsigned load_partial(const Run& R, const unsigned* p, int i) {
   if (R.loc) {
     const LocalRef L = local_ref(p);
-    for (int i = R.lane; i < R.G; i += 64)
-      m = fmaxf(m, __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(L.r, L.off + (uint32_t)i * 4u, 0, LOC_AUX)));
-  } else {
-    for (int i = R.lane; i < R.G; i += 64)
-      m = fmaxf(m, __uint_as_float(__hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+    return __builtin_amdgcn_raw_buffer_load_b32(L.r, L.off + (uint32_t)i * 4u, 0, LOC_AUX);
   }
-  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(m))));
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-
+// called by every lane of a wave; the maxima of the G partials of words 0
+// and 1 that publish_max2 wrote, returned to all of them.  Every load of both
+// words is issued before the first is waited for, so the pair costs one
+// round trip for G <= 256 (a lane past the last partial loads the last one
+// again: the maximum is the same).
+__device__ __forceinline__ void load_max_published2(const Run& R, const unsigned* slot, float& m0, float& m1) {
+  const unsigned* p0 = slot;
+  const unsigned* p1 = slot + R.G;
+  const int last = R.G - 1;
+  float a = 0.f, b = 0.f;
+  if (R.G <= 64) {
+    const int i = R.lane < last ? R.lane : last;
+    a = __uint_as_float(load_partial(R, p0, i));
+    b = __uint_as_float(load_partial(R, p1, i));
+  } else {
+    for (int base = 0; base < R.G; base += 256) {
+      unsigned va[4], vb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i0 = base + 64 * u + R.lane;
+        const int i = i0 < last ? i0 : last;
+        va[u] = load_partial(R, p0, i);
+        vb[u] = load_partial(R, p1, i);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a = fmaxf(a, __uint_as_float(va[u]));
+        b = fmaxf(b, __uint_as_float(vb[u]));
+      }
+    }
+  }
+  m0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(a))));
+  m1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(b))));
+}
 // Arrival is two-level: a workgroup increments the counter of its XCD group
 // (blocks of one launch are dispatched round-robin, block b on XCD b % 8;
 // the grouping only spreads the atomics over separate cache lines, nothing
@@ -1267,7 +1331,8 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   const float2* __restrict__ Qp = D.Q;
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
-  const float Pmax = load_max_published(R, slot, 0), Omax = load_max_published(R, slot, 1);
+  float Pmax, Omax;
+  load_max_published2(R, slot, Pmax, Omax);
   const float bO = D.a.betaObject;
   const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
   // ePIE.py:128-160 weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
@@ -1520,7 +1585,8 @@ __device__ __forceinline__ void pie_rows_update_fl(Run& R, int k, const unsigned
   const float2* __restrict__ Qp = D.Q;
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
-  const float Pmax = load_max_published(R, slot, 0), Omax = load_max_published(R, slot, 1);
+  float Pmax, Omax;
+  load_max_published2(R, slot, Pmax, Omax);
   const float bO = D.a.betaObject;
   const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
   const float bP = engine == PTY_EPIE ? D.a.betaProbe : stepP;
@@ -2134,8 +2200,10 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
   const float2* __restrict__ E = D.ESW;
   const bool cb = D.a.propagator == PTY_FRAUNHOFER;
   const float invN = 1.0f / (float)NP;
-  const float invP = 1.0f / load_slot(R, slot, 2 * su);
-  const float invO = 1.0f / load_slot(R, slot, 2 * su + 1);
+  float mP, mO;
+  load_slot2(R, slot, 2 * su, 2 * su + 1, mP, mO);
+  const float invP = 1.0f / mP;
+  const float invO = 1.0f / mO;
   const float bO = D.a.betaObject;
   const float bP = su >= 1 ? D.a.sliceBeta : D.a.betaProbe;
   const int esl = fromDetector ? ns - 1 : su;
