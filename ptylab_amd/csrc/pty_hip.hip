@@ -195,7 +195,7 @@ struct Run {
   bool xbar;  // XCD barrier (cluster spread over the XCDs; xcd_barrier_setup done)
   unsigned xgen;  // XCD barrier: generation
   unsigned xcd, xpop, nx;  // thread 0: its XCD, the cluster's workgroups there, non-empty XCDs
-  const short* lcix;  // LDS copy of DevInst::cix (per-size kernels), or null
+  const short* lcix;  // LDS copy of DevInst::cix, or null
 };
 
 // ---------------------------------------------------------------------------
@@ -2515,6 +2515,24 @@ __device__ __forceinline__ void init_run(Run& R, const DevInst* D, int g, float2
   __syncthreads();
 }
 
+// The packed-row column codes of the workgroup's instance in LDS (when they
+// fit): the row passes look them up before every exchange-row access, and
+// from LDS that lookup does not add a memory round trip to the row's load.
+// families whose exchange rows can be packed (the angular-spectrum PIE ones)
+constexpr bool fam_packed(int fam) {
+  return fam == FAM_PIE_ASP || fam == FAM_PIE_SASP || fam == FAM_PC_ASP || fam == FAM_PC_SASP;
+}
+template <int NP, int NT, int CAP = PACK_LDS>
+__device__ __forceinline__ void cix_to_lds(Run& R, short* s_cix) {
+  const DevInst& D = *R.D;
+  const int n = D.cix ? (D.nF / D.fpl) * NP : 0;
+  if (n > 0 && n <= CAP) {
+    for (int i = threadIdx.x; i < n; i += NT) s_cix[i] = D.cix[i];
+    __syncthreads();
+    R.lcix = s_cix;
+  }
+}
+
 // One kernel per probe size and pass family (PIE with the Fraunhofer
 // detector, PIE with ASP, e3PIE): each instantiation carries only its own
 // passes and gets its own register allocation and workgroup size.
@@ -2533,17 +2551,7 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   if (bm.x < 0) return;
   Run R;
   init_run<NP, Cfg<NP>::NT, 0>(R, insts + bm.x, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
-  {
-    // the packed-row column codes in LDS: the row passes look them up
-    // before every exchange-row access
-    const DevInst& D = *R.D;
-    const int n = D.cix ? (D.nF / D.fpl) * NP : 0;
-    if (n > 0 && n <= PACK_LDS) {
-      for (int i = threadIdx.x; i < n; i += Cfg<NP>::NT) s_cix[i] = D.cix[i];
-      __syncthreads();
-      R.lcix = s_cix;
-    }
-  }
+  cix_to_lds<NP, Cfg<NP>::NT>(R, s_cix);
   if (R.G > 1 && !xcd_barrier_setup(R)) return;
   run_iterations<NP, FAM, 0>(R, it0, it1, jb, je);
 }
@@ -2587,6 +2595,8 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ float2 s_fx[(BATCH_NT / 64) * BATCH_FX];
   __shared__ int s_fail;
   __shared__ unsigned s_mx[3];
+  constexpr int CIX = fam_packed(FAM) ? PACK_LDS : 1;
+  __shared__ short s_cix[CIX];
   __shared__ int s_blk;
   // The plan packs every cluster onto one XCD: bmap entry N_XCD * s + x is
   // slot s of XCD x.  A workgroup reads the XCD it runs on (XCC_ID hardware
@@ -2618,30 +2628,35 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   switch (D->a.Np) {
     case 16:
       init_run<16, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+      cix_to_lds<16, BATCH_NT, CIX>(R, s_cix);
       R.loc = true;
       R.fl = D->fl && 16 / R.G >= R.ipr;
       run_iterations<16, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 32:
       init_run<32, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+      cix_to_lds<32, BATCH_NT, CIX>(R, s_cix);
       R.loc = true;
       R.fl = D->fl && 32 / R.G >= R.ipr;
       run_iterations<32, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 64:
       init_run<64, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+      cix_to_lds<64, BATCH_NT, CIX>(R, s_cix);
       R.loc = true;
       R.fl = D->fl && 64 / R.G >= R.ipr;
       run_iterations<64, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 128:
       init_run<128, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+      cix_to_lds<128, BATCH_NT, CIX>(R, s_cix);
       R.loc = true;
       R.fl = D->fl && 128 / R.G >= R.ipr;
       run_iterations<128, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
       break;
     case 256:
       init_run<256, BATCH_NT, BATCH_CW>(R, D, bm.y, s_tw, s_red, s_x, s_fx, &s_fail, s_mx);
+      cix_to_lds<256, BATCH_NT, CIX>(R, s_cix);
       R.loc = true;
       R.fl = D->fl && 256 / R.G >= R.ipr;
       run_iterations<256, FAM, BATCH_CW, true>(R, it0, it1, jb, je);
