@@ -1121,7 +1121,11 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
     const int pc0 = (cg0_ + base_) * CW;                               // first column of the round
     const int Cv = (cgn_ - base_ < gpr_ ? cgn_ - base_ : gpr_) * CW;  // its columns
     {
+      // every load of the round (its W block and its frame columns, at most
+      // CL elements each per thread) is issued before the first LDS write
+      // waits for one
       float2 w[CL];
+      float iv[CL];
 #pragma unroll
       for (int i = 0; i < CL; ++i) {
         const int e = i * R.nt + (int)threadIdx.x;
@@ -1132,12 +1136,21 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
 #pragma unroll
       for (int i = 0; i < CL; ++i) {
         const int e = i * R.nt + (int)threadIdx.x;
+        const int n = e >> lgC, c = e & (C - 1);
+        iv[i] = 0.f;
+        if (e < (NP << lgC)) iv[i] = ldg(Ik + (size_t)n * NP + pc0 + (c < Cv ? c : Cv - 1));
+      }
+#pragma unroll
+      for (int i = 0; i < CL; ++i) {
+        const int e = i * R.nt + (int)threadIdx.x;
         const int s = e >> lgNC, n = (e >> lgC) & (NP - 1), c = e & (C - 1);
         if (s < nF && c < Cv) tile[s * TS + c * cs + n] = w[i];
       }
-      for (int e = threadIdx.x; e < (NP << lgC); e += R.nt) {
+#pragma unroll
+      for (int i = 0; i < CL; ++i) {
+        const int e = i * R.nt + (int)threadIdx.x;
         const int n = e >> lgC, c = e & (C - 1);
-        ikt[c * csi + n] = ldg(Ik + (size_t)n * NP + pc0 + (c < Cv ? c : Cv - 1));
+        if (e < (NP << lgC)) ikt[c * csi + n] = iv[i];
       }
     }
     __syncthreads();
@@ -1742,7 +1755,31 @@ __device__ __forceinline__ void momentum_pass(Run& R) {
     const size_t chunk = (n + R.G - 1) / R.G;
     const size_t b = chunk * R.g;
     const size_t e = b + chunk < n ? b + chunk : n;
-    for (size_t i = b + threadIdx.x; i < e; i += R.nt) {
+    // MU elements per thread at a time, all their loads issued before the
+    // first store (a store can alias the next element's loads, so the
+    // compiler would otherwise wait out one round trip per element)
+    constexpr int MU = 4;
+    size_t i = b + threadIdx.x;
+    for (; i + (size_t)(MU - 1) * R.nt < e; i += (size_t)MU * R.nt) {
+      float2 x[MU], bb[MU], m[MU];
+#pragma unroll
+      for (int u = 0; u < MU; ++u) {
+        const size_t iu = i + (size_t)u * R.nt;
+        x[u] = cload(R, X + iu);
+        bb[u] = cload(R, B + iu);
+        m[u] = cload(R, M + iu);
+      }
+#pragma unroll
+      for (int u = 0; u < MU; ++u) {
+        const size_t iu = i + (size_t)u * R.nt;
+        const float2 mn = make_float2((bb[u].x - x[u].x) + m[u].x * eta, (bb[u].y - x[u].y) + m[u].y * eta);
+        const float2 xn = make_float2(x[u].x - mn.x * nu, x[u].y - mn.y * nu);
+        cstore(R, M + iu, mn);
+        cstore(R, X + iu, xn);
+        cstore(R, B + iu, xn);
+      }
+    }
+    for (; i < e; i += R.nt) {
       const float2 x = cload(R, X + i), bb = cload(R, B + i), m = cload(R, M + i);
       const float2 mn = make_float2((bb.x - x.x) + m.x * eta, (bb.y - x.y) + m.y * eta);
       const float2 xn = make_float2(x.x - mn.x * nu, x.y - mn.y * nu);
