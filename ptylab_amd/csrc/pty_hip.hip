@@ -416,6 +416,39 @@ __device__ __forceinline__ void load_max_published2(const Run& R, const unsigned
   m0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(a))));
   m1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(b))));
 }
+// load_max_published2 in two halves, for a pass that reads the maxima for a
+// later one: max_issue issues the loads (G <= 256: up to four partials per
+// lane and word), max_finish reduces them (a larger cluster reads them all
+// there)
+struct MaxLoads {
+  unsigned a[4], b[4];
+};
+__device__ __forceinline__ void max_issue(const Run& R, const unsigned* slot, MaxLoads& m) {
+  const int last = R.G - 1;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int i0 = 64 * u + R.lane;
+    const int i = i0 < last ? i0 : last;
+    m.a[u] = R.G <= 256 ? load_partial(R, slot, i) : 0u;
+    m.b[u] = R.G <= 256 ? load_partial(R, slot + R.G, i) : 0u;
+  }
+}
+__device__ __forceinline__ void max_finish(const Run& R, const unsigned* slot, const MaxLoads& m, float& m0,
+                                           float& m1) {
+  if (R.G > 256) {
+    load_max_published2(R, slot, m0, m1);
+    return;
+  }
+  float a = 0.f, b = 0.f;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    a = fmaxf(a, __uint_as_float(m.a[u]));
+    b = fmaxf(b, __uint_as_float(m.b[u]));
+  }
+  m0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(a))));
+  m1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(b))));
+}
+
 // Arrival is two-level: a workgroup increments the counter of its XCD group
 // (blocks of one launch are dispatched round-robin, block b on XCD b % 8;
 // the grouping only spreads the atomics over separate cache lines, nothing
@@ -1272,8 +1305,9 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
 // ASP detector row pass: inverse row transform gives the detector field, the
 // intensity constraint is applied in natural order, then the forward row
 // transform starts the back-propagation.
-template <int NP, bool PK>
-__device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep, float* de) {
+template <int NP, bool PK, bool MX = false>
+__device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep, float* de,
+                                                    const unsigned* slot = nullptr) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -1281,6 +1315,11 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
   const float* __restrict__ Ik = D.Ik + (size_t)k * Np2;
   const int nF = D.nF, fs = D.fs;
   const int ws = PK ? xrow_stride<NP>(D) : NP;
+  // MX: wave 0 reads the position's maxima here, two passes before the
+  // update pass needs them (they are final since the first barrier), with
+  // its first item's loads in flight, and leaves them in LDS (smx[3], smx[4])
+  MaxLoads mxl;
+  if (MX && R.wave == 0) max_issue(R, slot, mxl);
   double err = 0.0;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
     float2 v[L];
@@ -1316,6 +1355,14 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
     if (act) store_row_live<NP>(R, Wr, v, ix, R.t);
     ITEM_ROUND_SYNC();
   ITEMS_END
+  if (MX && R.wave == 0) {
+    float m0, m1;
+    max_finish(R, slot, mxl, m0, m1);
+    if (R.lane == 0) {
+      R.smx[3] = __float_as_uint(m0);
+      R.smx[4] = __float_as_uint(m1);
+    }
+  }
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
@@ -1327,7 +1374,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
 // and each object / probe field is written by one owner slot (the field with
 // psm index 0, resp. osm index 0).  Each object pixel of the patch has one
 // owner.
-template <int NP, bool QN, bool ASPF, bool QPH, bool PK, bool KEEP = false>
+template <int NP, bool QN, bool ASPF, bool QPH, bool PK, bool KEEP = false, bool MXL = false>
 __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* slot, bool sync_tail,
                                                 const float2 (&ko)[Line<NP>::L], const float2 (&kp)[Line<NP>::L]) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
@@ -1345,7 +1392,13 @@ __device__ __forceinline__ void pie_rows_update(Run& R, int k, const unsigned* s
   const float invN = 1.0f / (float)NP;
   const int engine = D.a.engine;
   float Pmax, Omax;
-  load_max_published2(R, slot, Pmax, Omax);
+  if (MXL) {
+    // left in LDS by the constraint pass of this position
+    Pmax = __uint_as_float(R.smx[3]);
+    Omax = __uint_as_float(R.smx[4]);
+  } else {
+    load_max_published2(R, slot, Pmax, Omax);
+  }
   const float bO = D.a.betaObject;
   const float stepP = engine == PTY_MPIE ? D.a.probeWeight * D.a.betaProbe : D.a.betaProbe;
   // ePIE.py:128-160 weights conj(P)/max(sum|P|^2), conj(O)/max(sum|O|^2);
@@ -2325,7 +2378,7 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if constexpr (ASP) {
     tf_cols<NP, CWR, true>(R, false);
     if (!cluster_sync(R)) return false;
-    asp_rows_constraint<NP, true>(R, k, j, ep, de);
+    asp_rows_constraint<NP, true, !BAT>(R, k, j, ep, de, slot);
     if (!cluster_sync(R)) return false;
     tf_cols<NP, CWR, true>(R, true);
     if (!cluster_sync(R)) return false;
@@ -2337,9 +2390,9 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
   else
     if (!BAT && keep)
-      pie_rows_update<NP, QN, ASP, QPH, true, !BAT>(R, k, slot, !sync_after, ko, kp);
+      pie_rows_update<NP, QN, ASP, QPH, true, !BAT, ASP && !BAT>(R, k, slot, !sync_after, ko, kp);
     else
-      pie_rows_update<NP, QN, ASP, QPH, true>(R, k, slot, !sync_after, ko, kp);
+      pie_rows_update<NP, QN, ASP, QPH, true, false, ASP && !BAT>(R, k, slot, !sync_after, ko, kp);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
@@ -2582,7 +2635,7 @@ __global__ __launch_bounds__(Cfg<NP>::NT) void pty_iterate_kernel(const DevInst*
   __shared__ float2 s_x[Cfg<NP>::XBUF];
   __shared__ float2 s_fx[Cfg<NP>::NW * wave_scratch<NP, 0>()];
   __shared__ int s_fail;
-  __shared__ unsigned s_mx[3];
+  __shared__ unsigned s_mx[5];
   __shared__ short s_cix[PACK_LDS];
   const int2 bm = bmap[blockIdx.x];
   if (bm.x < 0) return;
@@ -2631,7 +2684,7 @@ __global__ __launch_bounds__(BATCH_NT) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ float2 s_x[BATCH_XBUF];
   __shared__ float2 s_fx[(BATCH_NT / 64) * BATCH_FX];
   __shared__ int s_fail;
-  __shared__ unsigned s_mx[3];
+  __shared__ unsigned s_mx[5];
   constexpr int CIX = fam_packed(FAM) ? PACK_LDS : 1;
   __shared__ short s_cix[CIX];
   __shared__ int s_blk;
