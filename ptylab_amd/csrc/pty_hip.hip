@@ -1126,8 +1126,9 @@ __device__ __forceinline__ void detector_cols_fraunhofer_direct(Run& R, int k, i
 // (XSTG of the kernel configurations), too much for NP = 512.
 template <int NP>
 constexpr bool col_staged() { return NP <= 256; }
-template <int NP, int CWR>
-__device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, int j, double* ep, int fs, float* de) {
+template <int NP, int CWR, bool MX = false>
+__device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, int j, double* ep, int fs, float* de,
+                                                                const unsigned* slot) {
   constexpr int CW = Col<NP, CWR>::CW, CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -1149,6 +1150,10 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
   float2* tile = R.xa;
   float* ikt = (float*)(R.xa + nFp * TS);
   float* xs = ikt + C * csi;
+  // MX: wave 0 reads the position's maxima for the update pass here (as in
+  // asp_rows_constraint)
+  MaxLoads mxl;
+  if (MX && R.wave == 0) max_issue(R, slot, mxl);
   double err = 0.0;
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
     const int pc0 = (cg0_ + base_) * CW;                               // first column of the round
@@ -1238,14 +1243,25 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
     }
     CITEM_ROUND_SYNC();
   CITEMS_END
+  if (MX && R.wave == 0) {
+    float m0, m1;
+    max_finish(R, slot, mxl, m0, m1);
+    if (R.lane == 0) {
+      R.smx[3] = __float_as_uint(m0);
+      R.smx[4] = __float_as_uint(m1);
+    }
+  }
   const double s = block_sum(R, err);
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
 
-template <int NP, int CWR>
-__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de) {
+// MX: also read the position's maxima into LDS for the update pass (the
+// staged form; the direct form leaves them to the update pass, MX false)
+template <int NP, int CWR, bool MX = false>
+__device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de,
+                                                         const unsigned* slot = nullptr) {
   if constexpr (col_staged<NP>())
-    detector_cols_fraunhofer_staged<NP, CWR>(R, k, j, ep, fs, de);
+    detector_cols_fraunhofer_staged<NP, CWR, MX>(R, k, j, ep, fs, de, slot);
   else
     detector_cols_fraunhofer_direct<NP, CWR>(R, k, j, ep, fs, de);
 }
@@ -2363,6 +2379,13 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   // the Fraunhofer family spilled more registers and lost 6 % on one-mode
   // instances, which have no field sums to save
   constexpr bool FLK = BAT && ASP && !QN && !PC;
+  // the pass before the update reads the maxima into LDS: the constraint
+  // pass of the per-size angular-spectrum kernels, the staged detector
+  // column pass of the per-size Fraunhofer ones (in the batched kernel the
+  // registers this holds through the pass cost more in spills than the
+  // round trip it saves)
+  constexpr bool MXF = !ASP && !BAT;
+  constexpr bool MXU = ASP ? !BAT : MXF && col_staged<NP>();
   // per-size kernels whose waves own one (line, field) item per row pass
   // keep the item's patch and probe rows in registers from the first row
   // pass to the last (the batched kernel has no registers to spare)
@@ -2383,16 +2406,16 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
     tf_cols<NP, CWR, true>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
-    detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, D.fs, de);
+    detector_cols_fraunhofer<NP, CWR, MXF && col_staged<NP>()>(R, k, j, ep, D.fs, de, slot);
     if (!cluster_sync(R)) return false;
   }
   if (FLK && R.fl)
     pie_rows_update_fl<NP, ASP, QPH>(R, k, slot);
   else
     if (!BAT && keep)
-      pie_rows_update<NP, QN, ASP, QPH, true, !BAT, ASP && !BAT>(R, k, slot, !sync_after, ko, kp);
+      pie_rows_update<NP, QN, ASP, QPH, true, !BAT, MXU>(R, k, slot, !sync_after, ko, kp);
     else
-      pie_rows_update<NP, QN, ASP, QPH, true, false, ASP && !BAT>(R, k, slot, !sync_after, ko, kp);
+      pie_rows_update<NP, QN, ASP, QPH, true, false, MXU>(R, k, slot, !sync_after, ko, kp);
   if (sync_after && !cluster_sync(R)) return false;
   if constexpr (PC) {
     if (it >= D.a.pc_start) {
