@@ -1278,9 +1278,14 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
 // band-limited function lie scattered in the bit-reversed column order, and
 // an equal share of them per workgroup keeps the slowest workgroup, which
 // the barrier after the pass waits for, from carrying several.
-template <int NP, int CWR, bool PK>
-__device__ __forceinline__ void tf_cols(Run& R, bool conj) {
+// MX: wave 0 also reads words 0, 1 of the maxima slot mslot for the pass
+// after this one and leaves them in LDS (smx[3], smx[4]), as the detector
+// column passes do.
+template <int NP, int CWR, bool PK, bool MX = false>
+__device__ __forceinline__ void tf_cols(Run& R, bool conj, const unsigned* mslot = nullptr) {
   constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC, NG = Col<NP, CWR>::NG;
+  MaxLoads mxl;
+  if (MX && R.wave == 0) max_issue(R, mslot, mxl);
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
   float2* __restrict__ W = D.W;
@@ -1315,6 +1320,14 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj) {
     }
     fft_inv_colg<NP, CWR, true>(R, v, rho);
     store_colg<NP, CWR>(R, Wf, v, pw, rho, ws);
+  }
+  if (MX && R.wave == 0) {
+    float m0, m1;
+    max_finish(R, mslot, mxl, m0, m1);
+    if (R.lane == 0) {
+      R.smx[3] = __float_as_uint(m0);
+      R.smx[4] = __float_as_uint(m1);
+    }
   }
 }
 
@@ -2292,7 +2305,7 @@ __device__ __forceinline__ void e3_rows_forward(Run& R, int k, int s, unsigned* 
 // the last exit wave) or a slice back-propagation (difference to esw_su).
 // Object slice su and probe slice su are rewritten; for su >= 1 the updated
 // probe slice is forward-transformed into W for the next back-propagation.
-template <int NP>
+template <int NP, bool MXL = false>
 __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsigned* slot, bool fromDetector) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
   const DevInst& D = *R.D;
@@ -2307,7 +2320,13 @@ __device__ __forceinline__ void e3_rows_update(Run& R, int k, int su, const unsi
   const bool cb = D.a.propagator == PTY_FRAUNHOFER;
   const float invN = 1.0f / (float)NP;
   float mP, mO;
-  load_slot2(R, slot, 2 * su, 2 * su + 1, mP, mO);
+  if (MXL) {
+    // words 2 su, 2 su + 1, left in LDS by the column pass before this one
+    mP = __uint_as_float(R.smx[3]);
+    mO = __uint_as_float(R.smx[4]);
+  } else {
+    load_slot2(R, slot, 2 * su, 2 * su + 1, mP, mO);
+  }
   const float invP = 1.0f / mP;
   const float invO = 1.0f / mO;
   const float bO = D.a.betaObject;
@@ -2451,9 +2470,14 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   return true;
 }
 
-template <int NP, int CWR>
+template <int NP, int CWR, bool BAT>
 __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* slot, unsigned* zslot, double* ep,
                                                float* de) {
+  // per-size kernels up to Np = 128: every update pass takes its slice's
+  // maxima from LDS, read by the column pass before it (final since the
+  // forward chain); the Np = 256 kernel already spills, and the registers
+  // held through the column pass cost it more than the round trip saved
+  constexpr bool MX = !BAT && NP <= 128;
   const DevInst& D = *R.D;
   const int ns = D.a.nslice;
   e3_rows_first<NP>(R, k, slot, zslot);
@@ -2464,14 +2488,14 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
     e3_rows_forward<NP>(R, k, s, slot);
     if (!cluster_sync(R)) return false;
   }
-  detector_cols_fraunhofer<NP, CWR>(R, k, j, ep, 1, de);
+  detector_cols_fraunhofer<NP, CWR, MX>(R, k, j, ep, 1, de, slot + (size_t)2 * (ns - 1) * R.G);
   if (!cluster_sync(R)) return false;
-  e3_rows_update<NP>(R, k, ns - 1, slot, true);
+  e3_rows_update<NP, MX>(R, k, ns - 1, slot, true);
   if (!cluster_sync(R)) return false;
   for (int s = ns - 1; s >= 1; --s) {
-    tf_cols<NP, CWR, false>(R, true);
+    tf_cols<NP, CWR, false, MX>(R, true, slot + (size_t)2 * (s - 1) * R.G);
     if (!cluster_sync(R)) return false;
-    e3_rows_update<NP>(R, k, s - 1, slot, false);
+    e3_rows_update<NP, MX>(R, k, s - 1, slot, false);
     if (!cluster_sync(R)) return false;
   }
   return true;
@@ -2538,7 +2562,7 @@ __device__ __forceinline__ void run_iterations(Run& R, int it0, int it1, int jb,
                       : nullptr;
       bool ok;
       if constexpr (BF == FAM_E3PIE)
-        ok = e3pie_position<NP, CWR>(R, k, j, slot, zslot, ep, de);
+        ok = e3pie_position<NP, CWR, BAT>(R, k, j, slot, zslot, ep, de);
       else
         ok = pie_position<NP, BF == FAM_PIE_ASP || BF == FAM_QN_ASP || BF == FAM_PIE_SASP, CWR,
                           BF == FAM_QN_FRAUNHOFER || BF == FAM_QN_ASP, BF == FAM_PIE_SASP, PC, BAT>(
