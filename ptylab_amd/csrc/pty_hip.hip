@@ -420,28 +420,31 @@ __device__ __forceinline__ void load_max_published2(const Run& R, const unsigned
 // later one: max_issue issues the loads (G <= 256: up to four partials per
 // lane and word), max_finish reduces them (a larger cluster reads them all
 // there)
+template <int MU = 4>
 struct MaxLoads {
-  unsigned a[4], b[4];
+  unsigned a[MU], b[MU];
 };
-__device__ __forceinline__ void max_issue(const Run& R, const unsigned* slot, MaxLoads& m) {
+template <int MU>
+__device__ __forceinline__ void max_issue(const Run& R, const unsigned* slot, MaxLoads<MU>& m) {
   const int last = R.G - 1;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < MU; ++u) {
     const int i0 = 64 * u + R.lane;
     const int i = i0 < last ? i0 : last;
-    m.a[u] = R.G <= 256 ? load_partial(R, slot, i) : 0u;
-    m.b[u] = R.G <= 256 ? load_partial(R, slot + R.G, i) : 0u;
+    m.a[u] = R.G <= 64 * MU ? load_partial(R, slot, i) : 0u;
+    m.b[u] = R.G <= 64 * MU ? load_partial(R, slot + R.G, i) : 0u;
   }
 }
-__device__ __forceinline__ void max_finish(const Run& R, const unsigned* slot, const MaxLoads& m, float& m0,
+template <int MU>
+__device__ __forceinline__ void max_finish(const Run& R, const unsigned* slot, const MaxLoads<MU>& m, float& m0,
                                            float& m1) {
-  if (R.G > 256) {
+  if (R.G > 64 * MU) {
     load_max_published2(R, slot, m0, m1);
     return;
   }
   float a = 0.f, b = 0.f;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < MU; ++u) {
     a = fmaxf(a, __uint_as_float(m.a[u]));
     b = fmaxf(b, __uint_as_float(m.b[u]));
   }
@@ -1126,7 +1129,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_direct(Run& R, int k, i
 // (XSTG of the kernel configurations), too much for NP = 512.
 template <int NP>
 constexpr bool col_staged() { return NP <= 256; }
-template <int NP, int CWR, bool MX = false>
+template <int NP, int CWR, int MX = 0>
 __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, int j, double* ep, int fs, float* de,
                                                                 const unsigned* slot) {
   constexpr int CW = Col<NP, CWR>::CW, CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC;
@@ -1152,7 +1155,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
   float* xs = ikt + C * csi;
   // MX: wave 0 reads the position's maxima for the update pass here (as in
   // asp_rows_constraint)
-  MaxLoads mxl;
+  MaxLoads<MX ? MX : 1> mxl;
   if (MX && R.wave == 0) max_issue(R, slot, mxl);
   double err = 0.0;
   CITEMS_BEGIN(R, NP, p, f, act, islot, lslot, frw)
@@ -1257,7 +1260,7 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
 
 // MX: also read the position's maxima into LDS for the update pass (the
 // staged form; the direct form leaves them to the update pass, MX false)
-template <int NP, int CWR, bool MX = false>
+template <int NP, int CWR, int MX = 0>
 __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de,
                                                          const unsigned* slot = nullptr) {
   if constexpr (col_staged<NP>())
@@ -1281,10 +1284,10 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
 // MX: wave 0 also reads words 0, 1 of the maxima slot mslot for the pass
 // after this one and leaves them in LDS (smx[3], smx[4]), as the detector
 // column passes do.
-template <int NP, int CWR, bool PK, bool MX = false>
+template <int NP, int CWR, bool PK, int MX = 0>
 __device__ __forceinline__ void tf_cols(Run& R, bool conj, const unsigned* mslot = nullptr) {
   constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC, NG = Col<NP, CWR>::NG;
-  MaxLoads mxl;
+  MaxLoads<MX ? MX : 1> mxl;
   if (MX && R.wave == 0) max_issue(R, mslot, mxl);
   const DevInst& D = *R.D;
   const size_t Np2 = (size_t)NP * NP;
@@ -1334,7 +1337,7 @@ __device__ __forceinline__ void tf_cols(Run& R, bool conj, const unsigned* mslot
 // ASP detector row pass: inverse row transform gives the detector field, the
 // intensity constraint is applied in natural order, then the forward row
 // transform starts the back-propagation.
-template <int NP, bool PK, bool MX = false>
+template <int NP, bool PK, int MX = 0>
 __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double* ep, float* de,
                                                     const unsigned* slot = nullptr) {
   constexpr int L = Line<NP>::L, LPF = Line<NP>::LPF;
@@ -1347,7 +1350,7 @@ __device__ __forceinline__ void asp_rows_constraint(Run& R, int k, int j, double
   // MX: wave 0 reads the position's maxima here, two passes before the
   // update pass needs them (they are final since the first barrier), with
   // its first item's loads in flight, and leaves them in LDS (smx[3], smx[4])
-  MaxLoads mxl;
+  MaxLoads<MX ? MX : 1> mxl;
   if (MX && R.wave == 0) max_issue(R, slot, mxl);
   double err = 0.0;
   ITEMS_BEGIN(R, NP, r, f, act, islot, lslot, frw)
@@ -2420,12 +2423,12 @@ __device__ __forceinline__ bool pie_position(Run& R, int k, int j, unsigned* slo
   if constexpr (ASP) {
     tf_cols<NP, CWR, true>(R, false);
     if (!cluster_sync(R)) return false;
-    asp_rows_constraint<NP, true, !BAT>(R, k, j, ep, de, slot);
+    asp_rows_constraint<NP, true, BAT ? 0 : 4>(R, k, j, ep, de, slot);
     if (!cluster_sync(R)) return false;
     tf_cols<NP, CWR, true>(R, true);
     if (!cluster_sync(R)) return false;
   } else {
-    detector_cols_fraunhofer<NP, CWR, MXF && col_staged<NP>()>(R, k, j, ep, D.fs, de, slot);
+    detector_cols_fraunhofer<NP, CWR, MXF && col_staged<NP>() ? (BAT ? 1 : 4) : 0>(R, k, j, ep, D.fs, de, slot);
     if (!cluster_sync(R)) return false;
   }
   if (FLK && R.fl)
@@ -2488,12 +2491,12 @@ __device__ __forceinline__ bool e3pie_position(Run& R, int k, int j, unsigned* s
     e3_rows_forward<NP>(R, k, s, slot);
     if (!cluster_sync(R)) return false;
   }
-  detector_cols_fraunhofer<NP, CWR, MX>(R, k, j, ep, 1, de, slot + (size_t)2 * (ns - 1) * R.G);
+  detector_cols_fraunhofer<NP, CWR, MX ? 4 : 0>(R, k, j, ep, 1, de, slot + (size_t)2 * (ns - 1) * R.G);
   if (!cluster_sync(R)) return false;
   e3_rows_update<NP, MX>(R, k, ns - 1, slot, true);
   if (!cluster_sync(R)) return false;
   for (int s = ns - 1; s >= 1; --s) {
-    tf_cols<NP, CWR, false, MX>(R, true, slot + (size_t)2 * (s - 1) * R.G);
+    tf_cols<NP, CWR, false, MX ? 4 : 0>(R, true, slot + (size_t)2 * (s - 1) * R.G);
     if (!cluster_sync(R)) return false;
     e3_rows_update<NP, MX>(R, k, s - 1, slot, false);
     if (!cluster_sync(R)) return false;
