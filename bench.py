@@ -42,7 +42,9 @@ BATCH_EPIE_TRAFFIC_FILE = ROOT / "profiles" / "pty_batch_kernel_epie_fraunhofer_
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    # 40 timed iterations: about a second of GPU time for the default
+    # workload, long enough for an outside utilisation sampler to see it
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-positions", type=int, default=3000,
