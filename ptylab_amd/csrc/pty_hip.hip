@@ -417,8 +417,8 @@ __device__ __forceinline__ void load_max_published2(const Run& R, const unsigned
   m1 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(wave_max(b))));
 }
 // load_max_published2 in two halves, for a pass that reads the maxima for a
-// later one: max_issue issues the loads (G <= 256: up to four partials per
-// lane and word), max_finish reduces them (a larger cluster reads them all
+// later one: max_issue issues the loads (MU partials per lane and word, for
+// G <= 64 MU), max_finish reduces them (a larger cluster reads them all
 // there)
 template <int MU = 4>
 struct MaxLoads {
@@ -1258,8 +1258,9 @@ __device__ __forceinline__ void detector_cols_fraunhofer_staged(Run& R, int k, i
   if (threadIdx.x == 0) cstored(R, ep + (size_t)j * R.G + R.g, s);
 }
 
-// MX: also read the position's maxima into LDS for the update pass (the
-// staged form; the direct form leaves them to the update pass, MX false)
+// MX > 0: also read the position's maxima into LDS for the update pass, MX
+// partial loads per lane and word (the staged form; the direct form leaves
+// them to the update pass, MX = 0)
 template <int NP, int CWR, int MX = 0>
 __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, double* ep, int fs, float* de,
                                                          const unsigned* slot = nullptr) {
@@ -1281,9 +1282,9 @@ __device__ __forceinline__ void detector_cols_fraunhofer(Run& R, int k, int j, d
 // band-limited function lie scattered in the bit-reversed column order, and
 // an equal share of them per workgroup keeps the slowest workgroup, which
 // the barrier after the pass waits for, from carrying several.
-// MX: wave 0 also reads words 0, 1 of the maxima slot mslot for the pass
-// after this one and leaves them in LDS (smx[3], smx[4]), as the detector
-// column passes do.
+// MX > 0: wave 0 also reads words 0, 1 of the maxima slot mslot for the pass
+// after this one (MX partial loads per lane and word) and leaves them in LDS
+// (smx[3], smx[4]), as the detector column passes do.
 template <int NP, int CWR, bool PK, int MX = 0>
 __device__ __forceinline__ void tf_cols(Run& R, bool conj, const unsigned* mslot = nullptr) {
   constexpr int CL = Col<NP, CWR>::CL, LPC = Col<NP, CWR>::LPC, NG = Col<NP, CWR>::NG;
